@@ -1,0 +1,264 @@
+/*
+ * nnpp.h — C ABI of the MI355X-native NNPP (nonlocal Nernst–Planck–Poisson) time step.
+ *
+ * Drop-in boundary for the step path of alhermann/nnpp-mg-implant-corrosion-3d. Every entry
+ * point below names the reference interface it replaces (paths relative to the reference
+ * repository). The reference's interface is a set of header-only C++ templates over Eigen
+ * containers called from main(); here it becomes plain C: host pointers + sizes, integer
+ * status codes (0 = NNPP_OK), no exceptions and no C++/torch types across the boundary.
+ *
+ * Ownership: host arrays belong to the caller and are only accessed during the call.
+ * Device buffers belong to the context. Threading: one context per host thread; all work is
+ * issued on the context's own HIP stream and every call blocks until it is complete.
+ *
+ * Field numbering: a problem has S species (the reference hard-codes S = 10: Mg2+, H+, OH-,
+ * HCO3-, CO3 2-, HPO4 2-, PO4 3-, Ca2+, Na+, Cl-; main.cpp:526-529) plus the potential phi,
+ * i.e. F = S + 1 fields; field f = 0..S-1 is species f+1 of the reference's 1-based naming,
+ * and field S is phi (the reference's U11). Per-field host arrays are packed field-major:
+ * array[f * n_nodes + i].
+ */
+#ifndef NNPP_H
+#define NNPP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NNPP_MAX_SPECIES 16
+#define NNPP_DATA2_LEN 51   /* utils.h:2348-2398 reads 51 values from <job>.data2 */
+#define NNPP_DATAC_LEN 7    /* utils.h:2400-2478: z, DSolid, DLiquid, s1, s2, Kprop1, Kprop2 */
+#define NNPP_DATAC_PHI_LEN 4 /* utils.h:2480-2483: sigma0, sigma1, s111, K11prop1 */
+
+/* Status codes. */
+enum {
+  NNPP_OK = 0,
+  NNPP_ERR_ARG = 1,          /* bad argument / size */
+  NNPP_ERR_OVERFLOW = 2,     /* a neighbourhood does not fit max_len (grid.h:187-189 throws) */
+  NNPP_ERR_NONFINITE = 3,    /* NaN/Inf detected in the state after a step */
+  NNPP_ERR_NOCONV = 4,       /* potential solve did not reach its tolerance within maxit */
+  NNPP_ERR_HIP = 5,          /* HIP runtime error (message in nnpp_last_error) */
+  NNPP_ERR_STATE = 6,        /* call order violated (e.g. step before neighbours exist) */
+  NNPP_ERR_UNSUPPORTED = 7,  /* parameter combination this build does not implement */
+  NNPP_ERR_IO = 8            /* input file missing or short (readers only) */
+};
+
+/* Reaction model. */
+enum {
+  NNPP_CHEM_GENERIC = 0,     /* no bulk reactions; optional first-order Mg sink r_diss at
+                                interface solid nodes (no reference counterpart) */
+  NNPP_CHEM_REFERENCE = 1    /* the reference's R1..R8 terms, utils.h:162-174,241-253,...;
+                                requires n_species == 10 */
+};
+
+/* Arithmetic of the step. */
+enum {
+  NNPP_FP64 = 0,             /* every operation in FP64 */
+  NNPP_MIXED = 1             /* FP32 bond sums with compensated per-node accumulation, FP32
+                                differences taken from FP64 state; FP64 CG dots, residual
+                                checks and state update */
+};
+
+/* Lower clamp applied to a species after each update (main.cpp:805-837). */
+enum { NNPP_CLAMP_NONE = 0, NNPP_CLAMP_ALWAYS = 1, NNPP_CLAMP_LIQUID = 2 };
+
+/*
+ * Problem parameters; mirrors utils::parameter (utils.h:2186-2485). Fields in the .data2 block
+ * keep the reference's order and names (pKH2O is the reference's pKH20). Species arrays hold
+ * .datac1 .. .datac<S>; sigma0 .. K_phi hold .datac<S+1>. Fields after "extensions" have no
+ * counterpart in the reference files; nnpp_params_defaults() sets them to the reference's
+ * hard-coded behaviour.
+ */
+typedef struct nnpp_params {
+  int32_t n_nodes;     /* .data1[0] nodesNo */
+  int32_t n_steps;     /* .data1[1] stepsNo */
+  int32_t n_species;   /* S, 1..NNPP_MAX_SPECIES (reference: 10) */
+  int32_t chemistry;   /* NNPP_CHEM_* */
+
+  /* .data2 (51 values, utils.h:2348-2398) */
+  double t0, t1, dt, dx, dy, dz, delta, CSolid, CSat, C0Na, C0Ca, C0HCO3, C0HPO4, C0Cl;
+  double pKMgOH2, pKMgCO3, pKMgHCO3, pKMgPO4, pKCaCO3, pKCa5OHPO4, pKH2O, pKHCO3, pKHPO4;
+  double phim, phimse, beta, alphaA, alphaC;
+  double k1f, k1b, k2f, k2b, k3f, k3b, k4f, k4b, k5f, k5b, k6f, k6b, k7f, k7b, k8f, k8b, k9f, k9b;
+  double JHp, T, R, F, vol_red;
+
+  /* .datac1 .. .datac<S> (7 values each, utils.h:2400-2478) */
+  double z[NNPP_MAX_SPECIES];
+  double D_solid[NNPP_MAX_SPECIES];
+  double D_liquid[NNPP_MAX_SPECIES];
+  double s1[NNPP_MAX_SPECIES];   /* diffusion kernel exponent: 1/xi^(3+2 s1) */
+  double s2[NNPP_MAX_SPECIES];   /* migration kernel exponent: 1/xi^(3+2 s2) */
+  double K1[NNPP_MAX_SPECIES];   /* Kprop1 */
+  double K2[NNPP_MAX_SPECIES];   /* Kprop2 */
+
+  /* .datac<S+1> (4 values, utils.h:2480-2483) */
+  double sigma0, sigma1, s_phi, K_phi;
+
+  /* extensions */
+  int32_t solid_bonds[NNPP_MAX_SPECIES]; /* 1: species diffuses over solid-solid bonds and its
+                                            solid-liquid bonds are scaled by BV (reference:
+                                            Mg only, utils.h:100-159); 0: solid-solid bonds
+                                            are skipped (utils.h:211-213) */
+  int32_t clamp[NNPP_MAX_SPECIES];       /* NNPP_CLAMP_* per species (main.cpp:805-837) */
+  double clamp_floor;       /* reference: 1e-8 / CSolid */
+  double front_threshold;   /* solid -> liquid when U_Mg <= this; reference CSat / CSolid */
+  double phase_eps;         /* icvalues IC: solid iff U_Mg > phase_eps; reference EPS = 1e-5 */
+  double r_diss;            /* NNPP_CHEM_GENERIC interface sink rate (1/time) */
+  int32_t sub_steps;        /* voln1rel refresh cadence; reference sub_step_no = 8 */
+  int32_t reserved;
+} nnpp_params;
+
+/* Step statistics (accumulated over the steps of one nnpp_step call). */
+typedef struct nnpp_step_stats {
+  int64_t steps;            /* steps taken */
+  int64_t cg_iterations;    /* total CG iterations (inner iterations in mixed mode) */
+  int64_t cg_spmv_fp64;     /* FP64 Poisson operator applications */
+  int64_t cg_spmv_fp32;     /* FP32 Poisson operator applications (mixed mode) */
+  double last_relres;       /* final relative residual of the last solve (FP64 check) */
+  int64_t flipped;          /* solid -> liquid flips */
+  int64_t solid_count;      /* solid nodes after the last step */
+  double voln1rel;          /* current relative Mg volume feeding BV */
+  int64_t nnz_flux;         /* directed bonds per flux evaluation */
+  int64_t nnz_ll;           /* liquid-liquid directed bonds in the last Poisson operator */
+} nnpp_step_stats;
+
+typedef struct nnpp_ctx nnpp_ctx;
+
+/* ---------------------------------------------------------------- host-only helpers ----- */
+
+/* Fill *p with the reference's hard-coded behaviour for S species: solid_bonds = {1,0,...},
+ * clamp = ALWAYS for species 1..9 and LIQUID for species 10 (main.cpp:805-837),
+ * clamp_floor = 1e-8/CSolid, front_threshold = CSat/CSolid, phase_eps = 1e-5,
+ * sub_steps = 8. Call after the numeric fields are set (it reads CSolid and CSat). */
+int nnpp_params_defaults(nnpp_params* p, int32_t n_species, int32_t chemistry);
+
+/* Read <job>.data1, <job>.data2, <job>.datac1 .. <job>.datac<S+1> into *p and apply
+ * nnpp_params_defaults. Replaces main.cpp:210-231 + utils::parameter::set_int/set_double
+ * (utils.h:2334-2484). Unlike the reference, a missing file is an error (NNPP_ERR_IO). */
+int nnpp_read_params(const char* job, int32_t n_species, int32_t chemistry, nnpp_params* p);
+
+/* Whitespace-separated readers; replace utils::read_vector_from_file and
+ * utils::read_matrix_from_file (utils.h:1851-1887). nodes: n lines "x y z" (main.cpp:234). */
+int nnpp_read_vector(const char* path, int64_t n, double* out);
+int nnpp_read_ivector(const char* path, int64_t n, int32_t* out);
+int nnpp_read_nodes(const char* path, int64_t n, double* x, double* y, double* z);
+
+/* ELL adapter: turns a CSR neighbour list (rows sorted by id, no self) into the reference's
+ * layout, an n x max_len column-major int matrix with self at column 0, neighbours in
+ * ascending id after it and -1 padding, plus lengths that count self (grid.h:145-149,170-195,
+ * main.cpp:34,138). Returns NNPP_ERR_OVERFLOW if a row does not fit. */
+int nnpp_neighbors_to_ell(int32_t n, const int64_t* row_ptr, const int32_t* col,
+                          int32_t max_len, int32_t* ell, int32_t* lengths);
+
+/* ------------------------------------------------------------------- context ------------- */
+
+/* Create a context on HIP device `device`. Replaces nothing (the reference has no device). */
+int nnpp_create(nnpp_ctx** ctx, int32_t device);
+int nnpp_destroy(nnpp_ctx* ctx);
+const char* nnpp_last_error(const nnpp_ctx* ctx);
+
+/* Set the parameters; replaces utils::parameter (utils.h:2186-2485). */
+int nnpp_set_params(nnpp_ctx* ctx, const nnpp_params* p);
+
+/* Upload node coordinates (SoA) and optional per-node volumes (NULL: dx*dy*dz for every node,
+ * as utils.h:122-124). Replaces the nodes matrix read at main.cpp:234. */
+int nnpp_set_nodes(nnpp_ctx* ctx, int32_t n, const double* x, const double* y, const double* z,
+                   const double* volumes);
+
+/* Spatial-hash neighbour construction on the GPU for the nodes set by nnpp_set_nodes.
+ * Replaces build_clouds_hash(max_node_no, r, dx, dy, dz, nodes, clouds, cloud_lengths,
+ * max_cloud_length) (grid.h:43-204; called at main.cpp:304-305 with r = delta*(1+1e-5) and
+ * cells 10*dx x 10*dy x 10*dz). Same predicate, same cell logic; the result is kept on the
+ * device as CSR (rows sorted by neighbour id, self excluded). *nnz_out = directed bonds,
+ * *max_len_out = longest row INCLUDING self (the reference's max_cloud_length).
+ * NNPP_ERR_OVERFLOW when a row (with self) would exceed max_len, as grid.h:187-189. */
+int nnpp_build_neighbors(nnpp_ctx* ctx, double r, double cell_dx, double cell_dy,
+                         double cell_dz, int32_t max_len, int64_t* nnz_out,
+                         int32_t* max_len_out);
+
+/* Copy the device CSR out (row_ptr: n+1, col: nnz). */
+int nnpp_get_neighbors(nnpp_ctx* ctx, int64_t* row_ptr, int32_t* col);
+
+/* Upload a CSR built elsewhere (rows sorted by id, no self) instead of building it. */
+int nnpp_set_neighbors(nnpp_ctx* ctx, const int64_t* row_ptr, const int32_t* col);
+
+/* Boundary conditions: bctypes (n ints; 0 = Dirichlet node) and the F per-field value arrays
+ * packed field-major (F*n doubles), raw as read from <job>.bcvalues1..F. A node/field is
+ * Dirichlet iff bctypes==0 and value != -1; species values are divided by CSolid, phi is
+ * used raw (main.cpp:571-625,676-711). */
+int nnpp_set_bc(nnpp_ctx* ctx, const int32_t* bctypes, const double* bcvalues);
+
+/* State slots. */
+enum { NNPP_SLOT_N = 0,  /* U^n: the state the explicit step advances */
+       NNPP_SLOT_N1 = 1  /* U^{n+1}: the Newton iterate used by nnpp_residuals */ };
+
+/* Upload / download all F fields of a slot (F*n doubles, field-major). */
+int nnpp_set_state(nnpp_ctx* ctx, int32_t slot, const double* U);
+int nnpp_get_state(nnpp_ctx* ctx, int32_t slot, double* U);
+
+/* Node phase, 1 = solid, 0 = liquid (main.cpp:413-457). */
+int nnpp_set_phase(nnpp_ctx* ctx, const int32_t* phase);
+int nnpp_get_phase(nnpp_ctx* ctx, int32_t* phase);
+
+/* Initial state from icvalues (F*n, field-major), main.cpp:400-421: U^n = icvalues, phase =
+ * (U_Mg > phase_eps), then Dirichlet values are written into every field (the reference
+ * writes them into the first Newton iterate, main.cpp:556-640). Sets solid_count, vol0rel =
+ * sum over solid nodes of (U_Mg - CSat/CSolid) (main.cpp:474-479) and voln1rel = 0, and
+ * resets the step counter to s0 = 1. */
+int nnpp_init_state(nnpp_ctx* ctx, const double* icvalues);
+
+/* Set the relative volume and step counter explicitly (restart, main.cpp:506-542). */
+int nnpp_set_time(nnpp_ctx* ctx, int64_t step, double voln1rel, double vol0rel);
+
+/* Residuals F_k of every field for the iterate U^{n+1} (slot N1) and the previous state
+ * (slot N), exactly as utils::Residuals (utils.h:27-842): bond sums of the iterate, lagged
+ * potential, reactions, (U1-Un)/dt, phi residual on rows with a liquid-liquid bond.
+ * apply_bc != 0 also applies main.cpp:676-711 (Dirichlet rows: F = U1 - bc).
+ * F_out: F*n doubles, field-major. precision: NNPP_FP64 or NNPP_MIXED. */
+int nnpp_residuals(nnpp_ctx* ctx, double voln1rel, int32_t apply_bc, int32_t precision,
+                   double* F_out);
+
+/* Poisson operator (K11 of utils.h:1794-1848 without its identity rows): y_i =
+ * sum over liquid-liquid bonds of w_ij (x_i - x_j) for rows with such a bond, y_i = 0
+ * elsewhere; w_ij = Beta(xi) V_j / xi^(3+2 s_phi). Uses the phase on the device. */
+int nnpp_poisson_apply(nnpp_ctx* ctx, const double* x, double* y);
+
+/* Solver controls for the potential (replaces Eigen GMRES at main.cpp:764-779):
+ * relative residual rtol (default 1e-10), maxit (default 20000); for NNPP_MIXED the inner
+ * FP32 solve reduces the residual by inner_rtol per refinement pass (default 1e-4). */
+int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol);
+
+/* Solve the potential for the charge of slot N: phi^{n+1} on free nodes (liquid, with a
+ * liquid-liquid bond, not Dirichlet), Dirichlet phi = bc, other nodes keep their value.
+ * Warm-starts from the phi in slot N and writes the result there. */
+int nnpp_poisson_solve(nnpp_ctx* ctx, int32_t precision, int32_t* iters, double* relres);
+
+/* Advance nsteps explicit steps (the reference's time loop main.cpp:547-1025 with the
+ * implicit Newton/GMRES solve replaced by one explicit update):
+ *   1. species: U^{n+1} = U^n - dt * G(U^n), G = utils::Residuals without (U1-Un)/dt and
+ *      with U^{n+1} := U^n, then Dirichlet rows, then the clamp (main.cpp:805-837);
+ *   2. phi^{n+1} from the charge of U^n (utils.h:831-840) by CG, warm-started;
+ *   3. front update: solid -> liquid where U_Mg^{n+1} <= front_threshold (main.cpp:915-922);
+ *   4. every sub_steps steps: voln1rel = sum_solid (U_Mg - CSat/CSolid) / vol0rel
+ *      (main.cpp:944-949), which feeds BV = 10^(-vol_red (1 - voln1rel)).
+ * stats may be NULL. Returns NNPP_ERR_NOCONV if a solve fails (state is still advanced),
+ * NNPP_ERR_NONFINITE if check_finite != 0 and the state holds NaN/Inf. */
+int nnpp_step(nnpp_ctx* ctx, int32_t nsteps, int32_t precision, int32_t check_finite,
+              nnpp_step_stats* stats);
+
+/* Front update alone (main.cpp:915-922) on slot N. */
+int nnpp_front_update(nnpp_ctx* ctx, double threshold, int32_t* n_flipped);
+
+/* Raw device pointer of a field in a slot (for zero-copy interop; valid until the context
+ * is destroyed or nnpp_set_nodes is called again). */
+int nnpp_state_device_ptr(nnpp_ctx* ctx, int32_t slot, int32_t field, void** dptr);
+
+/* HIP stream the context issues its work on (hipStream_t as void*). */
+int nnpp_stream(nnpp_ctx* ctx, void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNPP_H */

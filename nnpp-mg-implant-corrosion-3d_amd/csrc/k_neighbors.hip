@@ -1,0 +1,341 @@
+// k_neighbors.hip — spatial-hash neighbour construction (replaces build_clouds_hash,
+// grid.h:43-204) and the per-row sort that turns it into id-sorted CSR.
+//
+// Bit-exactness: the hash (grid.h:84), the cell corners (grid.h:134-139), the sphere-cube test
+// (grid.h:27-41) and the candidate predicate sqr(dx)+sqr(dy)+sqr(dz) <= sqr(r) (grid.h:186) are
+// evaluated with the same FP64 operations in the same order and without FMA contraction (this
+// file is compiled with -ffp-contract=off and the pragma below). The reference tests every
+// cell against the sphere (grid.h:163); here a node tests only the cells of the index box one
+// cell wider than its sphere's bounding box, which contains every cell the test can accept.
+#pragma clang fp contract(off)
+
+#include <climits>
+#include <vector>
+
+#include "nnpp_internal.hpp"
+
+namespace {
+
+__device__ __forceinline__ double sqr_(double v) { return v * v; }
+
+// grid.h:27-41
+__device__ __forceinline__ bool sphere_cube(double cx, double cy, double cz, double cr,
+                                            double bx, double by, double bz, double w, double h,
+                                            double d) {
+  const double bmin[3] = {bx, by, bz};
+  const double bmax[3] = {bx + w, by + h, bz + d};
+  const double c[3] = {cx, cy, cz};
+  const double r2 = sqr_(cr);
+  double dmin = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (c[a] < bmin[a]) dmin += sqr_(c[a] - bmin[a]);
+    else if (c[a] > bmax[a]) dmin += sqr_(c[a] - bmax[a]);
+  }
+  return dmin <= r2;
+}
+
+struct Grid {
+  double x0, y0, z0, cdx, cdy, cdz, r;
+  int mx, my, mz_last, max_hash;
+};
+
+// grid.h:82-85
+__global__ void k_hash(int32_t n, const double* __restrict__ x, const double* __restrict__ y,
+                       const double* __restrict__ z, Grid g, int* __restrict__ hash,
+                       int* __restrict__ max_hash) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int h = (int)((x[i] - g.x0) / g.cdx) + g.mx * (int)((y[i] - g.y0) / g.cdy) +
+                g.mx * g.my * (int)((z[i] - g.z0) / g.cdz);
+  hash[i] = h;
+  atomicMax(max_hash, h);
+}
+
+__global__ void k_cell_count(int32_t n, const int* __restrict__ hash, int* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(&cnt[hash[i]], 1);
+}
+
+__global__ void k_cell_scatter(int32_t n, const double* __restrict__ x,
+                               const double* __restrict__ y, const double* __restrict__ z,
+                               const int* __restrict__ hash, int* __restrict__ cursor,
+                               double* __restrict__ cx, double* __restrict__ cy,
+                               double* __restrict__ cz, int32_t* __restrict__ cid) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int pos = atomicAdd(&cursor[hash[i]], 1);
+  cx[pos] = x[i];
+  cy[pos] = y[i];
+  cz[pos] = z[i];
+  cid[pos] = (int32_t)i;
+}
+
+// Candidate scan of one node; FILL == false counts, FILL == true writes the hits.
+template <bool FILL>
+__global__ __launch_bounds__(NNPP_BLOCK) void k_scan(
+    int32_t n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, Grid g, const int* __restrict__ cstart,
+    const double* __restrict__ cx, const double* __restrict__ cy, const double* __restrict__ cz,
+    const int32_t* __restrict__ cid, int32_t* __restrict__ counts,
+    const int64_t* __restrict__ row_ptr, int32_t* __restrict__ col, int32_t max_len) {
+  const int64_t i = nnpp_xcd_block(blockIdx.x, gridDim.x) * NNPP_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x[i], yi = y[i], zi = z[i];
+  const double r2 = sqr_(g.r);
+  int ixl = (int)floor((xi - g.r - g.x0) / g.cdx) - 1, ixh = (int)floor((xi + g.r - g.x0) / g.cdx) + 1;
+  int iyl = (int)floor((yi - g.r - g.y0) / g.cdy) - 1, iyh = (int)floor((yi + g.r - g.y0) / g.cdy) + 1;
+  int izl = (int)floor((zi - g.r - g.z0) / g.cdz) - 1, izh = (int)floor((zi + g.r - g.z0) / g.cdz) + 1;
+  ixl = max(ixl, 0);
+  iyl = max(iyl, 0);
+  izl = max(izl, 0);
+  ixh = min(ixh, g.mx - 1);
+  iyh = min(iyh, g.my - 1);
+  izh = min(izh, g.mz_last);
+  const int mxy = g.mx * g.my;
+  int32_t cnt = 0;
+  const int64_t out = FILL ? row_ptr[i] : 0;
+  const int64_t row_end = FILL ? row_ptr[i + 1] : 0;
+  for (int iz = izl; iz <= izh; iz++)
+    for (int iy = iyl; iy <= iyh; iy++)
+      for (int ix = ixl; ix <= ixh; ix++) {
+        const int m = ix + g.mx * iy + mxy * iz;
+        if (m >= g.max_hash) continue;
+        const int rr = (m % mxy) % g.mx, cc = (m % mxy) / g.mx, pp = m / mxy;
+        const double bx = g.x0 + rr * g.cdx, by = g.y0 + cc * g.cdy, bz = g.z0 + pp * g.cdz;
+        if (!sphere_cube(xi, yi, zi, g.r, bx, by, bz, g.cdx, g.cdy, g.cdz)) continue;
+        const int k1 = cstart[m + 1];
+        for (int k = cstart[m]; k < k1; k++) {
+          if (sqr_(xi - cx[k]) + sqr_(yi - cy[k]) + sqr_(zi - cz[k]) <= r2) {
+            const int32_t cand = cid[k];
+            if (cand != i) {
+              if (FILL) {
+                if (out + cnt < row_end) col[out + cnt] = cand;
+              }
+              cnt++;
+            }
+          }
+        }
+      }
+  if (!FILL) counts[i] = cnt;
+}
+
+// One wavefront per row: bitonic sort of the row in LDS (P = power of two >= longest row).
+__global__ __launch_bounds__(NNPP_BLOCK) void k_sort_rows(int32_t n, int P,
+                                                          const int64_t* __restrict__ row_ptr,
+                                                          int32_t* __restrict__ col) {
+  extern __shared__ int32_t s_rows[];
+  const int wid = threadIdx.x / NNPP_WAVE, lane = threadIdx.x % NNPP_WAVE;
+  const int64_t row = (int64_t)blockIdx.x * (NNPP_BLOCK / NNPP_WAVE) + wid;
+  int32_t* s = s_rows + wid * P;
+  int64_t b0 = 0;
+  int len = 0;
+  if (row < n) {
+    b0 = row_ptr[row];
+    len = (int)(row_ptr[row + 1] - b0);
+  }
+  for (int t = lane; t < P; t += NNPP_WAVE) s[t] = (t < len) ? col[b0 + t] : INT_MAX;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = lane; t < P; t += NNPP_WAVE) {
+        const int u = t ^ j;
+        if (u > t) {
+          const int32_t a = s[t], b = s[u];
+          const bool up = ((t & k) == 0);
+          if ((a > b) == up) {
+            s[t] = b;
+            s[u] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int t = lane; t < len; t += NNPP_WAVE) col[b0 + t] = s[t];
+}
+
+__global__ void k_rowptr_from_counts(int32_t n, const int64_t* __restrict__ excl,
+                                     int64_t* __restrict__ row_ptr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) row_ptr[i] = excl[i];
+}
+
+}  // namespace
+
+int nnpp_device_exclusive_scan_i32(nnpp_ctx* c, const int32_t* in, int64_t n, int64_t* out);
+
+template <typename T>
+static int dev_alloc(nnpp_ctx* c, T** p, size_t count) {
+  NNPP_HIP(c, hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
+  return NNPP_OK;
+}
+
+int nnpp_build_neighbors_dev(nnpp_ctx* c, double r, double cdx, double cdy, double cdz,
+                             int32_t max_len, int64_t* nnz_out, int32_t* max_len_out) {
+  const int32_t n = c->n;
+  if (!(cdx > 0) || !(cdy > 0) || !(cdz > 0) || max_len < 1 || !(r >= 0))
+    return nnpp_set_error(c, NNPP_ERR_ARG, "build_neighbors: bad radius/cell size/max_len");
+  hipStream_t s = c->stream;
+  if (!c->row_ptr) {
+    int st = dev_alloc(c, &c->row_ptr, (size_t)n + 1);
+    if (st) return st;
+  }
+  if (n == 0) {
+    NNPP_HIP(c, hipMemsetAsync(c->row_ptr, 0, sizeof(int64_t), s));
+    NNPP_HIP(c, hipStreamSynchronize(s));
+    c->nnz = 0;
+    c->max_len = 0;
+    if (nnz_out) *nnz_out = 0;
+    if (max_len_out) *max_len_out = 0;
+    c->geom_dirty = c->pot_dirty = true;
+    return NNPP_OK;
+  }
+  // Step 0 (grid.h:59-71): bounding box of the coordinates; min/max are order independent,
+  // so the host copies kept at upload time give the same values.
+  std::vector<double> hx(n), hy(n), hz(n);
+  NNPP_HIP(c, hipMemcpyAsync(hx.data(), c->x, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(hy.data(), c->y, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(hz.data(), c->z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  double x0 = hx[0], x1 = hx[0], y0 = hy[0], y1 = hy[0], z0 = hz[0], z1 = hz[0];
+  for (int32_t i = 0; i < n; i++) {
+    x0 = hx[i] < x0 ? hx[i] : x0;
+    x1 = x1 < hx[i] ? hx[i] : x1;
+    y0 = hy[i] < y0 ? hy[i] : y0;
+    y1 = y1 < hy[i] ? hy[i] : y1;
+    z0 = hz[i] < z0 ? hz[i] : z0;
+    z1 = z1 < hz[i] ? hz[i] : z1;
+  }
+  Grid g;
+  g.x0 = x0;
+  g.y0 = y0;
+  g.z0 = z0;
+  g.cdx = cdx;
+  g.cdy = cdy;
+  g.cdz = cdz;
+  g.r = r;
+  g.mx = (int)((x1 - x0) / cdx) + 1;  // grid.h:78-79
+  g.my = (int)((y1 - y0) / cdy) + 1;
+  const int64_t nb = nnpp_nblocks(n);
+
+  int *hash = nullptr, *dmax = nullptr, *cell = nullptr, *cursor = nullptr;
+  double *cx = nullptr, *cy = nullptr, *cz = nullptr;
+  int32_t *cid = nullptr, *counts = nullptr;
+  int64_t* excl = nullptr;
+  int st = NNPP_OK;
+#define NB_TRY(expr)           \
+  do {                         \
+    st = (expr);               \
+    if (st) goto cleanup;      \
+  } while (0)
+#define NB_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      st = nnpp_set_error(c, NNPP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+      goto cleanup;                                                                    \
+    }                                                                                  \
+  } while (0)
+  {
+    NB_TRY(dev_alloc(c, &hash, n));
+    NB_TRY(dev_alloc(c, &dmax, 1));
+    NB_HIP(hipMemsetAsync(dmax, 0, sizeof(int), s));
+    k_hash<<<nb, NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, g, hash, dmax);
+    NB_HIP(hipGetLastError());
+    int hmax = 0;
+    NB_HIP(hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, s));
+    NB_HIP(hipStreamSynchronize(s));
+    g.max_hash = hmax + 1;  // grid.h:95
+    g.mz_last = (g.max_hash - 1) / (g.mx * g.my);
+    NB_TRY(dev_alloc(c, &cell, (size_t)g.max_hash + 1));
+    NB_TRY(dev_alloc(c, &cursor, (size_t)g.max_hash + 1));
+    NB_HIP(hipMemsetAsync(cell, 0, sizeof(int) * ((size_t)g.max_hash + 1), s));
+    k_cell_count<<<nb, NNPP_BLOCK, 0, s>>>(n, hash, cell);
+    NB_HIP(hipGetLastError());
+    {
+      std::vector<int> hc((size_t)g.max_hash + 1);
+      NB_HIP(hipMemcpyAsync(hc.data(), cell, sizeof(int) * g.max_hash, hipMemcpyDeviceToHost, s));
+      NB_HIP(hipStreamSynchronize(s));
+      int acc = 0;
+      for (int m = 0; m < g.max_hash; m++) {
+        const int v = hc[m];
+        hc[m] = acc;
+        acc += v;
+      }
+      hc[g.max_hash] = acc;
+      NB_HIP(hipMemcpyAsync(cell, hc.data(), sizeof(int) * ((size_t)g.max_hash + 1),
+                            hipMemcpyHostToDevice, s));
+      NB_HIP(hipMemcpyAsync(cursor, hc.data(), sizeof(int) * ((size_t)g.max_hash + 1),
+                            hipMemcpyHostToDevice, s));
+      NB_HIP(hipStreamSynchronize(s));  // hc leaves scope
+    }
+    NB_TRY(dev_alloc(c, &cx, n));
+    NB_TRY(dev_alloc(c, &cy, n));
+    NB_TRY(dev_alloc(c, &cz, n));
+    NB_TRY(dev_alloc(c, &cid, n));
+    k_cell_scatter<<<nb, NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, hash, cursor, cx, cy, cz, cid);
+    NB_HIP(hipGetLastError());
+    NB_TRY(dev_alloc(c, &counts, n));
+    k_scan<false><<<nb, NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, g, cell, cx, cy, cz, cid, counts,
+                                            nullptr, nullptr, max_len);
+    NB_HIP(hipGetLastError());
+    NB_TRY(dev_alloc(c, &excl, (size_t)n + 1));
+    NB_TRY(nnpp_device_exclusive_scan_i32(c, counts, n, excl));
+    int64_t nnz = 0;
+    NB_HIP(hipMemcpyAsync(&nnz, excl + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    // longest row (self excluded) for the overflow test and the sort width
+    std::vector<int32_t> hcnt(n);
+    NB_HIP(hipMemcpyAsync(hcnt.data(), counts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    NB_HIP(hipStreamSynchronize(s));
+    int32_t longest = 0;
+    for (int32_t i = 0; i < n; i++) longest = hcnt[i] > longest ? hcnt[i] : longest;
+    if (longest >= max_len) {  // grid.h:187-189
+      st = nnpp_set_error(c, NNPP_ERR_OVERFLOW,
+                          "Not enough room to store clouds. Increase max_node_no.");
+      goto cleanup;
+    }
+    k_rowptr_from_counts<<<nnpp_nblocks((int64_t)n + 1), NNPP_BLOCK, 0, s>>>(n, excl, c->row_ptr);
+    NB_HIP(hipGetLastError());
+    if (nnz > c->col_cap) {
+      if (c->col) NB_HIP(hipFree(c->col));
+      c->col = nullptr;
+      NB_TRY(dev_alloc(c, &c->col, (size_t)nnz));
+      c->col_cap = nnz;
+    }
+    k_scan<true><<<nb, NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, g, cell, cx, cy, cz, cid, nullptr,
+                                           c->row_ptr, c->col, max_len);
+    NB_HIP(hipGetLastError());
+    int P = 1;
+    while (P < longest) P <<= 1;
+    if (P > 4096) {
+      st = nnpp_set_error(c, NNPP_ERR_UNSUPPORTED, "build_neighbors: rows longer than 4096");
+      goto cleanup;
+    }
+    const int rows_per_block = NNPP_BLOCK / NNPP_WAVE;
+    k_sort_rows<<<(n + rows_per_block - 1) / rows_per_block, NNPP_BLOCK,
+                  sizeof(int32_t) * P * rows_per_block, s>>>(n, P, c->row_ptr, c->col);
+    NB_HIP(hipGetLastError());
+    NB_HIP(hipStreamSynchronize(s));
+    c->nnz = nnz;
+    c->max_len = longest + 1;
+    if (nnz_out) *nnz_out = nnz;
+    if (max_len_out) *max_len_out = longest + 1;
+    c->geom_dirty = c->pot_dirty = true;
+  }
+cleanup:
+  hipStreamSynchronize(s);
+  if (hash) hipFree(hash);
+  if (dmax) hipFree(dmax);
+  if (cell) hipFree(cell);
+  if (cursor) hipFree(cursor);
+  if (cx) hipFree(cx);
+  if (cy) hipFree(cy);
+  if (cz) hipFree(cz);
+  if (cid) hipFree(cid);
+  if (counts) hipFree(counts);
+  if (excl) hipFree(excl);
+#undef NB_TRY
+#undef NB_HIP
+  return st;
+}

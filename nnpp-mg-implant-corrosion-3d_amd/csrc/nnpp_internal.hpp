@@ -1,0 +1,224 @@
+// nnpp_internal.hpp — context layout and shared device helpers of libnnpp (gfx950).
+//
+// HBM layout (all arrays node-indexed in the caller's node order):
+//   x, y, z                 FP64 SoA coordinates
+//   vol                     FP64 per-node volume (optional)
+//   row_ptr/col             CSR neighbour list (API-facing, sorted by id, self excluded)
+//   bond slices             "sliced ELL" with a slice height of one wavefront (64 rows):
+//                           slice c holds rows 64c..64c+63; entry t of lane l sits at
+//                           off[c] + 64 t + l, so the 64 lanes of a wave read 256 contiguous
+//                           bytes of column ids per bond step. Padding entries hold -1.
+//   bond weights            Beta(xi) V_j / xi^(3+2s) per slice entry, FP64 and FP32
+//   U[slot]                 F = S+1 fields, field-major FP64 (U[f*n + i])
+//   phase, bcmask, bcval    uint8 phase, uint16 per-node Dirichlet field mask, FP64 values
+//   potential slices        same slice format restricted to free-free liquid bonds, weights
+//                           pre-scaled by the row's volume ratio; diagonal separately
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/nnpp.h"
+
+#define NNPP_WAVE 64
+#define NNPP_BLOCK 256
+#define NNPP_NXCD 8
+
+struct nnpp_slices {
+  int64_t nslice = 0;      // number of 64-row slices
+  int64_t slots = 0;       // total entries (incl. padding)
+  int64_t nnz = 0;         // real entries
+  int64_t* off = nullptr;  // [nslice + 1]
+  int32_t* len = nullptr;  // [nslice] entries per lane
+  int32_t* col = nullptr;  // [slots]
+  double* w64 = nullptr;   // [slots]
+  float* w32 = nullptr;    // [slots]
+  int64_t cap_slots = 0;
+};
+
+struct nnpp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  nnpp_params p{};
+  bool have_params = false;
+  int32_t n = 0;
+  double* x = nullptr;
+  double* y = nullptr;
+  double* z = nullptr;
+  double* vol = nullptr;
+
+  // CSR
+  int64_t nnz = -1;
+  int64_t* row_ptr = nullptr;
+  int32_t* col = nullptr;
+  int64_t col_cap = 0;
+  int32_t max_len = 0;
+
+  nnpp_slices bonds;       // every bond, weights with the flux exponent
+  nnpp_slices pot;         // free-free liquid bonds of the potential operator
+  bool geom_dirty = true;  // bond slices need a rebuild
+  bool pot_dirty = true;   // potential operator needs a rebuild (phase/bc changed)
+  double w_exponent = 3.0; // 3 + 2 s shared by every field
+
+  uint8_t* phase = nullptr;
+  uint16_t* bcmask = nullptr;
+  double* bcval = nullptr;  // F*n, species already divided by CSolid
+  bool have_bc = false;
+
+  double* U[2] = {nullptr, nullptr};  // slots N, N1
+  double* Unew = nullptr;             // F*n scratch of the explicit step
+  double* Fres = nullptr;             // F*n residual output
+
+  // potential solve
+  uint8_t* pfree = nullptr;
+  double* pdiag = nullptr;   // scaled diagonal (FP64)
+  double* pbdir = nullptr;   // Dirichlet part of the right-hand side (unscaled)
+  double* psc = nullptr;     // row scale V_i / (dx dy dz)
+  double* cg_b = nullptr;
+  double* cg_x = nullptr;
+  double* cg_r = nullptr;
+  double* cg_z = nullptr;
+  double* cg_p = nullptr;
+  double* cg_q = nullptr;
+  float* f_e = nullptr;      // FP32 inner-solve vectors (mixed mode)
+  float* f_r = nullptr;
+  float* f_z = nullptr;
+  float* f_p = nullptr;
+  float* f_q = nullptr;
+  float* pdiag32 = nullptr;
+  int64_t pot_nfree = 0;
+
+  // reductions: partials + ticket + scalars; pinned host mirror of the scalars
+  double* red_part = nullptr;   // [nblk_max * 4]
+  unsigned* red_ticket = nullptr;
+  double* scal = nullptr;       // device scalars (see enum CgScalar)
+  double* h_scal = nullptr;     // pinned host copy
+
+  double rtol = 1e-10;
+  int32_t maxit = 20000;
+  double inner_rtol = 1e-4;
+
+  int64_t step = 1;
+  double voln1rel = 0.0;
+  double vol0rel = 0.0;
+  int64_t solid_count = 0;
+};
+
+// Device scalars used by the CG kernels and reductions.
+enum CgScalar {
+  SC_RHO = 0, SC_ALPHA, SC_BETA, SC_PQ, SC_RR, SC_TARGET2, SC_DONE, SC_ITERS, SC_BNORM2,
+  SC_R0, SC_R1, SC_R2, SC_R3, SC_COUNT
+};
+
+int nnpp_set_error(nnpp_ctx* c, int code, const std::string& msg);
+
+#define NNPP_HIP(ctx, call)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return nnpp_set_error((ctx), NNPP_ERR_HIP,                                         \
+                            std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+// ---------------------------------------------------------------- device helpers -------
+
+// Blocks are handed to the 8 XCDs round-robin; remap so each XCD walks a contiguous range of
+// node blocks (neighbour data then stays in that XCD's L2). Bijective for any block count.
+__device__ __forceinline__ int64_t nnpp_xcd_block(int64_t b, int64_t nb) {
+  const int64_t q = nb / NNPP_NXCD, r = nb % NNPP_NXCD;
+  const int64_t xcd = b % NNPP_NXCD, k = b / NNPP_NXCD;
+  return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// utils.h:17-25
+__device__ __forceinline__ double nnpp_beta(double xi, double delta, double dx) {
+  if (xi <= delta - dx / 2.0) return 1.0;
+  if (xi <= delta + dx / 2.0) return (delta + dx / 2.0 - xi) / dx;
+  return 0.0;
+}
+
+__device__ __forceinline__ double nnpp_wave_sum(double v) {
+  for (int o = NNPP_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, NNPP_WAVE);
+  return v;
+}
+
+// Block-wide sum of K values (blockDim.x == NNPP_BLOCK); result valid in thread 0.
+template <int K>
+__device__ __forceinline__ void nnpp_block_sum(double (&v)[K], double* lds /*[K*4]*/) {
+  const int lane = threadIdx.x % NNPP_WAVE, wid = threadIdx.x / NNPP_WAVE;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = nnpp_wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) lds[k * (NNPP_BLOCK / NNPP_WAVE) + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double s = 0.0;
+      for (int w = 0; w < NNPP_BLOCK / NNPP_WAVE; w++) s += lds[k * (NNPP_BLOCK / NNPP_WAVE) + w];
+      v[k] = s;
+    }
+}
+
+__device__ __forceinline__ double nnpp_ld_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Deterministic grid reduction: each block writes K partials; the last block to arrive sums
+// them in block order and returns true (in thread 0) with the totals in v. The ticket resets
+// itself, so no memset is needed between launches.
+template <int K>
+__device__ __forceinline__ bool nnpp_grid_sum(double (&v)[K], double* part, unsigned* ticket,
+                                              int64_t nblk, int64_t bid) {
+  __shared__ double lds[K * (NNPP_BLOCK / NNPP_WAVE)];
+  __shared__ bool last;
+  nnpp_block_sum<K>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      __hip_atomic_store(&part[bid * K + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    last = (t == (unsigned)(nblk - 1));
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  // the last block sums the partials; lanes stride over blocks, then a fixed-order tree
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  for (int64_t b = threadIdx.x; b < nblk; b += NNPP_BLOCK)
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] += nnpp_ld_coherent(&part[b * K + k]);
+  nnpp_block_sum<K>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = acc[k];
+    *ticket = 0u;
+  }
+  return threadIdx.x == 0;
+}
+
+static inline int64_t nnpp_nblocks(int64_t n) { return (n + NNPP_BLOCK - 1) / NNPP_BLOCK; }
+
+// ---------------------------------------------------------------- launchers (host) -----
+int nnpp_build_neighbors_dev(nnpp_ctx* c, double r, double cdx, double cdy, double cdz,
+                             int32_t max_len, int64_t* nnz_out, int32_t* max_len_out);
+int nnpp_build_bond_slices(nnpp_ctx* c);
+int nnpp_build_potential(nnpp_ctx* c);
+int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
+                     const double* Un, double* out);
+int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, double* phi,
+                             int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32);
+int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out);
+int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flips,
+                   double* volsum);
+int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo);
+int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
+int nnpp_ensure_reduce(nnpp_ctx* c, int64_t nblk);

@@ -1,0 +1,629 @@
+/*
+ * nnpp_oracle.c — CPU FP64 restatement of the reference's NNPP step path.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline as the checker; the product library never uses it.
+ *
+ * Each function names the reference lines it restates (paths relative to the reference
+ * repository). Expressions keep the reference's operand order so that FP64 results agree to
+ * the last few ulps; summation over a node's bonds runs in ascending neighbour id (the
+ * reference runs in its cell order, grid.h:163-194), which is the only intended difference.
+ * Build with -ffp-contract=off (the reference is built without FMA, CMakeLists.txt:52).
+ */
+#include "nnpp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static inline double sqr_(double v) { return v * v; } /* grid.h:22-24 */
+
+/* utils.h:17-25 */
+double or_beta(double xi, double delta, double dx) {
+  if (xi <= delta - dx / 2.0) {
+    return 1.0;
+  } else if ((xi > delta - dx / 2.0) && (xi <= delta + dx / 2.0)) {
+    return (delta + dx / 2.0 - xi) / (dx);
+  } else {
+    return 0.0;
+  }
+}
+
+/* grid.h:27-41 */
+int or_sphere_cube_intersects(double cx, double cy, double cz, double cr, double rx, double ry,
+                              double rz, double rw, double rh, double rd) {
+  const double bmin[3] = {rx, ry, rz};
+  const double bmax[3] = {rx + rw, ry + rh, rz + rd};
+  const double c[3] = {cx, cy, cz};
+  const double r2 = sqr_(cr);
+  double dmin = 0.0;
+  for (int a = 0; a < 3; a++) {
+    if (c[a] < bmin[a]) dmin += sqr_(c[a] - bmin[a]);
+    else if (c[a] > bmax[a]) dmin += sqr_(c[a] - bmax[a]);
+  }
+  return dmin <= r2;
+}
+
+static int cmp_i32(const void* a, const void* b) {
+  int32_t u = *(const int32_t*)a, v = *(const int32_t*)b;
+  return (u > v) - (u < v);
+}
+
+/*
+ * grid.h:43-204 (build_clouds_hash). Steps 0-4 are restated exactly: bounding box, hash
+ * ix + mx*iy + mx*my*iz with truncating casts, cell index, cell corners x0 + r*dx. Step 5
+ * applies the same sphere-cube test and the same candidate predicate; instead of testing every
+ * cell (grid.h:163) it tests the cells of the index box one cell wider than the sphere's
+ * bounding box, which contains every cell the test can accept (a cell outside it is at least
+ * one cell width beyond the sphere). Within a row the accepted ids are sorted ascending.
+ * col == NULL: fill row_ptr (and max_len_out) only.
+ */
+int or_build_neighbors(int32_t n, const double* x, const double* y, const double* z, double r,
+                       double cdx, double cdy, double cdz, int32_t max_len, int64_t* row_ptr,
+                       int32_t* col, int32_t* max_len_out) {
+  if (n < 0 || max_len < 1 || !(cdx > 0) || !(cdy > 0) || !(cdz > 0)) return NNPP_ERR_ARG;
+  row_ptr[0] = 0;
+  if (max_len_out) *max_len_out = 0;
+  if (n == 0) return NNPP_OK;
+
+  double x0 = x[0], x1 = x[0], y0 = y[0], y1 = y[0], z0 = z[0], z1 = z[0];
+  for (int32_t i = 0; i < n; i++) { /* grid.h:59-71 */
+    x0 = x[i] < x0 ? x[i] : x0;
+    x1 = x1 < x[i] ? x[i] : x1;
+    y0 = y[i] < y0 ? y[i] : y0;
+    y1 = y1 < y[i] ? y[i] : y1;
+    z0 = z[i] < z0 ? z[i] : z0;
+    z1 = z1 < z[i] ? z[i] : z1;
+  }
+  const int mx = (int)((x1 - x0) / cdx) + 1; /* grid.h:78-79 */
+  const int my = (int)((y1 - y0) / cdy) + 1;
+  int* hash = (int*)malloc(sizeof(int) * (size_t)n);
+  int max_hash = 0;
+  for (int32_t i = 0; i < n; i++) { /* grid.h:82-85 */
+    hash[i] = (int)((x[i] - x0) / cdx) + mx * (int)((y[i] - y0) / cdy) +
+              mx * my * (int)((z[i] - z0) / cdz);
+    if (hash[i] + 1 > max_hash) max_hash = hash[i] + 1;
+  }
+  /* grid.h:90-124: nodes grouped by cell (counting sort; order inside a cell is irrelevant
+   * once rows are sorted by id). */
+  int64_t* start = (int64_t*)calloc((size_t)max_hash + 1, sizeof(int64_t));
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  for (int32_t i = 0; i < n; i++) start[hash[i] + 1]++;
+  for (int m = 0; m < max_hash; m++) start[m + 1] += start[m];
+  {
+    int64_t* fillp = (int64_t*)malloc(sizeof(int64_t) * (size_t)max_hash);
+    memcpy(fillp, start, sizeof(int64_t) * (size_t)max_hash);
+    for (int32_t i = 0; i < n; i++) order[fillp[hash[i]]++] = i;
+    free(fillp);
+  }
+  const int mxy = mx * my;
+  const int mz_last = (max_hash - 1) / mxy;
+  const double r2 = sqr_(r);
+  int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  int status = NNPP_OK;
+
+#pragma omp parallel
+  {
+    int32_t cap = 4096;
+    int32_t* buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
+#pragma omp for schedule(dynamic, 256)
+    for (int32_t i = 0; i < n; i++) {
+      const double xi = x[i], yi = y[i], zi = z[i];
+      int ixl = (int)floor((xi - r - x0) / cdx) - 1, ixh = (int)floor((xi + r - x0) / cdx) + 1;
+      int iyl = (int)floor((yi - r - y0) / cdy) - 1, iyh = (int)floor((yi + r - y0) / cdy) + 1;
+      int izl = (int)floor((zi - r - z0) / cdz) - 1, izh = (int)floor((zi + r - z0) / cdz) + 1;
+      if (ixl < 0) ixl = 0;
+      if (iyl < 0) iyl = 0;
+      if (izl < 0) izl = 0;
+      if (ixh > mx - 1) ixh = mx - 1;
+      if (iyh > my - 1) iyh = my - 1;
+      if (izh > mz_last) izh = mz_last;
+      int32_t cnt = 0;
+      for (int iz = izl; iz <= izh; iz++)
+        for (int iy = iyl; iy <= iyh; iy++)
+          for (int ix = ixl; ix <= ixh; ix++) {
+            const int m = ix + mx * iy + mxy * iz;
+            if (m >= max_hash) continue;
+            const int rr = (m % mxy) % mx, cc = (m % mxy) / mx, pp = m / mxy; /* grid.h:134-139 */
+            const double bx = x0 + rr * cdx, by = y0 + cc * cdy, bz = z0 + pp * cdz;
+            if (!or_sphere_cube_intersects(xi, yi, zi, r, bx, by, bz, cdx, cdy, cdz)) continue;
+            for (int64_t k = start[m]; k < start[m + 1]; k++) { /* grid.h:184-193 */
+              const int32_t cand = order[k];
+              if (sqr_(xi - x[cand]) + sqr_(yi - y[cand]) + sqr_(zi - z[cand]) <= r2 &&
+                  cand != i) {
+                if (cnt == cap) {
+                  cap *= 2;
+                  buf = (int32_t*)realloc(buf, sizeof(int32_t) * (size_t)cap);
+                }
+                buf[cnt++] = cand;
+              }
+            }
+          }
+      counts[i] = cnt;
+      if (cnt >= max_len) { /* grid.h:187-189: fill (self included) may not reach max_node_no */
+#pragma omp atomic write
+        status = NNPP_ERR_OVERFLOW;
+      } else if (col) {
+        qsort(buf, (size_t)cnt, sizeof(int32_t), cmp_i32);
+        memcpy(col + row_ptr[i], buf, sizeof(int32_t) * (size_t)cnt);
+      }
+    }
+    free(buf);
+  }
+  if (!col) {
+    int32_t mlen = 0;
+    for (int32_t i = 0; i < n; i++) {
+      row_ptr[i + 1] = row_ptr[i] + counts[i];
+      if (counts[i] + 1 > mlen) mlen = counts[i] + 1; /* grid.h:198-203, lengths include self */
+    }
+    if (max_len_out) *max_len_out = mlen;
+  } else if (max_len_out) {
+    int32_t mlen = 0;
+    for (int32_t i = 0; i < n; i++)
+      if (counts[i] + 1 > mlen) mlen = counts[i] + 1;
+    *max_len_out = mlen;
+  }
+  free(counts);
+  free(order);
+  free(start);
+  free(hash);
+  return status;
+}
+
+/* Volume of neighbour nn in a bond term: the reference multiplies by dx*dy*dz
+ * (utils.h:122-124); with per-node volumes it is vol[nn]. */
+static inline double bond_vol(const double* vol, const nnpp_params* p, double bval, int32_t nn) {
+  if (vol) return bval * vol[nn];
+  return bval * p->dx * p->dy * p->dz;
+}
+
+/* Reference reactions R1..R10 (utils.h:162-174, 241-253, 321-330, 396-401, 466-476, 541-547,
+ * 612-618, 683-690; none for Na+ and Cl-). u1 = iterate U^{n+1} at node i, un = U^n at node i
+ * (indexed by 0-based field), phin = lagged potential at node i. */
+static double reference_reaction(int k, const nnpp_params* p, int phase, int intf,
+                                 const double* u1, const double* un, double phin) {
+  const double CS = p->CSolid;
+  const int liquid = (phase == 0);
+  double R = 0.0;
+  switch (k) {
+    case 0: /* Mg2+ */
+      if (intf == 1 || liquid)
+        R = -p->k1b * fmax((u1[0] * un[2] * un[2] - pow(10., -p->pKMgOH2) / CS), 0.0)
+            - p->k2b * fmax((u1[0] * un[4] - pow(10., -p->pKMgCO3) / CS), 0.0)
+            - p->k3b * fmax((u1[0] * un[4] - pow(10., -p->pKMgHCO3) / CS), 0.0)
+            - p->k4b * fmax((un[0] * un[0] * un[0] * un[5] * un[5] - pow(10., -p->pKMgPO4) / CS), 0.0);
+      break;
+    case 1: /* H+ */
+      if (intf == 1) {
+        double phic = (p->R * p->T / (p->F * p->z[0])) * log(un[0] / (p->CSat / p->CSolid));
+        double eta = p->phim + p->phimse - phin - phic;
+        R += (p->JHp / (p->z[1] * p->F * p->CSolid)) * exp(-p->alphaC * p->F * eta / (p->R * p->T));
+      } else if (liquid) {
+        R = p->k7b * (pow(10., -p->pKH2O) / CS - u1[1] * un[2])
+            + p->k8b * (pow(10., -p->pKHCO3) / CS - u1[1] * un[4])
+            + p->k9b * (pow(10., -p->pKHPO4) / CS - u1[1] * un[6]);
+      }
+      break;
+    case 2: /* OH- */
+      if (intf == 1) {
+        R = -p->k1b * fmax((un[2] * un[2] * un[0] - pow(10., -p->pKMgOH2) / CS), 0.0)
+            - p->k6b * fmax((u1[2] * un[7] * un[7] * un[7] * un[7] * un[7] * un[6] * un[6] * un[6] - pow(10., -p->pKCa5OHPO4) / CS), 0.0);
+      } else if (liquid) {
+        R = -p->k1b * fmax((un[2] * un[2] * un[0] - pow(10., -p->pKMgOH2) / CS), 0.0)
+            - p->k6b * fmax((u1[2] * un[7] * un[7] * un[7] * un[7] * un[7] * un[6] * un[6] * un[6] - pow(10., -p->pKCa5OHPO4) / CS), 0.0)
+            + p->k7b * (pow(10., -p->pKH2O) / CS - u1[2] * un[1]);
+      }
+      break;
+    case 3: /* HCO3- */
+      if (phase == 1 && intf == 1) R = 0.0;
+      else if (liquid) R = p->k8b * (pow(10., -p->pKHCO3) / CS - u1[3] * un[1]);
+      break;
+    case 4: /* CO3 2- */
+      if (phase == 1 && intf == 1)
+        R = -p->k2b * fmax((u1[4] * un[0] - pow(10., -p->pKMgCO3) / CS), 0.0)
+            - p->k3b * fmax((u1[4] * un[0] - pow(10., -p->pKMgHCO3) / CS), 0.0)
+            - p->k5b * fmax((u1[4] * un[7] - pow(10., -p->pKCaCO3) / CS), 0.0);
+      else if (liquid)
+        R = -p->k2b * fmax((u1[4] * un[0] - pow(10., -p->pKMgCO3) / CS), 0.0)
+            - p->k3b * fmax((u1[4] * un[0] - pow(10., -p->pKMgHCO3) / CS), 0.0)
+            - p->k5b * fmax((u1[4] * un[7] - pow(10., -p->pKCaCO3) / CS), 0.0)
+            + p->k8b * (pow(10., -p->pKHCO3) / CS - un[1] * u1[4]);
+      break;
+    case 5: /* HPO4 2- */
+      if (phase == 1 && intf == 1)
+        R = -p->k4b * fmax((un[0] * un[0] * un[0] * un[5] * un[5] - pow(10., -p->pKMgPO4) / CS), 0.0);
+      else if (liquid)
+        R = -p->k4b * fmax((un[0] * un[0] * un[0] * un[5] * un[5] - pow(10., -p->pKMgPO4) / CS), 0.0)
+            + p->k9b * (-pow(10., -p->pKHPO4) / CS + un[6] * un[1]);
+      break;
+    case 6: /* PO4 3- */
+      if (phase == 1 && intf == 1)
+        R = -p->k6b * fmax((un[2] * un[7] * un[7] * un[7] * un[7] * un[7] * un[6] * un[6] * un[6] - pow(10., -p->pKCa5OHPO4) / CS), 0.0);
+      else if (liquid)
+        R = -p->k6b * fmax((un[2] * un[7] * un[7] * un[7] * un[7] * un[7] * un[6] * un[6] * un[6] - pow(10., -p->pKCa5OHPO4) / CS), 0.0)
+            + p->k9b * (pow(10., -p->pKHPO4) / CS - u1[6] * un[1]);
+      break;
+    case 7: /* Ca2+ */
+      if ((phase == 1 && intf == 1) || liquid)
+        R = -p->k5b * fmax((u1[7] * un[4] - pow(10., -p->pKCaCO3) / CS), 0.0)
+            - p->k6b * fmax((un[2] * un[7] * un[7] * un[7] * un[7] * un[7] * un[6] * un[6] * un[6] - pow(10., -p->pKCa5OHPO4) / CS), 0.0);
+      break;
+    default: /* Na+, Cl- */
+      break;
+  }
+  return R;
+}
+
+/* Generic chemistry (no reference counterpart): first-order sink of species 0 at interface
+ * solid nodes. */
+static double generic_reaction(int k, const nnpp_params* p, int phase, int intf,
+                               const double* u1) {
+  if (k == 0 && phase == 1 && intf == 1) return -p->r_diss * u1[0];
+  return 0.0;
+}
+
+/*
+ * utils.h:27-842 (Residuals), one pass per field as in the reference, generalised to S species:
+ *   species k: F = (U1_k,i - Un_k,i)/dt + sum_KR_k1 + sum_KR_k2 - R_k,
+ *     sum_KR_k1 += -(K1 D BV (U1_k,n - U1_k,i) / xi^(3+2 s1)) Beta dx dy dz,
+ *     sum_KR_k2 += -(K2 D BV z F U1_k,n (phin_n - phin_i) / (R T xi^(3+2 s2))) Beta dx dy dz,
+ *     D/BV by phase pair (utils.h:100-159 for species with solid_bonds, 211-238 otherwise);
+ *   phi: F = (1/(K sigma1)) sum_LL -(K sigma1 (phi1_n - phi1_i)/xi^(3+2 s111)) Beta dx dy dz
+ *            - (1/(K sigma1)) F CSolid sum_k z_k Un_k   on rows with a liquid-liquid bond,
+ *        0 elsewhere (utils.h:798-841).
+ */
+int or_residuals(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                 const double* z, const double* vol, const int64_t* row_ptr, const int32_t* col,
+                 const int32_t* phase, const double* U1, const double* Un, double voln1rel,
+                 double* Fout) {
+  const int S = p->n_species;
+  if (S < 1 || S > NNPP_MAX_SPECIES) return NNPP_ERR_ARG;
+  if (p->chemistry == NNPP_CHEM_REFERENCE && S != 10) return NNPP_ERR_UNSUPPORTED;
+  const double BV = pow(10., -p->vol_red * (1. - voln1rel));
+  const double* phin = Un + (size_t)S * n;  /* lagged potential U11n */
+  const double* phi1 = U1 + (size_t)S * n;  /* iterate potential U11n1 */
+
+  for (int k = 0; k < S; k++) {
+    const double* c1 = U1 + (size_t)k * n;
+    const double* cn = Un + (size_t)k * n;
+    const double K1 = p->K1[k], K2 = p->K2[k], Ds = p->D_solid[k], Dl = p->D_liquid[k];
+    const double zk = p->z[k], ex1 = 3.0 + 2.0 * p->s1[k], ex2 = 3.0 + 2.0 * p->s2[k];
+    const int sb = p->solid_bonds[k];
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; i++) {
+      int intf = 0;
+      if (phase[i] == 1) /* utils.h:82-89 */
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++)
+          if (phase[col[b]] == 0) {
+            intf = 1;
+            break;
+          }
+      double s1 = 0.0, s2 = 0.0;
+      for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+        const int32_t nn = col[b];
+        const double xx = x[nn] - x[i], yy = y[nn] - y[i], zz = z[nn] - z[i];
+        const double xi = sqrt(xx * xx + yy * yy + zz * zz);
+        const int pi = phase[i], pn = phase[nn];
+        double D, bv;
+        if (pi == 1 && pn == 1) {
+          if (!sb) continue;
+          D = Ds;
+          bv = BV;
+        } else if ((pi == 1 && pn == 0) || (pi == 0 && pn == 1)) {
+          D = Ds;
+          bv = sb ? BV : 1.0;
+        } else {
+          D = Dl;
+          bv = 1.0;
+        }
+        const double be = or_beta(xi, p->delta, p->dx);
+        s1 += bond_vol(vol, p, -(K1 * D * bv * (c1[nn] - c1[i]) / pow(xi, ex1)) * be, nn);
+        s2 += bond_vol(vol, p, -(K2 * D * bv * zk * p->F * c1[nn] * (phin[nn] - phin[i]) /
+                                 (p->R * p->T * pow(xi, ex2))) * be, nn);
+      }
+      double u1[NNPP_MAX_SPECIES], un[NNPP_MAX_SPECIES];
+      for (int q = 0; q < S; q++) {
+        u1[q] = U1[(size_t)q * n + i];
+        un[q] = Un[(size_t)q * n + i];
+      }
+      const double R = (p->chemistry == NNPP_CHEM_REFERENCE)
+                           ? reference_reaction(k, p, phase[i], intf, u1, un, phin[i])
+                           : generic_reaction(k, p, phase[i], intf, u1);
+      Fout[(size_t)k * n + i] = ((c1[i] - cn[i]) / p->dt) + s1 + s2 - R;
+    }
+  }
+
+  const double exphi = 3.0 + 2.0 * p->s_phi;
+  const double ks = p->K_phi * p->sigma1;
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; i++) {
+    double sum = 0.0;
+    int flag = 0;
+    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+      const int32_t nn = col[b];
+      if (!(phase[i] == 0 && phase[nn] == 0)) continue;
+      const double xx = x[nn] - x[i], yy = y[nn] - y[i], zz = z[nn] - z[i];
+      const double xi = sqrt(xx * xx + yy * yy + zz * zz);
+      sum += bond_vol(vol, p, -(ks * (phi1[nn] - phi1[i]) / pow(xi, exphi)) *
+                                  or_beta(xi, p->delta, p->dx), nn);
+      flag = 1;
+    }
+    double out = 0.0;
+    if (flag) {
+      double q = 0.0;
+      for (int k = 0; k < S; k++) q += p->z[k] * Un[(size_t)k * n + i];
+      out = (1. / ks) * sum - (1. / ks) * p->F * p->CSolid * q;
+    }
+    Fout[(size_t)S * n + i] = out;
+  }
+  return NNPP_OK;
+}
+
+/* main.cpp:676-711: Dirichlet rows of the residual. */
+int or_bc_override(int32_t n, const nnpp_params* p, const int32_t* bctypes,
+                   const double* bcvalues, const double* U1, double* Fout) {
+  const int S = p->n_species;
+  for (int32_t i = 0; i < n; i++) {
+    if (bctypes[i] != 0) continue;
+    for (int f = 0; f <= S; f++) {
+      const double v = bcvalues[(size_t)f * n + i];
+      if (v == -1) continue;
+      Fout[(size_t)f * n + i] = (f < S) ? U1[(size_t)f * n + i] - v / p->CSolid
+                                        : U1[(size_t)f * n + i] - v;
+    }
+  }
+  return NNPP_OK;
+}
+
+/* Potential bond weight of the K11 block (utils.h:1818-1827 without the K11prop1*sigma1
+ * factor that utils.h:1824-1826 divides out again): Beta(xi) V_n / xi^(3+2 s111). */
+static inline double phi_weight(const nnpp_params* p, const double* vol, double xi, int32_t nn) {
+  return bond_vol(vol, p, (1.0 / pow(xi, 3.0 + 2.0 * p->s_phi)) * or_beta(xi, p->delta, p->dx),
+                  nn);
+}
+
+static inline double bond_xi(const double* x, const double* y, const double* z, int32_t i,
+                             int32_t nn) {
+  const double xx = x[nn] - x[i], yy = y[nn] - y[i], zz = z[nn] - z[i];
+  return sqrt(xx * xx + yy * yy + zz * zz);
+}
+
+/* K11 action without identity rows: out_i = sum_LL w_in (in_i - in_n) on rows with a
+ * liquid-liquid bond, 0 elsewhere (utils.h:1794-1848). */
+int or_poisson_apply(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                     const double* z, const double* vol, const int64_t* row_ptr,
+                     const int32_t* col, const int32_t* phase, const double* in, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; i++) {
+    double acc = 0.0;
+    if (phase[i] == 0)
+      for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+        const int32_t nn = col[b];
+        if (phase[nn] != 0) continue;
+        acc += phi_weight(p, vol, bond_xi(x, y, z, i, nn), nn) * (in[i] - in[nn]);
+      }
+    out[i] = acc;
+  }
+  return NNPP_OK;
+}
+
+static double dot_(int32_t n, const double* a, const double* b) {
+  double s = 0.0;
+  for (int32_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/*
+ * Potential solve replacing the 11th GMRES solve (main.cpp:764-779) on K11 (utils.h:1794-1848)
+ * with residual F11 (utils.h:798-841). Unknowns: free nodes (liquid, at least one
+ * liquid-liquid bond, not Dirichlet for phi). Dirichlet columns move to the right-hand side and
+ * row i is scaled by V_i / (dx dy dz), which makes the operator symmetric positive
+ * (semi-)definite; Jacobi-preconditioned CG from the current phi, stopping at
+ * ||r||_2 <= rtol ||b||_2. Dirichlet nodes get phi = bc, other non-free nodes keep phi.
+ * *relres is recomputed from the final iterate.
+ */
+int or_poisson_solve(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                     const double* z, const double* vol, const int64_t* row_ptr,
+                     const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                     const double* bcvalues, const double* U, double* phi, double rtol,
+                     int32_t maxit, int32_t* iters, double* relres) {
+  const int S = p->n_species;
+  const double V0 = p->dx * p->dy * p->dz;
+  const double cb = p->F * p->CSolid / (p->K_phi * p->sigma1);
+  const double* bphi = bcvalues ? bcvalues + (size_t)S * n : NULL;
+  char* dir = (char*)calloc((size_t)n, 1);
+  char* fre = (char*)calloc((size_t)n, 1);
+  double* diag = (double*)calloc((size_t)n, sizeof(double));
+  double* b = (double*)calloc((size_t)n, sizeof(double));
+  double* r = (double*)calloc((size_t)n, sizeof(double));
+  double* zz = (double*)calloc((size_t)n, sizeof(double));
+  double* pp = (double*)calloc((size_t)n, sizeof(double));
+  double* q = (double*)calloc((size_t)n, sizeof(double));
+  for (int32_t i = 0; i < n; i++)
+    if (bctypes && bphi && bctypes[i] == 0 && bphi[i] != -1) {
+      dir[i] = 1;
+      phi[i] = bphi[i];
+    }
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; i++) {
+    if (dir[i] || phase[i] != 0) continue;
+    double d = 0.0, bd = 0.0;
+    int flag = 0;
+    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+      const int32_t nn = col[k];
+      if (phase[nn] != 0) continue;
+      const double w = phi_weight(p, vol, bond_xi(x, y, z, i, nn), nn);
+      d += w;
+      if (dir[nn]) bd += w * phi[nn];
+      flag = 1;
+    }
+    if (!flag) continue;
+    double qc = 0.0;
+    for (int k = 0; k < S; k++) qc += p->z[k] * U[(size_t)k * n + i];
+    const double sc = vol ? vol[i] / V0 : 1.0;
+    fre[i] = 1;
+    diag[i] = sc * d;
+    b[i] = sc * (cb * qc + bd);
+  }
+  /* r = b - M x */
+#define APPLY_M(in, out)                                                         \
+  _Pragma("omp parallel for schedule(static)") for (int32_t i = 0; i < n; i++) { \
+    double acc = 0.0;                                                            \
+    if (fre[i]) {                                                                \
+      for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; k++) {                    \
+        const int32_t nn = col[k];                                               \
+        if (phase[nn] != 0) continue;                                            \
+        const double w = phi_weight(p, vol, bond_xi(x, y, z, i, nn), nn);        \
+        acc += w * (in[i] - (fre[nn] ? in[nn] : 0.0));                           \
+      }                                                                          \
+      acc *= vol ? vol[i] / V0 : 1.0;                                            \
+    }                                                                            \
+    out[i] = acc;                                                                \
+  }
+  double* xv = (double*)calloc((size_t)n, sizeof(double));
+  for (int32_t i = 0; i < n; i++) xv[i] = fre[i] ? phi[i] : 0.0;
+  const double bnorm = sqrt(dot_(n, b, b));
+  int it = 0, status = NNPP_OK;
+  if (bnorm == 0.0) {
+    for (int32_t i = 0; i < n; i++)
+      if (fre[i]) xv[i] = 0.0;
+  } else {
+    APPLY_M(xv, q);
+    for (int32_t i = 0; i < n; i++) {
+      r[i] = fre[i] ? b[i] - q[i] : 0.0;
+      zz[i] = fre[i] ? r[i] / diag[i] : 0.0;
+      pp[i] = zz[i];
+    }
+    double rho = dot_(n, r, zz);
+    const double target = rtol * bnorm;
+    while (1) {
+      if (sqrt(dot_(n, r, r)) <= target) break;
+      if (it >= maxit) {
+        status = NNPP_ERR_NOCONV;
+        break;
+      }
+      APPLY_M(pp, q);
+      const double alpha = rho / dot_(n, pp, q);
+      for (int32_t i = 0; i < n; i++) {
+        xv[i] += alpha * pp[i];
+        r[i] -= alpha * q[i];
+        zz[i] = fre[i] ? r[i] / diag[i] : 0.0;
+      }
+      const double rho1 = dot_(n, r, zz);
+      const double beta = rho1 / rho;
+      rho = rho1;
+      for (int32_t i = 0; i < n; i++) pp[i] = zz[i] + beta * pp[i];
+      it++;
+    }
+  }
+  APPLY_M(xv, q);
+  for (int32_t i = 0; i < n; i++) r[i] = fre[i] ? b[i] - q[i] : 0.0;
+#undef APPLY_M
+  if (relres) *relres = bnorm > 0 ? sqrt(dot_(n, r, r)) / bnorm : 0.0;
+  if (iters) *iters = it;
+  for (int32_t i = 0; i < n; i++)
+    if (fre[i]) phi[i] = xv[i];
+  free(xv);
+  free(dir);
+  free(fre);
+  free(diag);
+  free(b);
+  free(r);
+  free(zz);
+  free(pp);
+  free(q);
+  return status;
+}
+
+/* main.cpp:400-421 (icvalues IC, phase = U_Mg > EPS), main.cpp:556-640 (Dirichlet values on
+ * the first iterate, written here into the state), main.cpp:474-479 (vol0rel). */
+int or_init_state(int32_t n, const nnpp_params* p, const int32_t* bctypes,
+                  const double* bcvalues, const double* icvalues, double* U, int32_t* phase,
+                  int64_t* solid_count, double* vol0rel) {
+  const int S = p->n_species;
+  memcpy(U, icvalues, sizeof(double) * (size_t)(S + 1) * n);
+  int64_t sc = 0;
+  for (int32_t i = 0; i < n; i++) {
+    phase[i] = (U[i] > 0.0 + p->phase_eps) ? 1 : 0;
+    sc += phase[i];
+  }
+  if (bctypes && bcvalues)
+    for (int32_t i = 0; i < n; i++) {
+      if (bctypes[i] != 0) continue;
+      for (int f = 0; f <= S; f++) {
+        const double v = bcvalues[(size_t)f * n + i];
+        if (v != -1) U[(size_t)f * n + i] = (f < S) ? v / p->CSolid : v;
+      }
+    }
+  double v0 = 0.0;
+  for (int32_t i = 0; i < n; i++)
+    if (phase[i] == 1) v0 += (U[i] - p->CSat / p->CSolid);
+  if (solid_count) *solid_count = sc;
+  if (vol0rel) *vol0rel = v0;
+  return NNPP_OK;
+}
+
+/*
+ * One explicit step (the reference's step main.cpp:547-1025 with the Newton/GMRES solve
+ * replaced by one explicit update; see include/nnpp.h nnpp_step):
+ *   species: U^{n+1} = U^n - dt * (Residuals(U1 := U^n, Un := U^n) - 0), Dirichlet, clamp;
+ *   phi^{n+1}: potential solve with the charge of U^n;
+ *   front update (main.cpp:915-922), voln1rel every sub_steps steps (main.cpp:944-949).
+ */
+int or_step(int32_t n, const nnpp_params* p, const double* x, const double* y, const double* z,
+            const double* vol, const int64_t* row_ptr, const int32_t* col,
+            const int32_t* bctypes, const double* bcvalues, double* U, int32_t* phase,
+            int64_t* step, double* voln1rel, double vol0rel, int64_t* solid_count, double rtol,
+            int32_t maxit, int32_t* cg_iters, double* relres, int32_t* flips) {
+  const int S = p->n_species;
+  const size_t nF = (size_t)(S + 1) * n;
+  double* G = (double*)malloc(sizeof(double) * nF);
+  int st = or_residuals(n, p, x, y, z, vol, row_ptr, col, phase, U, U, *voln1rel, G);
+  if (st != NNPP_OK) {
+    free(G);
+    return st;
+  }
+  double* phi = (double*)malloc(sizeof(double) * (size_t)n);
+  memcpy(phi, U + (size_t)S * n, sizeof(double) * (size_t)n);
+  int32_t its = 0;
+  double rr = 0.0;
+  const int cst = or_poisson_solve(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, U,
+                                   phi, rtol, maxit, &its, &rr);
+  for (int k = 0; k < S; k++) {
+    double* u = U + (size_t)k * n;
+    const double* g = G + (size_t)k * n;
+    const double* bv = bcvalues ? bcvalues + (size_t)k * n : NULL;
+    for (int32_t i = 0; i < n; i++) {
+      double v = u[i] - p->dt * g[i];
+      if (bctypes && bv && bctypes[i] == 0 && bv[i] != -1) v = bv[i] / p->CSolid;
+      if ((p->clamp[k] == NNPP_CLAMP_ALWAYS || (p->clamp[k] == NNPP_CLAMP_LIQUID && phase[i] == 0)) &&
+          v < p->clamp_floor)
+        v = p->clamp_floor;
+      u[i] = v;
+    }
+  }
+  memcpy(U + (size_t)S * n, phi, sizeof(double) * (size_t)n);
+  int32_t fl = 0;
+  for (int32_t i = 0; i < n; i++)
+    if (phase[i] == 1 && U[i] <= p->front_threshold) {
+      phase[i] = 0;
+      fl++;
+    }
+  if (solid_count) *solid_count -= fl;
+  if (p->sub_steps > 0 && (*step % p->sub_steps) == 0) {
+    double v = 0.0;
+    for (int32_t i = 0; i < n; i++)
+      if (phase[i] == 1) v += (U[i] - p->CSat / p->CSolid);
+    *voln1rel = v / vol0rel;
+  }
+  (*step)++;
+  if (cg_iters) *cg_iters = its;
+  if (relres) *relres = rr;
+  if (flips) *flips = fl;
+  free(phi);
+  free(G);
+  return cst;
+}
