@@ -1,0 +1,208 @@
+"""NNPP step benchmark on one MI355X.
+
+Workload: the 128^3 regular synthetic case (2,097,152 nodes, 249,027,944 directed bonds at
+delta = 3 dx), 11 species + phi, generic chemistry, Mg sphere, Dirichlet faces; each step =
+fused 11-species flux + explicit update + potential solve to relative residual 1e-10
+(warm-started CG) + front update. A "bond-update" is one directed bond processed once in one
+operator application (the flux counts it once for all species; each potential-operator
+application counts its liquid-liquid bonds).
+
+Reported value: bond-updates/s of the mixed-precision step (inputs resident in HBM). The FP64
+step is timed from the same state; the mixed/FP64 ratio and the per-field error of the mixed
+run against the FP64 run are reported beside it. `roofline` is for the dominant kernel (the
+potential-operator SpMV of the solve in the measured precision), timed with HIP events on the
+library's stream; `cpu_baseline` times the CPU oracle (OpenMP FP64 restatement of the
+reference loops) on the same workload for a bounded number of operator applications.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as ge  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def field_errors(a, b):
+    out = []
+    for f in range(a.shape[0]):
+        d = a[f] - b[f]
+        s = max(np.abs(b[f]).max(), 1e-300)
+        out.append({"max": float(np.abs(d).max() / s),
+                    "l2": float(np.linalg.norm(d) / max(np.linalg.norm(b[f]), 1e-300))})
+    return out
+
+
+def run_mode(nn, case, rp, col, precision, warmup, steps, rtol):
+    import torch
+    pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+    pb.set_neighbors(rp, col)
+    pb.set_bc(case.bctypes, case.bcvalues)
+    pb.init_state(case.icvalues)
+    pb.set_solver(rtol=rtol, maxit=50000, inner_rtol=1e-4)
+    if warmup:
+        pb.step(warmup, precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = pb.step(steps, precision)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return pb, st, dt
+
+
+def cpu_baseline(nn, case, rp, col, budget_s=20.0):
+    """Oracle (port of the reference loops, OpenMP FP64) on the same 128^3 workload: one flux
+    evaluation (all species, per-field passes as the reference) and potential-operator
+    applications until about budget_s seconds have passed."""
+    orc = ge.load_oracle()
+    O = orc.Oracle(nn.Params)
+    U, ph, _, _ = O.init_state(case)
+    t0 = time.perf_counter()
+    O.residuals(case, rp, col, ph, U, U)
+    t_flux = time.perf_counter() - t0
+    nnz = int(rp[-1])
+    reps = 0
+    t1 = time.perf_counter()
+    while time.perf_counter() - t1 < max(budget_s - t_flux, 1.0) and reps < 50:
+        O.poisson_apply(case, rp, col, ph, U[case.S])
+        reps += 1
+    t_pot = time.perf_counter() - t1
+    liquid = ph == 0
+    rows = np.repeat(np.arange(case.n), np.diff(rp))
+    ll = int(np.count_nonzero(liquid[rows] & liquid[col]))
+    bu = nnz + reps * ll
+    return {"value": bu / (t_flux + t_pot), "unit": "bond-updates/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "kind": "port",
+            "sample": f"oracle/nnpp_oracle.c on the same {case.n}-node case: 1 flux evaluation "
+                      f"({nnz} bonds, {case.S + 1} per-field passes, {t_flux:.2f} s) + {reps} "
+                      f"potential-operator applications ({ll} liquid bonds each, {t_pot:.2f} s)",
+            "flux_s": t_flux, "pot_apply_s": t_pot / max(reps, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    nn = ge.load_package()
+    ng = a.grid
+    log(f"[bench] building {ng}^3 case")
+    case = nn.synth.regular(ng, S=11, seed=0)
+    # neighbour list built on the GPU once, shared by both runs
+    pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+    t0 = time.perf_counter()
+    nnz, ml = pb.build_clouds_hash()
+    torch.cuda.synchronize()
+    t_nb = time.perf_counter() - t0
+    rp, col = pb.neighbors()
+    pb.close()
+    log(f"[bench] neighbours: nnz={nnz} max_len={ml} in {t_nb:.3f}s")
+
+    res = {}
+    for name, prec in (("fp64", nn.FP64), ("mixed", nn.MIXED)):
+        log(f"[bench] {name}: warmup {a.warmup}, timed {a.steps}")
+        pbm, st, dt = run_mode(nn, case, rp, col, prec, a.warmup, a.steps, a.rtol)
+        bu_flux = st["steps"] * st["nnz_flux"]
+        bu_cg = (st["cg_spmv_fp64"] + st["cg_spmv_fp32"]) * st["nnz_ll"]
+        res[name] = {"time_s": dt, "stats": st, "bu": bu_flux + bu_cg, "bu_flux": bu_flux,
+                     "bu_cg": bu_cg, "U": pbm.get_state(), "pb": pbm}
+        log(f"[bench] {name}: {dt / a.steps * 1e3:.2f} ms/step, cg iters {st['cg_iterations']}, "
+            f"relres {st['last_relres']:.2e}")
+
+    mixed = res["mixed"]
+    fp64 = res["fp64"]
+    errs = field_errors(mixed["U"], fp64["U"])
+    max_err = max(e["max"] for e in errs)
+    value = mixed["bu"] / mixed["time_s"]
+    value64 = fp64["bu"] / fp64["time_s"]
+
+    # roofline of the dominant kernel: potential SpMV (FP32 variant in the mixed run)
+    pbm = mixed["pb"]
+    n = case.n
+    nll = mixed["stats"]["nnz_ll"]
+    reps = 50
+    t_spmv32 = pbm.time_kernel(nn._abi.KERNEL_POT_SPMV, nn.MIXED, reps)
+    t_spmv64 = pbm.time_kernel(nn._abi.KERNEL_POT_SPMV, nn.FP64, reps)
+    t_flux64 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.FP64, 10)
+    t_flux32 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.MIXED, 10)
+    # algorithmic bytes per launch: col (4) + weight per liquid bond; per node read p_i, dd_i,
+    # write q_i (the gathered p_j counted once per node)
+    bytes32 = nll * (4 + 4) + n * (4 * 4)
+    bytes64 = nll * (4 + 8) + n * (4 * 8)
+    S = case.S
+    fbytes = lambda wb: nnz * (4 + wb) + n * (8 * (S + 1) + 1 + 2 + 8 * S)  # noqa: E731
+    ach = bytes32 / (t_spmv32 * 1e-3) / 1e9
+    kern = {
+        "pot_spmv_fp32": {"ms": t_spmv32, "GBps": bytes32 / (t_spmv32 * 1e-3) / 1e9},
+        "pot_spmv_fp64": {"ms": t_spmv64, "GBps": bytes64 / (t_spmv64 * 1e-3) / 1e9},
+        "flux_fp64": {"ms": t_flux64, "GBps": fbytes(8) / (t_flux64 * 1e-3) / 1e9},
+        "flux_mixed": {"ms": t_flux32, "GBps": fbytes(4) / (t_flux32 * 1e-3) / 1e9},
+    }
+    for r_ in res.values():
+        r_["pb"].close()
+
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu baseline")
+        cpu = cpu_baseline(nn, case, rp, col, a.cpu_budget)
+
+    out = {
+        "metric": "bond-updates/s per NNPP step, mixed vs FP64 @ rel err<=1e-6; % of HBM peak",
+        "value": value,
+        "unit": "bond-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": mixed["time_s"] / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bond sums (Kahan) / f64 state, dots, residual checks",
+        "data": "synthetic (seeded lattice case in the reference's input layout)",
+        "config": {"workload": f"regular {ng}^3 lattice, 11 species + phi, delta=3dx, Mg sphere "
+                               f"r=0.3, Dirichlet faces, CG rtol {a.rtol:g}",
+                   "nodes": n, "bonds": nnz, "liquid_bonds": nll, "species": S},
+        "fp64": {"value": value64, "ms_per_step": fp64["time_s"] / a.steps * 1e3,
+                 "cg_iterations": fp64["stats"]["cg_iterations"],
+                 "bu_flux": fp64["bu_flux"], "bu_cg": fp64["bu_cg"]},
+        "mixed": {"cg_inner_iterations": mixed["stats"]["cg_iterations"],
+                  "spmv_fp64": mixed["stats"]["cg_spmv_fp64"],
+                  "spmv_fp32": mixed["stats"]["cg_spmv_fp32"],
+                  "bu_flux": mixed["bu_flux"], "bu_cg": mixed["bu_cg"]},
+        "mixed_over_fp64": (fp64["time_s"] / mixed["time_s"]),
+        "mixed_vs_fp64_err": {"max_rel": max_err, "fields": errs, "after_steps": a.steps + a.warmup},
+        "neighbor_build_s": t_nb,
+        "kernels": kern,
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_m_spmv (FP32 potential SpMV + dot)",
+                     "bytes_per_launch": bytes32},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

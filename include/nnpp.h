@@ -127,6 +127,11 @@ typedef struct nnpp_ctx nnpp_ctx;
 
 /* ---------------------------------------------------------------- host-only helpers ----- */
 
+/* sizeof(nnpp_params) and sizeof(nnpp_step_stats) as compiled into the library, so bindings
+ * in other languages can check their struct mirrors. No reference counterpart. */
+int64_t nnpp_abi_params_size(void);
+int64_t nnpp_abi_stats_size(void);
+
 /* Fill *p with the reference's hard-coded behaviour for S species: solid_bonds = {1,0,...},
  * clamp = ALWAYS for species 1..9 and LIQUID for species 10 (main.cpp:805-837),
  * clamp_floor = 1e-8/CSolid, front_threshold = CSat/CSolid, phase_eps = 1e-5,
@@ -250,12 +255,22 @@ int nnpp_step(nnpp_ctx* ctx, int32_t nsteps, int32_t precision, int32_t check_fi
 /* Front update alone (main.cpp:915-922) on slot N. */
 int nnpp_front_update(nnpp_ctx* ctx, double threshold, int32_t* n_flipped);
 
-/* Raw device pointer of a field in a slot (for zero-copy interop; valid until the context
- * is destroyed or nnpp_set_nodes is called again). */
+/* Raw device pointer of a field in a slot (for zero-copy interop). nnpp_step swaps the
+ * buffers of slot N with its scratch buffer, so the pointer is valid until the next
+ * nnpp_step, nnpp_set_nodes or nnpp_destroy call. */
 int nnpp_state_device_ptr(nnpp_ctx* ctx, int32_t slot, int32_t field, void** dptr);
 
 /* HIP stream the context issues its work on (hipStream_t as void*). */
 int nnpp_stream(nnpp_ctx* ctx, void** stream);
+
+/* Measurement hook, no reference counterpart: launches one hot kernel `reps` times on the
+ * context's stream between two HIP events and returns the mean duration per launch in ms.
+ * kernel: NNPP_KERNEL_FLUX (explicit flux/update kernel on slot N) or NNPP_KERNEL_POT_SPMV
+ * (potential operator application with its dot product); precision selects the FP64 or the
+ * mixed variant. Leaves the state of slot N unchanged. */
+enum { NNPP_KERNEL_FLUX = 0, NNPP_KERNEL_POT_SPMV = 1 };
+int nnpp_time_kernel(nnpp_ctx* ctx, int32_t kernel, int32_t precision, int32_t reps,
+                     double* avg_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,15 @@
+"""MI355X-native NNPP time step (nonlocal Nernst-Planck-Poisson corrosion of a Mg implant).
+
+Python host mirror over libnnpp.so's C ABI (include/nnpp.h). The compute path is the HIP
+library; importing this package loads it and fails if it is missing.
+"""
+from . import _abi
+from ._abi import (CHEM_GENERIC, CHEM_REFERENCE, FP64, MIXED, SLOT_N, SLOT_N1, NnppError,
+                   Params, StepStats)
+from .problem import Problem
+from . import synth, formats
+
+lib = _abi.load()
+
+__all__ = ["CHEM_GENERIC", "CHEM_REFERENCE", "FP64", "MIXED", "SLOT_N", "SLOT_N1", "NnppError",
+           "Params", "StepStats", "Problem", "synth", "formats", "lib"]

@@ -1,0 +1,487 @@
+// k_misc.hip — device exclusive scan, bond-slice construction, front update (main.cpp:915-922
+// plus the volume sum of main.cpp:944-949), Dirichlet residual rows (main.cpp:676-711) and
+// the non-finite check.
+#include <vector>
+
+#include "nnpp_internal.hpp"
+
+namespace {
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = NNPP_BLOCK * SCAN_ITEMS;
+
+// Exclusive scan of one tile of 2048 values; tile totals go to bsum.
+template <typename TI>
+__global__ __launch_bounds__(NNPP_BLOCK) void k_scan_tile(const TI* __restrict__ in, int64_t n,
+                                                          int64_t* __restrict__ out,
+                                                          int64_t* __restrict__ bsum) {
+  __shared__ int64_t s[NNPP_BLOCK];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t v[SCAN_ITEMS];
+  int64_t tsum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    v[k] = (base + k < n) ? (int64_t)in[base + k] : 0;
+    tsum += v[k];
+  }
+  s[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int o = 1; o < NNPP_BLOCK; o <<= 1) {  // Hillis-Steele inclusive scan of thread sums
+    const int64_t add = (threadIdx.x >= (unsigned)o) ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += add;
+    __syncthreads();
+  }
+  int64_t run = s[threadIdx.x] - tsum;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == NNPP_BLOCK - 1) bsum[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void k_scan_add(int64_t n, int64_t* __restrict__ out,
+                           const int64_t* __restrict__ boff) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += boff[i / SCAN_TILE];
+}
+
+__global__ void k_set_total(int64_t n, int64_t* __restrict__ out,
+                            const int64_t* __restrict__ bsum_scan, int64_t nt,
+                            const int64_t* __restrict__ bsum) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = bsum_scan[nt - 1] + bsum[nt - 1];
+}
+
+// Longest row per 64-row slice.
+__global__ void k_slice_len(int32_t n, const int64_t* __restrict__ row_ptr,
+                            int32_t* __restrict__ len) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicMax(&len[i / NNPP_WAVE], (int32_t)(row_ptr[i + 1] - row_ptr[i]));
+}
+
+__global__ void k_times_wave(int64_t ns, const int32_t* __restrict__ len,
+                             int32_t* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ns) out[c] = len[c] * NNPP_WAVE;
+}
+
+struct WeightArgs {
+  double delta, dx, V0, expo;
+  const double* vol;
+};
+
+__device__ __forceinline__ double bond_weight(const WeightArgs& wa, const double* x,
+                                              const double* y, const double* z, int64_t i,
+                                              int32_t j) {
+  const double xx = x[j] - x[i], yy = y[j] - y[i], zz = z[j] - z[i];
+  const double xi = sqrt(xx * xx + yy * yy + zz * zz);
+  const double V = wa.vol ? wa.vol[j] : wa.V0;
+  return ((1.0 / pow(xi, wa.expo)) * nnpp_beta(xi, wa.delta, wa.dx)) * V;
+}
+
+// Fill the bond slices from CSR: entry t of row i at off[i/64] + 64 t + i%64.
+__global__ void k_slice_fill(int32_t n, int64_t nrows_pad, const int64_t* __restrict__ row_ptr,
+                             const int32_t* __restrict__ col, const double* __restrict__ x,
+                             const double* __restrict__ y, const double* __restrict__ z,
+                             WeightArgs wa, const int64_t* __restrict__ off,
+                             const int32_t* __restrict__ len, int32_t* __restrict__ scol,
+                             double* __restrict__ w64, float* __restrict__ w32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows_pad) return;
+  const int64_t c = i / NNPP_WAVE;
+  const int64_t base = off[c] + (i % NNPP_WAVE);
+  const int L = len[c];
+  const int64_t b0 = (i < n) ? row_ptr[i] : 0;
+  const int rl = (i < n) ? (int)(row_ptr[i + 1] - b0) : 0;
+  for (int t = 0; t < L; t++) {
+    const int64_t slot = base + (int64_t)t * NNPP_WAVE;
+    if (t < rl) {
+      const int32_t j = col[b0 + t];
+      const double w = bond_weight(wa, x, y, z, i, j);
+      scol[slot] = j;
+      w64[slot] = w;
+      w32[slot] = (float)w;
+    } else {
+      scol[slot] = -1;
+      w64[slot] = 0.0;
+      w32[slot] = 0.0f;
+    }
+  }
+}
+
+// Potential operator, step 1: free flags (liquid, not Dirichlet for phi, >= 1 liquid bond).
+__global__ void k_pot_flags(int32_t n, int S, const int64_t* __restrict__ row_ptr,
+                            const int32_t* __restrict__ col, const uint8_t* __restrict__ phase,
+                            const uint16_t* __restrict__ bcmask, uint8_t* __restrict__ pfree) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint8_t f = 0;
+  if (phase[i] == 0 && !((bcmask[i] >> S) & 1u)) {
+    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++)
+      if (phase[col[b]] == 0) {
+        f = 1;
+        break;
+      }
+  }
+  pfree[i] = f;
+}
+
+// Step 2: entries per free row (free-free liquid bonds) -> slice lengths.
+__global__ void k_pot_len(int32_t n, const int64_t* __restrict__ row_ptr,
+                          const int32_t* __restrict__ col, const uint8_t* __restrict__ phase,
+                          const uint8_t* __restrict__ pfree, int32_t* __restrict__ len,
+                          int32_t* __restrict__ rcount) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int cnt = 0;
+  if (pfree[i])
+    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+      const int32_t j = col[b];
+      if (phase[j] == 0 && pfree[j]) cnt++;
+    }
+  rcount[i] = cnt;
+  atomicMax(&len[i / NNPP_WAVE], cnt);
+}
+
+// Step 3: entries, scaled diagonal, Dirichlet-neighbour diagonal part and the Dirichlet part of
+// the right-hand side. Row i is scaled by V_i / V0 (symmetrises the operator for per-node
+// volumes; 1 for the reference's constant volume).
+__global__ void k_pot_fill(int32_t n, int64_t nrows_pad, int S, const int64_t* __restrict__ row_ptr,
+                           const int32_t* __restrict__ col, const uint8_t* __restrict__ phase,
+                           const uint8_t* __restrict__ pfree, const uint16_t* __restrict__ bcmask,
+                           const double* __restrict__ bcphi, const double* __restrict__ x,
+                           const double* __restrict__ y, const double* __restrict__ z,
+                           WeightArgs wa, const int64_t* __restrict__ off,
+                           const int32_t* __restrict__ len, int32_t* __restrict__ scol,
+                           double* __restrict__ w64, float* __restrict__ w32,
+                           double* __restrict__ pdd, float* __restrict__ pdd32,
+                           double* __restrict__ pinv, float* __restrict__ pinv32,
+                           double* __restrict__ pbdir, double* __restrict__ psc) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows_pad) return;
+  const int64_t c = i / NNPP_WAVE;
+  const int64_t base = off[c] + (i % NNPP_WAVE);
+  const int L = len[c];
+  int t = 0;
+  if (i < n) {
+    const double sc = wa.vol ? wa.vol[i] / wa.V0 : 1.0;
+    double diag = 0.0, ddir = 0.0, bdir = 0.0;
+    if (pfree[i]) {
+      for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+        const int32_t j = col[b];
+        if (phase[j] != 0) continue;
+        const double w = bond_weight(wa, x, y, z, i, j);
+        diag += w;
+        if (pfree[j]) {
+          const int64_t slot = base + (int64_t)t * NNPP_WAVE;
+          scol[slot] = j;
+          w64[slot] = sc * w;
+          w32[slot] = (float)(sc * w);
+          t++;
+        } else {  // a liquid neighbour of a free node is free or Dirichlet
+          ddir += w;
+          if ((bcmask[j] >> S) & 1u) bdir += w * bcphi[j];
+        }
+      }
+    }
+    pdd[i] = sc * ddir;
+    pdd32[i] = (float)(sc * ddir);
+    pinv[i] = pfree[i] ? 1.0 / (sc * diag) : 0.0;
+    pinv32[i] = pfree[i] ? (float)(1.0 / (sc * diag)) : 0.0f;
+    pbdir[i] = bdir;
+    psc[i] = sc;
+  }
+  for (; t < L; t++) {
+    const int64_t slot = base + (int64_t)t * NNPP_WAVE;
+    scol[slot] = -1;
+    w64[slot] = 0.0;
+    w32[slot] = 0.0f;
+  }
+}
+
+// main.cpp:915-922 front update, then the volume sum of main.cpp:944-949 over the new phase.
+__global__ __launch_bounds__(NNPP_BLOCK) void k_front(int32_t n, const double* __restrict__ U0,
+                                                      uint8_t* __restrict__ phase, double thr,
+                                                      double sat, double* part, unsigned* ticket,
+                                                      double* scal) {
+  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    uint8_t ph = phase[i];
+    const double u = U0[i];
+    if (ph == 1 && u <= thr) {
+      ph = 0;
+      phase[i] = 0;
+      v[0] = 1.0;
+    }
+    if (ph == 1) {
+      v[1] = (u - sat);
+      v[2] = 1.0;
+    }
+  }
+  if (nnpp_grid_sum<3>(v, part, ticket, gridDim.x, blockIdx.x)) {
+    scal[SC_R0] = v[0];
+    scal[SC_R1] = v[1];
+    scal[SC_R2] = v[2];
+  }
+}
+
+// main.cpp:676-711: F = U1 - bc on Dirichlet rows/fields (bc already divided by CSolid for
+// species).
+__global__ void k_bc_rows(int32_t n, int F, const uint16_t* __restrict__ bcmask,
+                          const double* __restrict__ bcval, const double* __restrict__ U1,
+                          double* __restrict__ Fo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned m = bcmask[i];
+  if (!m) return;
+  for (int f = 0; f < F; f++)
+    if ((m >> f) & 1u) Fo[(int64_t)f * n + i] = U1[(int64_t)f * n + i] - bcval[(int64_t)f * n + i];
+}
+
+__global__ __launch_bounds__(NNPP_BLOCK) void k_nonfinite(int64_t m, const double* __restrict__ U,
+                                                          double* part, unsigned* ticket,
+                                                          double* scal) {
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * NNPP_BLOCK)
+    if (!isfinite(U[i])) v[0] += 1.0;
+  if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) scal[SC_R3] = v[0];
+}
+
+// Poisson operator action for nnpp_poisson_apply: K11 rows of utils.h:1794-1848 without the
+// identity rows, weights computed from the coordinates.
+__global__ void k_pot_apply_csr(int32_t n, const int64_t* __restrict__ row_ptr,
+                                const int32_t* __restrict__ col,
+                                const uint8_t* __restrict__ phase, const double* __restrict__ x,
+                                const double* __restrict__ y, const double* __restrict__ z,
+                                WeightArgs wa, const double* __restrict__ in,
+                                double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  if (phase[i] == 0)
+    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+      const int32_t j = col[b];
+      if (phase[j] != 0) continue;
+      acc += bond_weight(wa, x, y, z, i, j) * (in[i] - in[j]);
+    }
+  out[i] = acc;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host launchers ---------
+
+template <typename TI>
+static int scan_impl(nnpp_ctx* c, const TI* in, int64_t n, int64_t* out) {
+  hipStream_t s = c->stream;
+  if (n == 0) {
+    NNPP_HIP(c, hipMemsetAsync(out, 0, sizeof(int64_t), s));
+    return NNPP_OK;
+  }
+  const int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  int64_t *bsum = nullptr, *boff = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&bsum, sizeof(int64_t) * nt));
+  NNPP_HIP(c, hipMalloc((void**)&boff, sizeof(int64_t) * (nt + 1)));
+  k_scan_tile<TI><<<nt, NNPP_BLOCK, 0, s>>>(in, n, out, bsum);
+  int st = NNPP_OK;
+  if (nt > 1) {
+    st = scan_impl<int64_t>(c, bsum, nt, boff);
+    if (st == NNPP_OK) {
+      k_scan_add<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, out, boff);
+      k_set_total<<<1, 1, 0, s>>>(n, out, boff, nt, bsum);
+    }
+  } else {
+    NNPP_HIP(c, hipMemcpyAsync(out + n, bsum, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  hipFree(bsum);
+  hipFree(boff);
+  if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
+  return st;
+}
+
+int nnpp_device_exclusive_scan_i32(nnpp_ctx* c, const int32_t* in, int64_t n, int64_t* out) {
+  return scan_impl<int32_t>(c, in, n, out);
+}
+
+static int slices_alloc(nnpp_ctx* c, nnpp_slices& sl, int64_t nslice) {
+  if (sl.nslice != nslice || !sl.off) {
+    if (sl.off) hipFree(sl.off);
+    if (sl.len) hipFree(sl.len);
+    NNPP_HIP(c, hipMalloc((void**)&sl.off, sizeof(int64_t) * (nslice + 1)));
+    NNPP_HIP(c, hipMalloc((void**)&sl.len, sizeof(int32_t) * (nslice ? nslice : 1)));
+    sl.nslice = nslice;
+  }
+  return NNPP_OK;
+}
+
+static int slices_reserve(nnpp_ctx* c, nnpp_slices& sl, int64_t slots) {
+  if (slots > sl.cap_slots) {
+    if (sl.col) hipFree(sl.col);
+    if (sl.w64) hipFree(sl.w64);
+    if (sl.w32) hipFree(sl.w32);
+    sl.col = nullptr;
+    sl.w64 = nullptr;
+    sl.w32 = nullptr;
+    const int64_t cap = slots + slots / 16 + 64;
+    NNPP_HIP(c, hipMalloc((void**)&sl.col, sizeof(int32_t) * cap));
+    NNPP_HIP(c, hipMalloc((void**)&sl.w64, sizeof(double) * cap));
+    NNPP_HIP(c, hipMalloc((void**)&sl.w32, sizeof(float) * cap));
+    sl.cap_slots = cap;
+  }
+  sl.slots = slots;
+  return NNPP_OK;
+}
+
+// Slice offsets from per-slice lengths: off = exclusive scan of 64 * len.
+static int slices_offsets(nnpp_ctx* c, nnpp_slices& sl) {
+  int32_t* tmp = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&tmp, sizeof(int32_t) * (sl.nslice ? sl.nslice : 1)));
+  k_times_wave<<<nnpp_nblocks(sl.nslice ? sl.nslice : 1), NNPP_BLOCK, 0, c->stream>>>(sl.nslice, sl.len, tmp);
+  int st = scan_impl<int32_t>(c, tmp, sl.nslice, sl.off);
+  hipFree(tmp);
+  if (st) return st;
+  int64_t slots = 0;
+  NNPP_HIP(c, hipMemcpy(&slots, sl.off + sl.nslice, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return slices_reserve(c, sl, slots);
+}
+
+static WeightArgs weight_args(const nnpp_ctx* c, double expo) {
+  WeightArgs wa;
+  wa.delta = c->p.delta;
+  wa.dx = c->p.dx;
+  wa.V0 = c->p.dx * c->p.dy * c->p.dz;
+  wa.expo = expo;
+  wa.vol = c->vol;
+  return wa;
+}
+
+int nnpp_build_bond_slices(nnpp_ctx* c) {
+  const int32_t n = c->n;
+  const int64_t ns = (n + NNPP_WAVE - 1) / NNPP_WAVE;
+  int st = slices_alloc(c, c->bonds, ns);
+  if (st) return st;
+  hipStream_t s = c->stream;
+  NNPP_HIP(c, hipMemsetAsync(c->bonds.len, 0, sizeof(int32_t) * (ns ? ns : 1), s));
+  if (n) k_slice_len<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->row_ptr, c->bonds.len);
+  NNPP_HIP(c, hipGetLastError());
+  st = slices_offsets(c, c->bonds);
+  if (st) return st;
+  const int64_t npad = ns * NNPP_WAVE;
+  if (npad)
+    k_slice_fill<<<nnpp_nblocks(npad), NNPP_BLOCK, 0, s>>>(
+        n, npad, c->row_ptr, c->col, c->x, c->y, c->z, weight_args(c, c->w_exponent),
+        c->bonds.off, c->bonds.len, c->bonds.col, c->bonds.w64, c->bonds.w32);
+  NNPP_HIP(c, hipGetLastError());
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  c->bonds.nnz = c->nnz;
+  c->geom_dirty = false;
+  return NNPP_OK;
+}
+
+int nnpp_build_potential(nnpp_ctx* c) {
+  const int32_t n = c->n;
+  const int S = c->p.n_species;
+  const int64_t ns = (n + NNPP_WAVE - 1) / NNPP_WAVE;
+  hipStream_t s = c->stream;
+  int st = slices_alloc(c, c->pot, ns);
+  if (st) return st;
+  int32_t* rcount = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&rcount, sizeof(int32_t) * (n ? n : 1)));
+  NNPP_HIP(c, hipMemsetAsync(c->pot.len, 0, sizeof(int32_t) * (ns ? ns : 1), s));
+  if (n) {
+    k_pot_flags<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, S, c->row_ptr, c->col, c->phase,
+                                                       c->bcmask, c->pfree);
+    k_pot_len<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->row_ptr, c->col, c->phase, c->pfree,
+                                                     c->pot.len, rcount);
+  }
+  NNPP_HIP(c, hipGetLastError());
+  st = slices_offsets(c, c->pot);
+  if (st) {
+    hipFree(rcount);
+    return st;
+  }
+  const int64_t npad = ns * NNPP_WAVE;
+  const double expo = 3.0 + 2.0 * c->p.s_phi;
+  if (npad)
+    k_pot_fill<<<nnpp_nblocks(npad), NNPP_BLOCK, 0, s>>>(
+        n, npad, S, c->row_ptr, c->col, c->phase, c->pfree, c->bcmask, c->bcval + (int64_t)S * n,
+        c->x, c->y, c->z, weight_args(c, expo), c->pot.off, c->pot.len, c->pot.col, c->pot.w64,
+        c->pot.w32, c->pdd, c->pdd32, c->pinv, c->pinv32, c->pbdir, c->psc);
+  NNPP_HIP(c, hipGetLastError());
+  // count real entries and free rows
+  std::vector<int32_t> rc(n);
+  std::vector<uint8_t> fr(n);
+  if (n) {
+    NNPP_HIP(c, hipMemcpyAsync(rc.data(), rcount, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    NNPP_HIP(c, hipMemcpyAsync(fr.data(), c->pfree, n, hipMemcpyDeviceToHost, s));
+  }
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  hipFree(rcount);
+  int64_t nnz = 0, nf = 0;
+  for (int32_t i = 0; i < n; i++) {
+    nnz += rc[i];
+    nf += fr[i];
+  }
+  c->pot.nnz = nnz;
+  c->pot_nfree = nf;
+  c->pot_dirty = false;
+  return NNPP_OK;
+}
+
+int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flips,
+                   double* volsum) {
+  (void)do_vol;
+  const int32_t n = c->n;
+  hipStream_t s = c->stream;
+  if (n) {
+    k_front<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, U, c->phase, thr, c->p.CSat / c->p.CSolid,
+                                                   c->red_part, c->red_ticket, c->scal);
+    NNPP_HIP(c, hipGetLastError());
+    NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT,
+                               hipMemcpyDeviceToHost, s));
+  }
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  const int32_t fl = n ? (int32_t)c->h_scal[SC_R0] : 0;
+  if (flips) *flips = fl;
+  if (volsum) *volsum = n ? c->h_scal[SC_R1] : 0.0;
+  if (n) c->solid_count = (int64_t)c->h_scal[SC_R2];
+  if (fl > 0) c->pot_dirty = true;
+  return NNPP_OK;
+}
+
+int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo) {
+  if (!c->n || !c->have_bc) return NNPP_OK;
+  k_bc_rows<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(c->n, c->p.n_species + 1, c->bcmask,
+                                                              c->bcval, U1, Fo);
+  NNPP_HIP(c, hipGetLastError());
+  return NNPP_OK;
+}
+
+int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad) {
+  const int64_t m = (int64_t)c->n * (c->p.n_species + 1);
+  *bad = 0;
+  if (!m) return NNPP_OK;
+  int64_t nb = nnpp_nblocks(m);
+  if (nb > 4096) nb = 4096;
+  k_nonfinite<<<nb, NNPP_BLOCK, 0, c->stream>>>(m, U, c->red_part, c->red_ticket, c->scal);
+  NNPP_HIP(c, hipGetLastError());
+  NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost,
+                             c->stream));
+  NNPP_HIP(c, hipStreamSynchronize(c->stream));
+  *bad = (int32_t)c->h_scal[SC_R3];
+  return NNPP_OK;
+}
+
+int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out) {
+  if (!c->n) return NNPP_OK;
+  k_pot_apply_csr<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(
+      c->n, c->row_ptr, c->col, c->phase, c->x, c->y, c->z,
+      weight_args(c, 3.0 + 2.0 * c->p.s_phi), in, out);
+  NNPP_HIP(c, hipGetLastError());
+  return NNPP_OK;
+}

@@ -75,7 +75,10 @@ struct nnpp_ctx {
 
   // potential solve
   uint8_t* pfree = nullptr;
-  double* pdiag = nullptr;   // scaled diagonal (FP64)
+  double* pdd = nullptr;     // scaled weight sum of Dirichlet liquid neighbours (FP64)
+  float* pdd32 = nullptr;
+  double* pinv = nullptr;    // inverse of the scaled diagonal, 0 on non-free rows
+  float* pinv32 = nullptr;
   double* pbdir = nullptr;   // Dirichlet part of the right-hand side (unscaled)
   double* psc = nullptr;     // row scale V_i / (dx dy dz)
   double* cg_b = nullptr;
@@ -89,7 +92,6 @@ struct nnpp_ctx {
   float* f_z = nullptr;
   float* f_p = nullptr;
   float* f_q = nullptr;
-  float* pdiag32 = nullptr;
   int64_t pot_nfree = 0;
 
   // reductions: partials + ticket + scalars; pinned host mirror of the scalars
@@ -221,4 +223,4 @@ int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flip
                    double* volsum);
 int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo);
 int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
-int nnpp_ensure_reduce(nnpp_ctx* c, int64_t nblk);
+int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision);

@@ -1,0 +1,176 @@
+"""Host-side mirror of the reference's step-path interface over the C ABI.
+
+One `Problem` owns one device context (nnpp_ctx). Method names follow the reference:
+`build_clouds_hash` (grid.h:43-204), `residuals` (utils::Residuals, utils.h:27-842),
+`step` (the time loop of main.cpp:547-1025 with an explicit update), `front_update`
+(main.cpp:915-922). Field arrays are numpy (F, n) float64, field f = species f+1 of the
+reference, field S = phi.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import FP64, MIXED, SLOT_N, SLOT_N1, NnppError, dptr, iptr, lptr
+
+
+class Problem:
+    def __init__(self, params, x, y, z, volumes=None, device=0):
+        self.lib = _abi.load()
+        self.params = params.copy()
+        self.S = params.S
+        self.F = self.S + 1
+        ctx = C.c_void_p()
+        st = self.lib.nnpp_create(C.byref(ctx), device)
+        if st != _abi.OK:
+            raise NnppError(st, "nnpp_create failed (no HIP device?)")
+        self.ctx = ctx
+        self._check(self.lib.nnpp_set_params(self.ctx, C.byref(self.params)))
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.z = np.ascontiguousarray(z, dtype=np.float64)
+        self.n = int(self.x.shape[0])
+        self.vol = None if volumes is None else np.ascontiguousarray(volumes, dtype=np.float64)
+        self._check(self.lib.nnpp_set_nodes(self.ctx, self.n, dptr(self.x), dptr(self.y),
+                                            dptr(self.z), dptr(self.vol) if self.vol is not None else None))
+        self.nnz = None
+
+    # -- plumbing ------------------------------------------------------------------------
+    def _check(self, st):
+        if st != _abi.OK:
+            msg = self.lib.nnpp_last_error(self.ctx)
+            raise NnppError(st, msg.decode() if msg else "")
+        return st
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.nnpp_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- neighbourhoods ------------------------------------------------------------------
+    def build_clouds_hash(self, max_node_no=1024, r=None, dx=None, dy=None, dz=None):
+        """grid.h:43-204 as called at main.cpp:304-305: r = delta*(1+1e-5), cells 10*dx.
+
+        Returns (nnz, max_cloud_length); max_cloud_length counts self as the reference does.
+        Raises NnppError(ERR_OVERFLOW) where the reference throws std::runtime_error."""
+        p = self.params
+        r = p.delta * (1.00 + 1e-5) if r is None else r
+        dx = p.dx * 10. if dx is None else dx
+        dy = p.dy * 10. if dy is None else dy
+        dz = p.dz * 10. if dz is None else dz
+        nnz = C.c_int64()
+        ml = C.c_int32()
+        self._check(self.lib.nnpp_build_neighbors(self.ctx, r, dx, dy, dz, max_node_no,
+                                                  C.byref(nnz), C.byref(ml)))
+        self.nnz = int(nnz.value)
+        return self.nnz, int(ml.value)
+
+    def neighbors(self):
+        rp = np.empty(self.n + 1, dtype=np.int64)
+        col = np.empty(max(self.nnz or 0, 1), dtype=np.int32)
+        self._check(self.lib.nnpp_get_neighbors(self.ctx, lptr(rp), iptr(col)))
+        return rp, col[: self.nnz]
+
+    def set_neighbors(self, row_ptr, col):
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        cl = np.ascontiguousarray(col, dtype=np.int32)
+        self._check(self.lib.nnpp_set_neighbors(self.ctx, lptr(rp), iptr(cl) if cl.size else None))
+        self.nnz = int(rp[-1])
+
+    # -- inputs / state ------------------------------------------------------------------
+    def set_bc(self, bctypes, bcvalues):
+        bt = np.ascontiguousarray(bctypes, dtype=np.int32)
+        bv = np.ascontiguousarray(bcvalues, dtype=np.float64).reshape(self.F, self.n)
+        self._check(self.lib.nnpp_set_bc(self.ctx, iptr(bt), dptr(bv)))
+
+    def init_state(self, icvalues):
+        ic = np.ascontiguousarray(icvalues, dtype=np.float64).reshape(self.F, self.n)
+        self._check(self.lib.nnpp_init_state(self.ctx, dptr(ic)))
+
+    def set_state(self, U, slot=SLOT_N):
+        a = np.ascontiguousarray(U, dtype=np.float64).reshape(self.F, self.n)
+        self._check(self.lib.nnpp_set_state(self.ctx, slot, dptr(a)))
+
+    def get_state(self, slot=SLOT_N):
+        a = np.empty((self.F, self.n), dtype=np.float64)
+        self._check(self.lib.nnpp_get_state(self.ctx, slot, dptr(a)))
+        return a
+
+    def set_phase(self, phase):
+        a = np.ascontiguousarray(phase, dtype=np.int32)
+        self._check(self.lib.nnpp_set_phase(self.ctx, iptr(a)))
+
+    def get_phase(self):
+        a = np.empty(self.n, dtype=np.int32)
+        self._check(self.lib.nnpp_get_phase(self.ctx, iptr(a)))
+        return a
+
+    def set_time(self, step, voln1rel, vol0rel):
+        self._check(self.lib.nnpp_set_time(self.ctx, step, voln1rel, vol0rel))
+
+    def set_solver(self, rtol=1e-10, maxit=20000, inner_rtol=1e-4):
+        self._check(self.lib.nnpp_set_solver(self.ctx, rtol, maxit, inner_rtol))
+
+    # -- operators -----------------------------------------------------------------------
+    def residuals(self, voln1rel=0.0, apply_bc=False, precision=FP64):
+        """utils::Residuals on (slot N1 = U^{n+1}, slot N = U^n); (F, n) array."""
+        out = np.empty((self.F, self.n), dtype=np.float64)
+        self._check(self.lib.nnpp_residuals(self.ctx, voln1rel, int(apply_bc), precision, dptr(out)))
+        return out
+
+    def poisson_apply(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.empty(self.n, dtype=np.float64)
+        self._check(self.lib.nnpp_poisson_apply(self.ctx, dptr(a), dptr(out)))
+        return out
+
+    def poisson_solve(self, precision=FP64):
+        it = C.c_int32()
+        rr = C.c_double()
+        st = self.lib.nnpp_poisson_solve(self.ctx, precision, C.byref(it), C.byref(rr))
+        if st not in (_abi.OK, _abi.ERR_NOCONV):
+            self._check(st)
+        return int(it.value), float(rr.value), st
+
+    def step(self, nsteps=1, precision=FP64, check_finite=True):
+        s = _abi.StepStats()
+        st = self.lib.nnpp_step(self.ctx, nsteps, precision, int(check_finite), C.byref(s))
+        d = s.as_dict()
+        d["status"] = st
+        if st not in (_abi.OK, _abi.ERR_NOCONV):
+            self._check(st)
+        return d
+
+    def front_update(self, threshold=None):
+        thr = self.params.front_threshold if threshold is None else threshold
+        fl = C.c_int32()
+        self._check(self.lib.nnpp_front_update(self.ctx, thr, C.byref(fl)))
+        return int(fl.value)
+
+    def state_device_ptr(self, field, slot=SLOT_N):
+        p = C.c_void_p()
+        self._check(self.lib.nnpp_state_device_ptr(self.ctx, slot, field, C.byref(p)))
+        return p.value
+
+    def stream(self):
+        p = C.c_void_p()
+        self._check(self.lib.nnpp_stream(self.ctx, C.byref(p)))
+        return p.value
+
+    def time_kernel(self, kernel, precision=FP64, reps=20):
+        """Mean duration (ms) of one launch of a hot kernel, HIP events on the context stream."""
+        ms = C.c_double()
+        self._check(self.lib.nnpp_time_kernel(self.ctx, kernel, precision, reps, C.byref(ms)))
+        return float(ms.value)

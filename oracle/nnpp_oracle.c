@@ -308,10 +308,10 @@ int or_residuals(int32_t n, const nnpp_params* p, const double* x, const double*
         const double xi = sqrt(xx * xx + yy * yy + zz * zz);
         const int pi = phase[i], pn = phase[nn];
         double D, bv;
-        if (pi == 1 && pn == 1) {
+        if (pi == 1 && pn == 1) { /* utils.h:100-116: BV = 1 on solid-solid bonds */
           if (!sb) continue;
           D = Ds;
-          bv = BV;
+          bv = 1.0;
         } else if ((pi == 1 && pn == 0) || (pi == 0 && pn == 1)) {
           D = Ds;
           bv = sb ? BV : 1.0;

@@ -1,0 +1,41 @@
+"""Write golden vectors from the reference's own code (oracle/_ref, compiled from the
+reference's grid.h/utils.h) for small synthetic cases. Run in the build container, where the
+reference sources exist; the .npz fixtures are committed so the checks also run elsewhere.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from helpers import ell_to_sorted_csr, orc, ref_chem_case  # noqa: E402
+
+
+def main():
+    R = orc.RefHarness()
+    for name, ng, jit, seed in (("reg7", 7, 0.0, 3), ("jit7", 7, 0.3, 4)):
+        case, phase, U1, Un = ref_chem_case(ng=ng, seed=seed, jitter=jit)
+        p = case.params
+        r = p.delta * (1.00 + 1e-5)
+        st, ell, ln, ml = R.build_clouds_hash(case.x, case.y, case.z, r, p.dx * 10, p.dy * 10,
+                                              p.dz * 10)
+        assert st == 0
+        rp, col = ell_to_sorted_csr(ell, ln)
+        voln1rel = 0.37
+        F = R.residuals(case, ell, ln, phase, U1, Un, voln1rel=voln1rel)
+        kr, kc, kv = R.newton_matrix(case, ell, ln, phase, U1, Un, 11, voln1rel=voln1rel)
+        beta_xi = np.linspace(0.0, 4.0 * p.dx, 41)
+        beta = np.array([R.lib.ref_beta(v, p.delta, p.dx) for v in beta_xi])
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), x=case.x, y=case.y, z=case.z,
+                            bctypes=case.bctypes, bcvalues=case.bcvalues, phase=phase, U1=U1,
+                            Un=Un, voln1rel=voln1rel, row_ptr=rp, col=col, max_len=ml,
+                            lengths=ln, F=F, K11_rows=kr, K11_cols=kc, K11_vals=kv,
+                            beta_xi=beta_xi, beta=beta)
+        print(name, case.n, int(rp[-1]), ml)
+
+
+if __name__ == "__main__":
+    main()

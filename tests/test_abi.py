@@ -1,0 +1,78 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol include/nnpp.h declares,
+its struct layouts match the Python mirrors, and the host-only entry points work."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+
+from helpers import ROOT, nn, reference_params
+
+
+def declared_symbols():
+    hdr = open(os.path.join(ROOT, "include", "nnpp.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(nnpp_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_symbol():
+    so = nn._abi.LIB_PATH
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (nnpp_\w+)", out))
+    decl = declared_symbols()
+    assert len(decl) >= 30
+    missing = [s for s in decl if s not in exported]
+    assert not missing, missing
+    assert set(decl) == set(nn._abi.SIGNATURES), "ctypes table out of sync with the header"
+
+
+def test_struct_layouts_match():
+    assert nn.lib.nnpp_abi_params_size() == C.sizeof(nn.Params)
+    assert nn.lib.nnpp_abi_stats_size() == C.sizeof(nn.StepStats)
+
+
+def test_params_defaults_and_errors():
+    p = nn.Params()
+    p.CSolid, p.CSat = 2.0, 0.5
+    assert nn.lib.nnpp_params_defaults(C.byref(p), 10, nn.CHEM_REFERENCE) == 0
+    assert p.clamp_floor == 1e-8 / 2.0 and p.front_threshold == 0.25 and p.sub_steps == 8
+    assert list(p.solid_bonds[:3]) == [1, 0, 0]
+    assert nn.lib.nnpp_params_defaults(C.byref(p), 11, nn.CHEM_REFERENCE) == nn._abi.ERR_UNSUPPORTED
+    assert nn.lib.nnpp_params_defaults(C.byref(p), 17, nn.CHEM_GENERIC) == nn._abi.ERR_ARG
+
+
+def test_job_files_roundtrip(tmp_path):
+    case = nn.synth.regular(6, S=4, seed=2)
+    job = str(tmp_path / "job" / "NNPP")
+    nn.formats.write_job(job, case)
+    back = nn.formats.read_job(job, 4)
+    assert back.n == case.n
+    for a in ("x", "y", "z", "bctypes", "bcvalues", "icvalues"):
+        assert np.array_equal(getattr(back, a), getattr(case, a)), a
+    for f in nn._abi.DATA2_FIELDS:
+        assert getattr(back.params, f) == getattr(case.params, f), f
+    for k in range(4):
+        assert np.array_equal(back.params.datac(k), case.params.datac(k))
+    assert np.array_equal(back.params.datac_phi(), case.params.datac_phi())
+
+
+def test_missing_file_is_an_error(tmp_path):
+    p = nn.Params()
+    assert nn.lib.nnpp_read_params(str(tmp_path / "nope").encode(), 10, 1, C.byref(p)) == nn._abi.ERR_IO
+    v = np.zeros(3)
+    assert nn.lib.nnpp_read_vector(str(tmp_path / "nope").encode(), 3, nn._abi.dptr(v)) == nn._abi.ERR_IO
+
+
+def test_reference_data_files_parse():
+    p = reference_params()
+    assert p.k4b == 1.0798e-2 and p.JHp == -0.5 and p.T == 298.15 and p.F == 96485.0
+
+
+def test_create_without_device_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        return
+    ctx = C.c_void_p()
+    assert nn.lib.nnpp_create(C.byref(ctx), 0) == nn._abi.ERR_HIP

@@ -1,0 +1,266 @@
+"""GPU parity of libnnpp against the CPU oracle and the reference golden vectors.
+
+Tolerances: neighbour lists bit-exact; FP64 operators (residuals, potential operator) within
+1e-12 relative (max-norm per field); FP64 multi-step trajectories within 1e-10 relative with
+both solves at rtol 1e-13; mixed precision within 1e-6 relative of FP64."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, nn, orc, rel_err, reference_params
+
+pytestmark = pytest.mark.gpu
+
+O = orc.Oracle(nn.Params)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def load_golden(name):
+    d = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    p = reference_params()
+    p.n_nodes = d["x"].size
+    case = nn.synth.Case(p, d["x"], d["y"], d["z"], None, d["bctypes"], d["bcvalues"],
+                         d["Un"].copy())
+    return d, case
+
+
+def problem(case):
+    return nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+
+
+def oracle_nb(case):
+    p = case.params
+    st, rp, col, ml = O.build_neighbors(case.x, case.y, case.z, p.delta * (1.00 + 1e-5),
+                                        p.dx * 10., p.dy * 10., p.dz * 10.)
+    assert st == 0
+    return rp, col, ml
+
+
+@pytest.mark.parametrize("name", ["reg7", "jit7"])
+def test_neighbors_bit_exact_vs_reference(name):
+    say("case", name)
+    d, case = load_golden(name)
+    with problem(case) as pb:
+        nnz, ml = pb.build_clouds_hash()
+        rp, col = pb.neighbors()
+    assert np.array_equal(rp, d["row_ptr"]) and np.array_equal(col, d["col"])
+    assert ml == int(d["max_len"])
+
+
+@pytest.mark.parametrize("kind,ng", [("regular", 24), ("regular", 17), ("screw", 20)])
+def test_neighbors_bit_exact_vs_oracle(kind, ng):
+    say("case", kind, ng)
+    case = getattr(nn.synth, kind)(ng, S=2)
+    rp_o, col_o, ml_o = oracle_nb(case)
+    with problem(case) as pb:
+        nnz, ml = pb.build_clouds_hash()
+        rp, col = pb.neighbors()
+    assert nnz == int(rp_o[-1]) and ml == ml_o
+    assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o)
+
+
+def test_neighbors_overflow_status():
+    say("case overflow")
+    case = nn.synth.regular(8, S=1)
+    with problem(case) as pb:
+        with pytest.raises(nn.NnppError) as e:
+            pb.build_clouds_hash(max_node_no=100)
+        assert e.value.code == nn._abi.ERR_OVERFLOW
+
+
+@pytest.mark.parametrize("name", ["reg7", "jit7"])
+def test_residuals_vs_reference_golden(name):
+    say("case residuals", name)
+    d, case = load_golden(name)
+    with problem(case) as pb:
+        pb.set_neighbors(d["row_ptr"], d["col"])
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.set_phase(d["phase"])
+        pb.set_state(d["Un"], nn.SLOT_N)
+        pb.set_state(d["U1"], nn.SLOT_N1)
+        F = pb.residuals(float(d["voln1rel"]))
+        Fb = pb.residuals(float(d["voln1rel"]), apply_bc=True)
+    for f in range(11):
+        assert rel_err(F[f], d["F"][f]) < 1e-12, f
+    Fo = O.bc_override(case, d["U1"], d["F"])
+    for f in range(11):
+        assert rel_err(Fb[f], Fo[f]) < 1e-12, f
+
+
+def test_residuals_generic_vs_oracle_and_mixed():
+    say("case residuals generic")
+    case = nn.synth.regular(16, S=11, seed=5)
+    rp, col, _ = oracle_nb(case)
+    rng = np.random.default_rng(9)
+    U, ph, _, _ = O.init_state(case)
+    U1 = U * rng.uniform(0.98, 1.02, size=U.shape)
+    U1[11] = rng.uniform(-0.05, 0.05, case.n)
+    U[11] = rng.uniform(-0.05, 0.05, case.n)
+    Fo = O.residuals(case, rp, col, ph, U1, U, 0.4)
+    with problem(case) as pb:
+        pb.set_neighbors(rp, col)
+        pb.set_phase(ph)
+        pb.set_state(U, nn.SLOT_N)
+        pb.set_state(U1, nn.SLOT_N1)
+        F = pb.residuals(0.4)
+        Fm = pb.residuals(0.4, precision=nn.MIXED)
+    for f in range(12):
+        assert rel_err(F[f], Fo[f]) < 1e-12, f
+        assert rel_err(Fm[f], Fo[f]) < 1e-5, f
+
+
+def test_poisson_apply_vs_oracle():
+    say("case poisson apply")
+    d, case = load_golden("jit7")
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(case.n)
+    yo = O.poisson_apply(case, d["row_ptr"], d["col"], d["phase"], v)
+    with problem(case) as pb:
+        pb.set_neighbors(d["row_ptr"], d["col"])
+        pb.set_phase(d["phase"])
+        y = pb.poisson_apply(v)
+    assert rel_err(y, yo) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [nn.FP64, nn.MIXED])
+def test_poisson_solve_vs_oracle(precision):
+    say("case poisson solve", precision)
+    case = nn.synth.regular(20, S=11, seed=2)
+    rp, col, _ = oracle_nb(case)
+    U, ph, _, _ = O.init_state(case)
+    phi_o, it_o, rr_o, st = O.poisson_solve(case, rp, col, ph, U, U[11], rtol=1e-13)
+    assert st == 0
+    with problem(case) as pb:
+        pb.set_neighbors(rp, col)
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=1e-13, maxit=100000, inner_rtol=1e-4)
+        it, rr, st = pb.poisson_solve(precision)
+        phi = pb.get_state()[11]
+    assert st == 0 and rr <= 1e-13
+    assert rel_err(phi, phi_o) < 1e-10
+    if precision == nn.FP64:
+        assert abs(it - it_o) <= 2
+
+
+def test_step_fp64_matches_oracle():
+    say("case step fp64 regular16")
+    case = nn.synth.regular(16, S=11, seed=0)
+    rp, col, _ = oracle_nb(case)
+    ref = O.run(case, rp, col, 10, rtol=1e-13)
+    with problem(case) as pb:
+        pb.build_clouds_hash()
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=1e-13, maxit=100000)
+        st = pb.step(10, nn.FP64)
+        U = pb.get_state()
+        ph = pb.get_phase()
+    assert st["status"] == 0 and st["steps"] == 10
+    assert np.array_equal(ph, ref["phase"])
+    for f in range(12):
+        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+
+
+def test_step_reference_chemistry_matches_oracle():
+    say("case step reference chemistry")
+    case = nn.synth.regular(12, S=10, seed=4)
+    p = reference_params()
+    # reference chemistry on the lattice of the synthetic case (production dx, dt)
+    case.params = p
+    s = p.dx * 12
+    case.x, case.y, case.z = case.x * s, case.y * s, case.z * s
+    case.params.n_nodes = case.n
+    rng = np.random.default_rng(5)
+    case.icvalues = 10.0 ** rng.uniform(-8, -1, size=(11, case.n))
+    case.icvalues[0][case.x > s / 2] = 0.0  # half the box solid
+    case.icvalues[10] = 0.0
+    case.bcvalues[:10][case.bcvalues[:10] != -1] = 1e-3 * p.CSolid
+    p.dt = 1.0
+    rp, col, _ = oracle_nb(case)
+    ref = O.run(case, rp, col, 5, rtol=1e-13)
+    with problem(case) as pb:
+        pb.set_neighbors(rp, col)
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=1e-13, maxit=100000)
+        st = pb.step(5, nn.FP64)
+        U = pb.get_state()
+    assert st["status"] == 0
+    for f in range(11):
+        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+
+
+def test_front_update_and_volume_cadence():
+    say("case front")
+    case = nn.synth.regular(14, S=3, seed=1)
+    case.params.front_threshold = 1.0  # about half the solid nodes flip in the first step
+    rp, col, _ = oracle_nb(case)
+    ref = O.run(case, rp, col, 9, rtol=1e-13)
+    with problem(case) as pb:
+        pb.set_neighbors(rp, col)
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=1e-13, maxit=100000)
+        st = pb.step(9, nn.FP64)
+        U = pb.get_state()
+        ph = pb.get_phase()
+    assert st["flipped"] > 0
+    assert np.array_equal(ph, ref["phase"])
+    assert abs(st["voln1rel"] - ref["voln1rel"]) <= 1e-12 * abs(ref["voln1rel"])
+    for f in range(4):
+        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+
+
+def test_mixed_step_close_to_fp64():
+    say("case mixed vs fp64")
+    case = nn.synth.regular(20, S=11, seed=3)
+    out = {}
+    for prec in (nn.FP64, nn.MIXED):
+        with problem(case) as pb:
+            pb.build_clouds_hash()
+            pb.set_bc(case.bctypes, case.bcvalues)
+            pb.init_state(case.icvalues)
+            pb.set_solver(rtol=1e-10)
+            st = pb.step(20, prec)
+            assert st["status"] == 0
+            out[prec] = pb.get_state()
+    for f in range(12):
+        assert rel_err(out[nn.MIXED][f], out[nn.FP64][f]) < 1e-6, f
+
+
+def test_deterministic_and_error_paths():
+    say("case determinism / errors")
+    case = nn.synth.regular(12, S=5, seed=6)
+    runs = []
+    for _ in range(2):
+        with problem(case) as pb:
+            pb.build_clouds_hash()
+            pb.set_bc(case.bctypes, case.bcvalues)
+            pb.init_state(case.icvalues)
+            pb.step(3, nn.FP64)
+            runs.append(pb.get_state())
+    assert np.array_equal(runs[0], runs[1])
+    with problem(case) as pb:
+        pb.build_clouds_hash()
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=1e-14, maxit=1)
+        st = pb.step(1, nn.FP64)
+        assert st["status"] == nn._abi.ERR_NOCONV
+        U = pb.get_state()
+        U[2, 5] = np.nan
+        pb.set_state(U)
+        pb.set_solver(rtol=1e-10)
+        st = pb.step(1, nn.FP64)
+        assert st["status"] == nn._abi.ERR_NONFINITE
+    with problem(case) as pb:
+        with pytest.raises(nn.NnppError) as e:
+            pb.step(1)
+        assert e.value.code == nn._abi.ERR_STATE
