@@ -149,6 +149,18 @@ int nnpp_read_vector(const char* path, int64_t n, double* out);
 int nnpp_read_ivector(const char* path, int64_t n, int32_t* out);
 int nnpp_read_nodes(const char* path, int64_t n, double* x, double* y, double* z);
 
+/* Output / restart CSV (utils.h:2053-2100 writer, utils.h:1928-2027 reader): header
+ * "X,Y,Z,<species names>,Phi,Phase", 16 significant digits, one row per node. Species names
+ * are the reference's (Mg++,H+,OH-,HCO3-,CO32-,HPO42-,PO43-,Ca++,Na+,Cl-) for S = 10 and
+ * U1..U<S> otherwise. The reader looks columns up by name; a missing column leaves zeros
+ * (the reference zeroes the state before reading, main.cpp:513-524). Unlike the reference,
+ * a missing file is NNPP_ERR_IO instead of a silent zero state. */
+int nnpp_write_output_csv(const char* path, int32_t n, int32_t n_species, const double* x,
+                          const double* y, const double* z, const double* U,
+                          const int32_t* phase);
+int nnpp_read_start_csv(const char* path, int32_t n, int32_t n_species, double* U,
+                        int32_t* phase);
+
 /* ELL adapter: turns a CSR neighbour list (rows sorted by id, no self) into the reference's
  * layout, an n x max_len column-major int matrix with self at column 0, neighbours in
  * ascending id after it and -1 padding, plus lengths that count self (grid.h:145-149,170-195,

@@ -101,6 +101,8 @@ SIGNATURES = {
     "nnpp_read_ivector": (C.c_int, [C.c_char_p, _i64, _ip]),
     "nnpp_read_nodes": (C.c_int, [C.c_char_p, _i64, _dp, _dp, _dp]),
     "nnpp_neighbors_to_ell": (C.c_int, [_i32, _lp, _ip, _i32, _ip, _ip]),
+    "nnpp_write_output_csv": (C.c_int, [C.c_char_p, _i32, _i32, _dp, _dp, _dp, _dp, _ip]),
+    "nnpp_read_start_csv": (C.c_int, [C.c_char_p, _i32, _i32, _dp, _ip]),
     "nnpp_create": (C.c_int, [C.POINTER(_p), _i32]),
     "nnpp_destroy": (C.c_int, [_p]),
     "nnpp_last_error": (C.c_char_p, [_p]),

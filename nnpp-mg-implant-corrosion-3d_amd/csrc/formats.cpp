@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -24,9 +26,71 @@ bool read_doubles(const std::string& path, int64_t n, double* out) {
   return true;
 }
 
+std::vector<std::string> column_names(int S) {
+  static const char* ref[10] = {"Mg++", "H+", "OH-", "HCO3-", "CO32-",
+                                "HPO42-", "PO43-", "Ca++", "Na+", "Cl-"};
+  std::vector<std::string> c;
+  for (int k = 0; k < S; k++) c.push_back(S == 10 ? ref[k] : "U" + std::to_string(k + 1));
+  c.push_back("Phi");
+  return c;
+}
+
 }  // namespace
 
 extern "C" {
+
+int nnpp_write_output_csv(const char* path, int32_t n, int32_t S, const double* x,
+                          const double* y, const double* z, const double* U,
+                          const int32_t* phase) {
+  if (!path || n < 0 || S < 1 || S > NNPP_MAX_SPECIES || (n && (!x || !y || !z || !U || !phase)))
+    return NNPP_ERR_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return NNPP_ERR_IO;
+  const std::vector<std::string> names = column_names(S);
+  std::fputs("X,Y,Z", f);
+  for (const std::string& c : names) std::fprintf(f, ",%s", c.c_str());
+  std::fputs(",Phase\n", f);
+  for (int32_t i = 0; i < n; i++) {  // precision 16, default float format (utils.h:2088-2096)
+    std::fprintf(f, "%.16g,%.16g,%.16g", x[i], y[i], z[i]);
+    for (int fld = 0; fld <= S; fld++) std::fprintf(f, ",%.16g", U[(int64_t)fld * n + i]);
+    std::fprintf(f, ",%d\n", phase[i]);
+  }
+  return std::fclose(f) == 0 ? NNPP_OK : NNPP_ERR_IO;
+}
+
+int nnpp_read_start_csv(const char* path, int32_t n, int32_t S, double* U, int32_t* phase) {
+  if (!path || n < 0 || S < 1 || S > NNPP_MAX_SPECIES || (n && (!U || !phase)))
+    return NNPP_ERR_ARG;
+  std::ifstream file(path);
+  if (!file) return NNPP_ERR_IO;
+  for (int64_t k = 0; k < (int64_t)(S + 1) * n; k++) U[k] = 0.0;
+  for (int32_t i = 0; i < n; i++) phase[i] = 0;
+  std::string line, col;
+  if (!std::getline(file, line)) return NNPP_ERR_IO;
+  std::map<std::string, size_t> pos;
+  {
+    std::istringstream h(line);
+    size_t idx = 0;
+    while (std::getline(h, col, ',')) pos[col] = idx++;
+  }
+  const std::vector<std::string> names = column_names(S);
+  int32_t row = 0;
+  std::vector<std::string> data;
+  while (row < n && std::getline(file, line)) {
+    std::istringstream ss(line);
+    data.clear();
+    while (std::getline(ss, col, ',')) data.push_back(col);
+    for (int fld = 0; fld <= S; fld++) {
+      auto it = pos.find(names[fld]);
+      if (it != pos.end() && it->second < data.size())
+        U[(int64_t)fld * n + row] = std::stod(data[it->second]);
+    }
+    auto it = pos.find("Phase");
+    if (it != pos.end() && it->second < data.size()) phase[row] = std::stoi(data[it->second]);
+    row++;
+  }
+  return row == n ? NNPP_OK : NNPP_ERR_IO;
+}
 
 int64_t nnpp_abi_params_size(void) { return (int64_t)sizeof(nnpp_params); }
 int64_t nnpp_abi_stats_size(void) { return (int64_t)sizeof(nnpp_step_stats); }
