@@ -92,6 +92,36 @@ def cpu_baseline(nn, case, rp, col, budget_s=20.0):
             "flux_s": t_flux, "pot_apply_s": t_pot / max(reps, 1)}
 
 
+def kernels_only(nn, case, reps):
+    """Time the hot kernels alone (for rocprofv3 --pmc runs): one mixed step first so the
+    solver vectors hold real data, then `reps` launches of each kernel."""
+    pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+    nnz, _ = pb.build_clouds_hash()
+    pb.set_bc(case.bctypes, case.bcvalues)
+    pb.init_state(case.icvalues)
+    st = pb.step(1, nn.MIXED)
+    out = {"nodes": case.n, "bonds": nnz, "liquid_bonds": st["nnz_ll"]}
+    for name, k, prec in (("pot_spmv_fp32", nn._abi.KERNEL_POT_SPMV, nn.MIXED),
+                          ("pot_spmv_fp64", nn._abi.KERNEL_POT_SPMV, nn.FP64),
+                          ("flux_fp64", nn._abi.KERNEL_FLUX, nn.FP64),
+                          ("flux_mixed", nn._abi.KERNEL_FLUX, nn.MIXED)):
+        out[name + "_ms"] = pb.time_kernel(k, prec, reps)
+    pb.close()
+    print(json.dumps(out))
+
+
+def copy_probe(mib):
+    """A plain device copy of known size, to scale FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(mib * 2 ** 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4,
+                      "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,13 +131,23 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only time the hot kernels (for counter runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (counter scale)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     import torch
     nn = ge.load_package()
     ng = a.grid
+    if a.kernels_only:
+        kernels_only(nn, nn.synth.regular(ng, S=11, seed=0), 20)
+        return
     log(f"[bench] building {ng}^3 case")
     case = nn.synth.regular(ng, S=11, seed=0)
     # neighbour list built on the GPU once, shared by both runs

@@ -140,7 +140,10 @@ int nnpp_params_defaults(nnpp_params* p, int32_t n_species, int32_t chemistry);
 
 /* Read <job>.data1, <job>.data2, <job>.datac1 .. <job>.datac<S+1> into *p and apply
  * nnpp_params_defaults. Replaces main.cpp:210-231 + utils::parameter::set_int/set_double
- * (utils.h:2334-2484). Unlike the reference, a missing file is an error (NNPP_ERR_IO). */
+ * (utils.h:2334-2484). Unlike the reference, a missing file is an error (NNPP_ERR_IO).
+ * Then, if <job>.nnpp exists, its "key = value" lines set the extension fields that the
+ * reference files cannot hold: r_diss, front_threshold, clamp_floor, phase_eps, sub_steps,
+ * solid_bonds (S integers), clamp (S integers); '#' starts a comment. */
 int nnpp_read_params(const char* job, int32_t n_species, int32_t chemistry, nnpp_params* p);
 
 /* Whitespace-separated readers; replace utils::read_vector_from_file and

@@ -168,7 +168,35 @@ int nnpp_read_params(const char* job, int32_t n_species, int32_t chemistry, nnpp
   p->sigma1 = dp[1];
   p->s_phi = dp[2];
   p->K_phi = dp[3];
-  return nnpp_params_defaults(p, n_species, chemistry);
+  int st = nnpp_params_defaults(p, n_species, chemistry);
+  if (st) return st;
+  std::ifstream ext((J + ".nnpp").c_str());
+  if (!ext) return NNPP_OK;
+  std::string line;
+  while (std::getline(ext, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    const size_t eq = line.find('=');
+    if (eq == std::string::npos) continue;
+    std::string key = line.substr(0, eq);
+    key.erase(0, key.find_first_not_of(" \t"));
+    key.erase(key.find_last_not_of(" \t") + 1);
+    std::istringstream val(line.substr(eq + 1));
+    if (key == "r_diss") val >> p->r_diss;
+    else if (key == "front_threshold") val >> p->front_threshold;
+    else if (key == "clamp_floor") val >> p->clamp_floor;
+    else if (key == "phase_eps") val >> p->phase_eps;
+    else if (key == "sub_steps") val >> p->sub_steps;
+    else if (key == "solid_bonds" || key == "clamp") {
+      int32_t* dst = key == "clamp" ? p->clamp : p->solid_bonds;
+      for (int k = 0; k < n_species; k++)
+        if (!(val >> dst[k])) return NNPP_ERR_IO;
+    } else if (!key.empty()) {
+      return NNPP_ERR_IO;  // unknown key: refuse rather than silently ignore
+    }
+    if (val.fail() && key != "solid_bonds" && key != "clamp") return NNPP_ERR_IO;
+  }
+  return NNPP_OK;
 }
 
 int nnpp_neighbors_to_ell(int32_t n, const int64_t* row_ptr, const int32_t* col,

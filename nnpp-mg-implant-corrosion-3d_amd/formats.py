@@ -45,6 +45,12 @@ def write_job(job, case):
         np.savetxt(job + f".icvalues{f_ + 1}", case.icvalues[f_], fmt="%.17g")
     if case.vol is not None:
         np.savetxt(job + ".volumes", case.vol, fmt="%.17g")
+    with open(job + ".nnpp", "w") as f:  # extension fields the reference files cannot hold
+        f.write(f"r_diss = {_fmt(p.r_diss)}\nfront_threshold = {_fmt(p.front_threshold)}\n"
+                f"clamp_floor = {_fmt(p.clamp_floor)}\nphase_eps = {_fmt(p.phase_eps)}\n"
+                f"sub_steps = {int(p.sub_steps)}\n"
+                f"solid_bonds = {' '.join(str(int(v)) for v in p.solid_bonds[:S])}\n"
+                f"clamp = {' '.join(str(int(v)) for v in p.clamp[:S])}\n")
 
 
 def read_job(job, n_species, chemistry=_abi.CHEM_GENERIC):

@@ -51,8 +51,11 @@ def test_job_files_roundtrip(tmp_path):
     assert back.n == case.n
     for a in ("x", "y", "z", "bctypes", "bcvalues", "icvalues"):
         assert np.array_equal(getattr(back, a), getattr(case, a)), a
-    for f in nn._abi.DATA2_FIELDS:
+    for f in nn._abi.DATA2_FIELDS + ["r_diss", "front_threshold", "clamp_floor", "phase_eps",
+                                     "sub_steps"]:
         assert getattr(back.params, f) == getattr(case.params, f), f
+    assert list(back.params.clamp) == list(case.params.clamp)
+    assert list(back.params.solid_bonds[:4]) == list(case.params.solid_bonds[:4])
     for k in range(4):
         assert np.array_equal(back.params.datac(k), case.params.datac(k))
     assert np.array_equal(back.params.datac_phi(), case.params.datac_phi())
@@ -76,3 +79,32 @@ def test_create_without_device_fails_cleanly():
         return
     ctx = C.c_void_p()
     assert nn.lib.nnpp_create(C.byref(ctx), 0) == nn._abi.ERR_HIP
+
+
+def test_output_csv_roundtrip_and_header(tmp_path):
+    rng = np.random.default_rng(0)
+    for S in (10, 3):
+        n = 17
+        x, y, z = rng.uniform(size=(3, n))
+        U = rng.uniform(-1, 1, size=(S + 1, n))
+        ph = rng.integers(0, 2, n).astype(np.int32)
+        path = str(tmp_path / f"o{S}.csv")
+        assert nn.lib.nnpp_write_output_csv(path.encode(), n, S, nn._abi.dptr(x), nn._abi.dptr(y),
+                                            nn._abi.dptr(z), nn._abi.dptr(U), nn._abi.iptr(ph)) == 0
+        head = open(path).readline().strip()
+        if S == 10:  # utils.h:2083-2086 column names
+            assert head == "X,Y,Z,Mg++,H+,OH-,HCO3-,CO32-,HPO42-,PO43-,Ca++,Na+,Cl-,Phi,Phase"
+        U2 = np.zeros_like(U)
+        ph2 = np.zeros_like(ph)
+        assert nn.lib.nnpp_read_start_csv(path.encode(), n, S, nn._abi.dptr(U2),
+                                          nn._abi.iptr(ph2)) == 0
+        assert np.array_equal(ph2, ph)
+        assert np.abs(U2 - U).max() <= 1e-15 * np.abs(U).max()  # 16 significant digits
+
+
+def test_driver_usage_and_missing_job(tmp_path):
+    exe = os.path.join(ROOT, "nnpp-mg-implant-corrosion-3d_amd", "bin", "nnpp_main")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0 and "Usage" in r.stdout
+    r = subprocess.run([exe, "-01", str(tmp_path / "none"), "0"], capture_output=True, text=True)
+    assert r.returncode == 1 and "reading parameters" in r.stderr

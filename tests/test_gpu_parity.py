@@ -258,9 +258,35 @@ def test_deterministic_and_error_paths():
         U[2, 5] = np.nan
         pb.set_state(U)
         pb.set_solver(rtol=1e-10)
-        st = pb.step(1, nn.FP64)
-        assert st["status"] == nn._abi.ERR_NONFINITE
+        with pytest.raises(nn.NnppError) as e:
+            pb.step(1, nn.FP64)
+        assert e.value.code == nn._abi.ERR_NONFINITE
     with problem(case) as pb:
         with pytest.raises(nn.NnppError) as e:
             pb.step(1)
         assert e.value.code == nn._abi.ERR_STATE
+
+
+def test_cpp_driver_end_to_end(tmp_path):
+    """host/nnpp_main on reference-format job files vs the oracle's trajectory."""
+    import subprocess
+    say("case nnpp_main driver")
+    case = nn.synth.regular(10, S=11, seed=8)
+    case.params.n_steps = 16
+    job = str(tmp_path / "NNPP")
+    nn.formats.write_job(job, case)
+    exe = os.path.join(os.path.dirname(nn._abi.LIB_PATH), "bin", "nnpp_main")
+    r = subprocess.run([exe, "-01", job, "0", "--species", "11", "--rtol", "1e-13"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rp, col, _ = oracle_nb(case)
+    ref = O.run(case, rp, col, 16, rtol=1e-13)
+    U = np.zeros((12, case.n))
+    ph = np.zeros(case.n, dtype=np.int32)
+    assert nn.lib.nnpp_read_start_csv((job + "-01-16.output.csv").encode(), case.n, 11,
+                                      nn._abi.dptr(U), nn._abi.iptr(ph)) == 0
+    assert np.array_equal(ph, ref["phase"])
+    for f in range(12):
+        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+    vol = open(job + "-01.volume").read().split()
+    assert len(vol) == 2 and abs(float(vol[-1].rstrip(",")) - ref["voln1rel"]) <= 1e-12 * abs(ref["voln1rel"])
