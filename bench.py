@@ -187,10 +187,12 @@ def main():
     t_spmv64 = pbm.time_kernel(nn._abi.KERNEL_POT_SPMV, nn.FP64, reps)
     t_flux64 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.FP64, 10)
     t_flux32 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.MIXED, 10)
-    # algorithmic bytes per launch: col (4) + weight per liquid bond; per node read p_i, dd_i,
-    # write q_i (the gathered p_j counted once per node)
-    bytes32 = nll * (4 + 4) + n * (4 * 4)
-    bytes64 = nll * (4 + 8) + n * (4 * 8)
+    # algorithmic bytes per launch of the potential SpMV: a 16-bit column step + the weight per
+    # liquid bond; per node the first column (4), p_i (gathered p_j counted once), dd_i, pinv
+    # not read, q_i written
+    cb = 4 if mixed["stats"].get("wide", 0) else 2
+    bytes32 = nll * (cb + 4) + n * (4 + 3 * 4)
+    bytes64 = nll * (cb + 8) + n * (4 + 3 * 8)
     S = case.S
     fbytes = lambda wb: nnz * (4 + wb) + n * (8 * (S + 1) + 1 + 2 + 8 * S)  # noqa: E731
     ach = bytes32 / (t_spmv32 * 1e-3) / 1e9

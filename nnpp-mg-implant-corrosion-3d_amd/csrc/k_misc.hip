@@ -61,10 +61,10 @@ __global__ void k_slice_len(int32_t n, const int64_t* __restrict__ row_ptr,
   atomicMax(&len[i / NNPP_WAVE], (int32_t)(row_ptr[i + 1] - row_ptr[i]));
 }
 
-__global__ void k_times_wave(int64_t ns, const int32_t* __restrict__ len,
-                             int32_t* __restrict__ out) {
+__global__ void k_times(int64_t ns, const int32_t* __restrict__ len, int32_t mult,
+                        int32_t* __restrict__ out) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ns) out[c] = len[c] * NNPP_WAVE;
+  if (c < ns) out[c] = len[c] * mult;
 }
 
 struct WeightArgs {
@@ -142,29 +142,34 @@ __global__ void k_pot_len(int32_t n, const int64_t* __restrict__ row_ptr,
       if (phase[j] == 0 && pfree[j]) cnt++;
     }
   rcount[i] = cnt;
-  atomicMax(&len[i / NNPP_WAVE], cnt);
+  atomicMax(&len[i / NNPP_WAVE], (cnt + 3) / 4);  // groups of 4 entries per lane
 }
 
-// Step 3: entries, scaled diagonal, Dirichlet-neighbour diagonal part and the Dirichlet part of
-// the right-hand side. Row i is scaled by V_i / V0 (symmetrises the operator for per-node
-// volumes; 1 for the reference's constant volume).
+// Step 3: entries (quad layout), scaled diagonal, Dirichlet-neighbour diagonal part and the
+// Dirichlet part of the right-hand side. Row i is scaled by V_i / V0 (symmetrises the operator
+// for per-node volumes; 1 for the reference's constant volume). Column steps that do not fit
+// 16 bits raise *maxstep above 65535 (the wide format is used then).
 __global__ void k_pot_fill(int32_t n, int64_t nrows_pad, int S, const int64_t* __restrict__ row_ptr,
                            const int32_t* __restrict__ col, const uint8_t* __restrict__ phase,
                            const uint8_t* __restrict__ pfree, const uint16_t* __restrict__ bcmask,
                            const double* __restrict__ bcphi, const double* __restrict__ x,
                            const double* __restrict__ y, const double* __restrict__ z,
                            WeightArgs wa, const int64_t* __restrict__ off,
-                           const int32_t* __restrict__ len, int32_t* __restrict__ scol,
+                           const int32_t* __restrict__ len, int32_t* __restrict__ first,
+                           int32_t* __restrict__ scol, uint16_t* __restrict__ d16,
                            double* __restrict__ w64, float* __restrict__ w32,
                            double* __restrict__ pdd, float* __restrict__ pdd32,
                            double* __restrict__ pinv, float* __restrict__ pinv32,
-                           double* __restrict__ pbdir, double* __restrict__ psc) {
+                           double* __restrict__ pbdir, double* __restrict__ psc,
+                           int* __restrict__ maxstep) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows_pad) return;
   const int64_t c = i / NNPP_WAVE;
-  const int64_t base = off[c] + (i % NNPP_WAVE);
-  const int L = len[c];
+  const int64_t base = off[c] + 4 * (i % NNPP_WAVE);
+  const int L = 4 * len[c];
   int t = 0;
+  int32_t prev = (int32_t)(i < n ? i : 0), f0 = prev;
+  int big = 0;
   if (i < n) {
     const double sc = wa.vol ? wa.vol[i] / wa.V0 : 1.0;
     double diag = 0.0, ddir = 0.0, bdir = 0.0;
@@ -175,10 +180,15 @@ __global__ void k_pot_fill(int32_t n, int64_t nrows_pad, int S, const int64_t* _
         const double w = bond_weight(wa, x, y, z, i, j);
         diag += w;
         if (pfree[j]) {
-          const int64_t slot = base + (int64_t)t * NNPP_WAVE;
+          const int64_t slot = base + 256 * (int64_t)(t >> 2) + (t & 3);
+          if (t == 0) f0 = prev = j;
+          const int step = j - prev;  // rows are sorted: step >= 0
+          big = step > big ? step : big;
           scol[slot] = j;
+          d16[slot] = (uint16_t)(step & 0xffff);
           w64[slot] = sc * w;
           w32[slot] = (float)(sc * w);
+          prev = j;
           t++;
         } else {  // a liquid neighbour of a free node is free or Dirichlet
           ddir += w;
@@ -186,6 +196,7 @@ __global__ void k_pot_fill(int32_t n, int64_t nrows_pad, int S, const int64_t* _
         }
       }
     }
+    first[i] = f0;
     pdd[i] = sc * ddir;
     pdd32[i] = (float)(sc * ddir);
     pinv[i] = pfree[i] ? 1.0 / (sc * diag) : 0.0;
@@ -193,12 +204,14 @@ __global__ void k_pot_fill(int32_t n, int64_t nrows_pad, int S, const int64_t* _
     pbdir[i] = bdir;
     psc[i] = sc;
   }
-  for (; t < L; t++) {
-    const int64_t slot = base + (int64_t)t * NNPP_WAVE;
-    scol[slot] = -1;
+  for (; t < L; t++) {  // padding: weight 0, column unchanged
+    const int64_t slot = base + 256 * (int64_t)(t >> 2) + (t & 3);
+    scol[slot] = prev;
+    d16[slot] = 0;
     w64[slot] = 0.0;
     w32[slot] = 0.0f;
   }
+  if (big > 65535) atomicMax(maxstep, big);
 }
 
 // main.cpp:915-922 front update, then the volume sum of main.cpp:944-949 over the new phase.
@@ -206,19 +219,19 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_front(int32_t n, const double* _
                                                       uint8_t* __restrict__ phase, double thr,
                                                       double sat, double* part, unsigned* ticket,
                                                       double* scal) {
-  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK) {
     uint8_t ph = phase[i];
     const double u = U0[i];
     if (ph == 1 && u <= thr) {
       ph = 0;
       phase[i] = 0;
-      v[0] = 1.0;
+      v[0] += 1.0;
     }
     if (ph == 1) {
-      v[1] = (u - sat);
-      v[2] = 1.0;
+      v[1] += (u - sat);
+      v[2] += 1.0;
     }
   }
   if (nnpp_grid_sum<3>(v, part, ticket, gridDim.x, blockIdx.x)) {
@@ -324,13 +337,16 @@ static int slices_reserve(nnpp_ctx* c, nnpp_slices& sl, int64_t slots) {
     if (sl.col) hipFree(sl.col);
     if (sl.w64) hipFree(sl.w64);
     if (sl.w32) hipFree(sl.w32);
+    if (sl.d16) hipFree(sl.d16);
     sl.col = nullptr;
     sl.w64 = nullptr;
     sl.w32 = nullptr;
-    const int64_t cap = slots + slots / 16 + 64;
+    sl.d16 = nullptr;
+    const int64_t cap = slots + slots / 16 + 256;
     NNPP_HIP(c, hipMalloc((void**)&sl.col, sizeof(int32_t) * cap));
     NNPP_HIP(c, hipMalloc((void**)&sl.w64, sizeof(double) * cap));
     NNPP_HIP(c, hipMalloc((void**)&sl.w32, sizeof(float) * cap));
+    NNPP_HIP(c, hipMalloc((void**)&sl.d16, sizeof(uint16_t) * cap));
     sl.cap_slots = cap;
   }
   sl.slots = slots;
@@ -338,10 +354,11 @@ static int slices_reserve(nnpp_ctx* c, nnpp_slices& sl, int64_t slots) {
 }
 
 // Slice offsets from per-slice lengths: off = exclusive scan of 64 * len.
-static int slices_offsets(nnpp_ctx* c, nnpp_slices& sl) {
+static int slices_offsets(nnpp_ctx* c, nnpp_slices& sl, int32_t entries_per_len) {
   int32_t* tmp = nullptr;
   NNPP_HIP(c, hipMalloc((void**)&tmp, sizeof(int32_t) * (sl.nslice ? sl.nslice : 1)));
-  k_times_wave<<<nnpp_nblocks(sl.nslice ? sl.nslice : 1), NNPP_BLOCK, 0, c->stream>>>(sl.nslice, sl.len, tmp);
+  k_times<<<nnpp_nblocks(sl.nslice ? sl.nslice : 1), NNPP_BLOCK, 0, c->stream>>>(
+      sl.nslice, sl.len, entries_per_len, tmp);
   int st = scan_impl<int32_t>(c, tmp, sl.nslice, sl.off);
   hipFree(tmp);
   if (st) return st;
@@ -369,7 +386,7 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
   NNPP_HIP(c, hipMemsetAsync(c->bonds.len, 0, sizeof(int32_t) * (ns ? ns : 1), s));
   if (n) k_slice_len<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->row_ptr, c->bonds.len);
   NNPP_HIP(c, hipGetLastError());
-  st = slices_offsets(c, c->bonds);
+  st = slices_offsets(c, c->bonds, NNPP_WAVE);
   if (st) return st;
   const int64_t npad = ns * NNPP_WAVE;
   if (npad)
@@ -400,19 +417,26 @@ int nnpp_build_potential(nnpp_ctx* c) {
                                                      c->pot.len, rcount);
   }
   NNPP_HIP(c, hipGetLastError());
-  st = slices_offsets(c, c->pot);
+  st = slices_offsets(c, c->pot, 4 * NNPP_WAVE);
   if (st) {
     hipFree(rcount);
     return st;
   }
+  if (!c->pot.first) NNPP_HIP(c, hipMalloc((void**)&c->pot.first, sizeof(int32_t) * (n ? n : 1)));
+  int* dmax = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&dmax, sizeof(int)));
+  NNPP_HIP(c, hipMemsetAsync(dmax, 0, sizeof(int), s));
   const int64_t npad = ns * NNPP_WAVE;
   const double expo = 3.0 + 2.0 * c->p.s_phi;
   if (npad)
     k_pot_fill<<<nnpp_nblocks(npad), NNPP_BLOCK, 0, s>>>(
         n, npad, S, c->row_ptr, c->col, c->phase, c->pfree, c->bcmask, c->bcval + (int64_t)S * n,
-        c->x, c->y, c->z, weight_args(c, expo), c->pot.off, c->pot.len, c->pot.col, c->pot.w64,
-        c->pot.w32, c->pdd, c->pdd32, c->pinv, c->pinv32, c->pbdir, c->psc);
+        c->x, c->y, c->z, weight_args(c, expo), c->pot.off, c->pot.len, c->pot.first,
+        c->pot.col, c->pot.d16, c->pot.w64, c->pot.w32, c->pdd, c->pdd32, c->pinv, c->pinv32,
+        c->pbdir, c->psc, dmax);
   NNPP_HIP(c, hipGetLastError());
+  int hmax = 0;
+  NNPP_HIP(c, hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, s));
   // count real entries and free rows
   std::vector<int32_t> rc(n);
   std::vector<uint8_t> fr(n);
@@ -422,6 +446,8 @@ int nnpp_build_potential(nnpp_ctx* c) {
   }
   NNPP_HIP(c, hipStreamSynchronize(s));
   hipFree(rcount);
+  hipFree(dmax);
+  c->pot.wide = hmax > 65535 ? 1 : 0;
   int64_t nnz = 0, nf = 0;
   for (int32_t i = 0; i < n; i++) {
     nnz += rc[i];
@@ -439,7 +465,8 @@ int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flip
   const int32_t n = c->n;
   hipStream_t s = c->stream;
   if (n) {
-    k_front<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, U, c->phase, thr, c->p.CSat / c->p.CSolid,
+    k_front<<<nnpp_red_grid(nnpp_nblocks(n)), NNPP_BLOCK, 0, s>>>(n, U, c->phase, thr,
+                                                                  c->p.CSat / c->p.CSolid,
                                                    c->red_part, c->red_ticket, c->scal);
     NNPP_HIP(c, hipGetLastError());
     NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT,
@@ -466,8 +493,7 @@ int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad) {
   const int64_t m = (int64_t)c->n * (c->p.n_species + 1);
   *bad = 0;
   if (!m) return NNPP_OK;
-  int64_t nb = nnpp_nblocks(m);
-  if (nb > 4096) nb = 4096;
+  const int64_t nb = nnpp_red_grid(nnpp_nblocks(m));
   k_nonfinite<<<nb, NNPP_BLOCK, 0, c->stream>>>(m, U, c->red_part, c->red_ticket, c->scal);
   NNPP_HIP(c, hipGetLastError());
   NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost,

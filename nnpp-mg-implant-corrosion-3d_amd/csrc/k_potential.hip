@@ -12,9 +12,15 @@
 // FP64: Jacobi-preconditioned CG, three kernels per iteration (SpMV + dot, update + dots,
 // direction update); all scalars stay on the device and a converged flag makes the remaining
 // launches of a batch exit at once, so the host synchronises once per batch.
-// MIXED: iterative refinement; the residual b - M x is formed in FP64 (FP64 weights and
-// vectors), the correction solve runs in FP32 (FP32 weights and vectors, Kahan-compensated
-// bond sums, FP64 dot products), the correction is added in FP64.
+// MIXED: CG with reliable updates. The iteration runs in FP32 (FP32 weights and vectors,
+// Kahan-compensated bond sums, FP64 dot products) on the residual normalised by ||b||; the
+// solution is kept in FP64. Whenever the FP32 recursive residual has dropped by a factor 100
+// (or reached the target), the FP32 correction is added to the FP64 solution, the residual is
+// recomputed in FP64 (FP64 weights and vectors) and replaces the FP32 one, and the iteration
+// continues with the same search direction. The FP64 residual is the stopping test.
+//
+// Every kernel with a reduction runs on at most NNPP_RED_GRID blocks that stride over the node
+// blocks (nnpp_blocks_of), so the last-block sum handles at most 2048 partials.
 #include <cmath>
 
 #include "nnpp_internal.hpp"
@@ -25,45 +31,86 @@ constexpr int CG_BATCH = 32;
 
 struct Slices {
   const int64_t* off;
-  const int32_t* len;
+  const int32_t* len;  // groups of 4 entries per lane
   const int32_t* col;
+  const uint16_t* d16;
+  const int32_t* first;
 };
 
 __device__ __forceinline__ bool cg_done(const double* scal) {
   return nnpp_ld_coherent(scal + SC_DONE) != 0.0;
 }
 
+// Columns of the next 4 entries of a row: one 16-byte load (wide) or one 8-byte load of
+// 16-bit steps accumulated onto the running column j.
+template <bool WIDE>
+__device__ __forceinline__ void quad_cols(const Slices& sl, int64_t s0, int32_t& j, int32_t (&c)[4]) {
+  if (WIDE) {
+    const int4 q = *reinterpret_cast<const int4*>(sl.col + s0);
+    c[0] = q.x;
+    c[1] = q.y;
+    c[2] = q.z;
+    c[3] = q.w;
+  } else {
+    const uint2 d = *reinterpret_cast<const uint2*>(sl.d16 + s0);
+    c[0] = j + (int32_t)(d.x & 0xffffu);
+    c[1] = c[0] + (int32_t)(d.x >> 16);
+    c[2] = c[1] + (int32_t)(d.y & 0xffffu);
+    c[3] = c[2] + (int32_t)(d.y >> 16);
+    j = c[3];
+  }
+}
+
+template <bool WIDE>
 __device__ __forceinline__ double spmv64_row(const Slices& sl, const double* __restrict__ w,
                                              const double* __restrict__ dd,
                                              const double* __restrict__ v, int64_t i) {
-  const int64_t base = sl.off[i / NNPP_WAVE] + (i % NNPP_WAVE);
-  const int L = sl.len[i / NNPP_WAVE];
+  const int64_t base = sl.off[i / NNPP_WAVE] + 4 * (i % NNPP_WAVE);
+  const int G = sl.len[i / NNPP_WAVE];
   const double vi = v[i];
+  int32_t j = WIDE ? 0 : sl.first[i];
   double acc = 0.0;
-  for (int t = 0; t < L; t++) {
-    const int64_t slot = base + (int64_t)t * NNPP_WAVE;
-    const int32_t j = sl.col[slot];
-    if (j < 0) break;
-    acc += w[slot] * (vi - v[j]);
+  for (int g = 0; g < G; g++) {
+    const int64_t s0 = base + 256 * (int64_t)g;
+    int32_t c[4];
+    quad_cols<WIDE>(sl, s0, j, c);
+    const double2 wa = *reinterpret_cast<const double2*>(w + s0);
+    const double2 wb = *reinterpret_cast<const double2*>(w + s0 + 2);
+    const double p0 = v[c[0]], p1 = v[c[1]], p2 = v[c[2]], p3 = v[c[3]];
+    acc += wa.x * (vi - p0);
+    acc += wa.y * (vi - p1);
+    acc += wb.x * (vi - p2);
+    acc += wb.y * (vi - p3);
   }
   return dd[i] * vi + acc;
 }
 
+__device__ __forceinline__ void kahan(float& s, float& cmp, float v) {
+  const float y = v - cmp;
+  const float t = s + y;
+  cmp = (t - s) - y;
+  s = t;
+}
+
+template <bool WIDE>
 __device__ __forceinline__ float spmv32_row(const Slices& sl, const float* __restrict__ w,
                                             const float* __restrict__ dd,
                                             const float* __restrict__ v, int64_t i) {
-  const int64_t base = sl.off[i / NNPP_WAVE] + (i % NNPP_WAVE);
-  const int L = sl.len[i / NNPP_WAVE];
+  const int64_t base = sl.off[i / NNPP_WAVE] + 4 * (i % NNPP_WAVE);
+  const int G = sl.len[i / NNPP_WAVE];
   const float vi = v[i];
+  int32_t j = WIDE ? 0 : sl.first[i];
   float acc = 0.0f, cmp = 0.0f;
-  for (int t = 0; t < L; t++) {
-    const int64_t slot = base + (int64_t)t * NNPP_WAVE;
-    const int32_t j = sl.col[slot];
-    if (j < 0) break;
-    const float y = w[slot] * (vi - v[j]) - cmp;
-    const float s = acc + y;
-    cmp = (s - acc) - y;
-    acc = s;
+  for (int g = 0; g < G; g++) {
+    const int64_t s0 = base + 256 * (int64_t)g;
+    int32_t c[4];
+    quad_cols<WIDE>(sl, s0, j, c);
+    const float4 wq = *reinterpret_cast<const float4*>(w + s0);
+    const float p0 = v[c[0]], p1 = v[c[1]], p2 = v[c[2]], p3 = v[c[3]];
+    kahan(acc, cmp, wq.x * (vi - p0));
+    kahan(acc, cmp, wq.y * (vi - p1));
+    kahan(acc, cmp, wq.z * (vi - p2));
+    kahan(acc, cmp, wq.w * (vi - p3));
   }
   return dd[i] * vi + acc;
 }
@@ -77,9 +124,11 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
                                                     const double* __restrict__ phi,
                                                     double* __restrict__ b, double* __restrict__ x,
                                                     double* part, unsigned* ticket, double* scal) {
-  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
   double v[1] = {0.0};
-  if (i < n) {
+  const int64_t nb = nnpp_nblocks(n);
+  for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
+    const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    if (i >= n) continue;
     double bi = 0.0, xi = 0.0;
     if (pfree[i]) {
       double q = 0.0;
@@ -89,13 +138,14 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
     }
     b[i] = bi;
     x[i] = xi;
-    v[0] = bi * bi;
+    v[0] += bi * bi;
   }
   if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) scal[SC_BNORM2] = v[0];
 }
 
-// r = b - M x in FP64. START: also z = D^-1 r, p = z, rho, done flag (target from ||b||).
-template <bool START>
+// r = b - M x in FP64 (RR = ||r||^2). START: also z = D^-1 r, p = z, rho, the target from
+// ||b|| and the converged flag.
+template <bool START, bool WIDE>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
                                                       const double* __restrict__ w,
                                                       const double* __restrict__ dd,
@@ -106,17 +156,19 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
                                                       double* __restrict__ pv, double rtol2,
                                                       double* part, unsigned* ticket,
                                                       double* scal) {
-  const int64_t i = nnpp_xcd_block(blockIdx.x, gridDim.x) * NNPP_BLOCK + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (i < n) {
-    const double ri = (pinv[i] != 0.0) ? b[i] - spmv64_row(sl, w, dd, x, i) : 0.0;
+  const int64_t nb = nnpp_nblocks(n);
+  for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
+    const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    if (i >= n) continue;
+    const double ri = (pinv[i] != 0.0) ? b[i] - spmv64_row<WIDE>(sl, w, dd, x, i) : 0.0;
     r[i] = ri;
-    v[1] = ri * ri;
+    v[1] += ri * ri;
     if (START) {
       const double zi = ri * pinv[i];
       zv[i] = zi;
       pv[i] = zi;
-      v[0] = ri * zi;
+      v[0] += ri * zi;
     }
   }
   if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
@@ -131,6 +183,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
   }
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
                                                         const double* __restrict__ w,
                                                         const double* __restrict__ dd,
@@ -138,12 +191,14 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
                                                         double* __restrict__ q, double* part,
                                                         unsigned* ticket, double* scal) {
   if (cg_done(scal)) return;
-  const int64_t i = nnpp_xcd_block(blockIdx.x, gridDim.x) * NNPP_BLOCK + threadIdx.x;
   double v[1] = {0.0};
-  if (i < n) {
-    const double qi = spmv64_row(sl, w, dd, p, i);
+  const int64_t nb = nnpp_nblocks(n);
+  for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
+    const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    if (i >= n) continue;
+    const double qi = spmv64_row<WIDE>(sl, w, dd, p, i);
     q[i] = qi;
-    v[0] = p[i] * qi;
+    v[0] += p[i] * qi;
   }
   if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) {
     scal[SC_PQ] = v[0];
@@ -160,16 +215,16 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_update(int32_t n, const doubl
                                                           unsigned* ticket, double* scal) {
   if (cg_done(scal)) return;
   const double alpha = nnpp_ld_coherent(scal + SC_ALPHA);
-  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     const double zi = ri * pinv[i];
     zv[i] = zi;
-    v[0] = ri * zi;
-    v[1] = ri * ri;
+    v[0] += ri * zi;
+    v[1] += ri * ri;
   }
   if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
     scal[SC_BETA] = v[0] / scal[SC_RHO];
@@ -188,35 +243,45 @@ __global__ void k_cg_dir(int32_t n, const double* __restrict__ zv, double* __res
   if (i < n) p[i] = zv[i] + beta * p[i];
 }
 
-// ---- mixed precision inner solve ----
+// ---- mixed precision: CG with reliable updates ----
 
-__global__ __launch_bounds__(NNPP_BLOCK) void k_m_start(int32_t n, const double* __restrict__ r64,
-                                                        double inv_s, const float* __restrict__ pinv,
-                                                        float* __restrict__ e, float* __restrict__ r,
-                                                        float* __restrict__ zv, float* __restrict__ p,
-                                                        double inner_rtol2, double* part,
-                                                        unsigned* ticket, double* scal) {
-  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
+// FP32 residual from the FP64 one (normalised by sb = 1/||b||), z = D^-1 r, rho, ||r||^2.
+// START also clears the correction e and sets p = z and the iteration count.
+template <bool START>
+__global__ __launch_bounds__(NNPP_BLOCK) void k_m_replace(int32_t n, const double* __restrict__ r64,
+                                                          double sb, const float* __restrict__ pinv,
+                                                          float* __restrict__ e,
+                                                          float* __restrict__ r,
+                                                          float* __restrict__ zv,
+                                                          float* __restrict__ p, double target2,
+                                                          double* part, unsigned* ticket,
+                                                          double* scal) {
   double v[2] = {0.0, 0.0};
-  if (i < n) {
-    const float ri = (float)(r64[i] * inv_s);
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK) {
+    const float ri = (float)(r64[i] * sb);
     const float zi = ri * pinv[i];
-    e[i] = 0.0f;
     r[i] = ri;
     zv[i] = zi;
-    p[i] = zi;
-    v[0] = (double)ri * (double)zi;
-    v[1] = (double)ri * (double)ri;
+    if (START) {
+      e[i] = 0.0f;
+      p[i] = zi;
+    }
+    v[0] += (double)ri * (double)zi;
+    v[1] += (double)ri * (double)ri;
   }
   if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
     scal[SC_RHO] = v[0];
     scal[SC_RR] = v[1];
-    scal[SC_TARGET2] = inner_rtol2 * v[1];
-    scal[SC_ITERS] = 0.0;
-    scal[SC_DONE] = (v[1] == 0.0) ? 1.0 : 0.0;
+    scal[SC_RRMAX] = v[1];
+    scal[SC_TARGET2] = target2;
+    if (START) scal[SC_ITERS] = 0.0;
+    // after a replacement the FP64 test has just failed: run at least one more iteration
+    scal[SC_DONE] = (START && v[1] <= target2) ? 1.0 : 0.0;
   }
 }
 
+template <bool WIDE>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
                                                        const float* __restrict__ w,
                                                        const float* __restrict__ dd,
@@ -224,12 +289,14 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
                                                        float* __restrict__ q, double* part,
                                                        unsigned* ticket, double* scal) {
   if (cg_done(scal)) return;
-  const int64_t i = nnpp_xcd_block(blockIdx.x, gridDim.x) * NNPP_BLOCK + threadIdx.x;
   double v[1] = {0.0};
-  if (i < n) {
-    const float qi = spmv32_row(sl, w, dd, p, i);
+  const int64_t nb = nnpp_nblocks(n);
+  for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
+    const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    if (i >= n) continue;
+    const float qi = spmv32_row<WIDE>(sl, w, dd, p, i);
     q[i] = qi;
-    v[0] = (double)p[i] * (double)qi;
+    v[0] += (double)p[i] * (double)qi;
   }
   if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) {
     scal[SC_PQ] = v[0];
@@ -237,60 +304,89 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
   }
 }
 
+// e += alpha p, r -= alpha q, z = D^-1 r; stops the batch (DONE) at the target or when the
+// residual has dropped by trig2 (squared) below its maximum since the last replacement.
 __global__ __launch_bounds__(NNPP_BLOCK) void k_m_update(int32_t n, const float* __restrict__ pinv,
                                                          const float* __restrict__ p,
                                                          const float* __restrict__ q,
                                                          float* __restrict__ e,
                                                          float* __restrict__ r,
-                                                         float* __restrict__ zv, double* part,
-                                                         unsigned* ticket, double* scal) {
+                                                         float* __restrict__ zv, double trig2,
+                                                         double* part, unsigned* ticket,
+                                                         double* scal) {
   if (cg_done(scal)) return;
   const float alpha = (float)nnpp_ld_coherent(scal + SC_ALPHA);
-  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK) {
     e[i] += alpha * p[i];
     const float ri = r[i] - alpha * q[i];
     r[i] = ri;
     const float zi = ri * pinv[i];
     zv[i] = zi;
-    v[0] = (double)ri * (double)zi;
-    v[1] = (double)ri * (double)ri;
+    v[0] += (double)ri * (double)zi;
+    v[1] += (double)ri * (double)ri;
   }
   if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
     scal[SC_BETA] = v[0] / scal[SC_RHO];
     scal[SC_RHO] = v[0];
     scal[SC_RR] = v[1];
     scal[SC_ITERS] += 1.0;
-    if (v[1] <= scal[SC_TARGET2]) scal[SC_DONE] = 1.0;
+    const double mx = scal[SC_RRMAX] > v[1] ? scal[SC_RRMAX] : v[1];
+    scal[SC_RRMAX] = mx;
+    if (v[1] <= scal[SC_TARGET2] || v[1] <= trig2 * mx) scal[SC_DONE] = 1.0;
   }
 }
 
+// p = z + beta p. The iteration that stops a batch skips it (the flag is already set), so the
+// host runs it once with FORCE before a residual replacement: continuing along a direction that
+// missed its update would take the step length r.z / p.Ap, which only holds for r orthogonal
+// to the previous direction, and diverges.
+template <bool FORCE>
 __global__ void k_m_dir(int32_t n, const float* __restrict__ zv, float* __restrict__ p,
                         const double* scal) {
-  if (cg_done(scal)) return;
+  if (!FORCE && cg_done(scal)) return;
   const float beta = (float)nnpp_ld_coherent(scal + SC_BETA);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = zv[i] + beta * p[i];
 }
 
-__global__ void k_m_apply(int32_t n, double s, const float* __restrict__ e,
+// x += e / sb (FP64), e = 0.
+__global__ void k_m_apply(int32_t n, double inv_sb, float* __restrict__ e,
                           double* __restrict__ x) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += s * (double)e[i];
+  if (i < n) {
+    x[i] += inv_sb * (double)e[i];
+    e[i] = 0.0f;
+  }
 }
 
 // phi = x on free nodes, Dirichlet value on Dirichlet nodes, unchanged elsewhere.
 __global__ void k_scatter_phi(int32_t n, int S, const uint8_t* __restrict__ pfree,
                               const uint16_t* __restrict__ bcmask, const double* __restrict__ bcphi,
-                              const double* __restrict__ x, const double* __restrict__ phi_old,
-                              double* __restrict__ phi) {
+                              const double* __restrict__ x, double* __restrict__ phi) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double v = phi_old[i];
-  if (pfree[i]) v = x[i];
-  else if ((bcmask[i] >> S) & 1u) v = bcphi[i];
-  phi[i] = v;
+  if (pfree[i]) phi[i] = x[i];
+  else if ((bcmask[i] >> S) & 1u) phi[i] = bcphi[i];
+}
+
+template <bool START, typename... A>
+void launch_resid(bool wide, int64_t g, hipStream_t s, A... a) {
+  if (wide) k_resid<START, true><<<g, NNPP_BLOCK, 0, s>>>(a...);
+  else k_resid<START, false><<<g, NNPP_BLOCK, 0, s>>>(a...);
+}
+
+template <typename... A>
+void launch_cg_spmv(bool wide, int64_t g, hipStream_t s, A... a) {
+  if (wide) k_cg_spmv<true><<<g, NNPP_BLOCK, 0, s>>>(a...);
+  else k_cg_spmv<false><<<g, NNPP_BLOCK, 0, s>>>(a...);
+}
+
+template <typename... A>
+void launch_m_spmv(bool wide, int64_t g, hipStream_t s, A... a) {
+  if (wide) k_m_spmv<true><<<g, NNPP_BLOCK, 0, s>>>(a...);
+  else k_m_spmv<false><<<g, NNPP_BLOCK, 0, s>>>(a...);
 }
 
 }  // namespace
@@ -302,7 +398,10 @@ static int read_scalars(nnpp_ctx* c) {
   return NNPP_OK;
 }
 
-// Uq: state whose species give the charge; phi_in: warm start / kept values; phi_out: result.
+static constexpr double RELIABLE_DROP2 = 1e-4;  // replace the residual every 2 decades
+
+// Uq: state whose species give the charge; phi: warm start in, result out (non-free,
+// non-Dirichlet entries are left as they are).
 int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, double* phi,
                              int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32) {
   const int32_t n = c->n;
@@ -312,16 +411,16 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   if (relres) *relres = 0.0;
   if (!n) return NNPP_OK;
   const int64_t nb = nnpp_nblocks(n);
-  Slices sl{c->pot.off, c->pot.len, c->pot.col};
+  const int64_t g = nnpp_red_grid(nb);  // grid of every reduction kernel
+  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first};
+  const bool wide = c->pot.wide != 0;
   double zs[NNPP_MAX_SPECIES] = {0};
   for (int k = 0; k < S; k++) zs[k] = c->p.z[k];
-  double* dz = c->cg_q;  // z_k staging: copy valences to device scratch
   NNPP_HIP(c, hipMemcpyAsync(c->scal + SC_COUNT, zs, sizeof(double) * NNPP_MAX_SPECIES,
                              hipMemcpyHostToDevice, s));
-  (void)dz;
   const double cq = (1. / (c->p.K_phi * c->p.sigma1)) * c->p.F * c->p.CSolid;
-  k_rhs<<<nb, NNPP_BLOCK, 0, s>>>(n, S, Uq, c->scal + SC_COUNT, cq, c->pfree, c->psc, c->pbdir,
-                                  phi, c->cg_b, c->cg_x, c->red_part, c->red_ticket, c->scal);
+  k_rhs<<<g, NNPP_BLOCK, 0, s>>>(n, S, Uq, c->scal + SC_COUNT, cq, c->pfree, c->psc, c->pbdir,
+                                 phi, c->cg_b, c->cg_x, c->red_part, c->red_ticket, c->scal);
   NNPP_HIP(c, hipGetLastError());
   int st = read_scalars(c);
   if (st) return st;
@@ -329,18 +428,16 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   if (!std::isfinite(bnorm2))
     return nnpp_set_error(c, NNPP_ERR_NONFINITE, "non-finite potential right-hand side");
   const double* bcphi = c->bcval + (int64_t)S * n;
+  const double rtol2 = c->rtol * c->rtol;
   int64_t n64 = 0, n32 = 0;
   int total = 0;
   int status = NNPP_OK;
   double rr = 0.0;
-  if (bnorm2 == 0.0) {
-    // zero right-hand side: x = 0 on the free nodes (already set to 0 by k_rhs? no: x = phi)
+  if (bnorm2 == 0.0) {  // zero right-hand side: the solution is 0 on the free nodes
     NNPP_HIP(c, hipMemsetAsync(c->cg_x, 0, sizeof(double) * n, s));
   } else if (precision != NNPP_MIXED) {
-    const double rtol2 = c->rtol * c->rtol;
-    k_resid<true><<<nb, NNPP_BLOCK, 0, s>>>(n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
-                                            c->cg_r, c->cg_z, c->cg_p, rtol2, c->red_part,
-                                            c->red_ticket, c->scal);
+    launch_resid<true>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x, c->cg_r,
+                       c->cg_z, c->cg_p, rtol2, c->red_part, c->red_ticket, c->scal);
     NNPP_HIP(c, hipGetLastError());
     n64++;
     st = read_scalars(c);
@@ -354,10 +451,10 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       }
       const int batch = left < CG_BATCH ? left : CG_BATCH;
       for (int it = 0; it < batch; it++) {
-        k_cg_spmv<<<nb, NNPP_BLOCK, 0, s>>>(n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q,
-                                            c->red_part, c->red_ticket, c->scal);
-        k_cg_update<<<nb, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
-                                              c->cg_z, c->red_part, c->red_ticket, c->scal);
+        launch_cg_spmv(wide, g, s, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
+                       c->red_ticket, c->scal);
+        k_cg_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
+                                             c->cg_z, c->red_part, c->red_ticket, c->scal);
         k_cg_dir<<<nb, NNPP_BLOCK, 0, s>>>(n, c->cg_z, c->cg_p, c->scal);
       }
       NNPP_HIP(c, hipGetLastError());
@@ -370,67 +467,77 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
     }
     n64 += total;
     // FP64 residual check of the final iterate
-    k_resid<false><<<nb, NNPP_BLOCK, 0, s>>>(n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
-                                             c->cg_r, nullptr, nullptr, 0.0, c->red_part,
-                                             c->red_ticket, c->scal);
+    launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+                        c->cg_r, nullptr, nullptr, 0.0, c->red_part, c->red_ticket, c->scal);
     NNPP_HIP(c, hipGetLastError());
     n64++;
     st = read_scalars(c);
     if (st) return st;
     rr = c->h_scal[SC_RR];
   } else {
-    const double target2 = c->rtol * c->rtol * bnorm2;
-    const double irt2 = c->inner_rtol * c->inner_rtol;
-    while (true) {
-      k_resid<false><<<nb, NNPP_BLOCK, 0, s>>>(n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b,
-                                               c->cg_x, c->cg_r, nullptr, nullptr, 0.0,
-                                               c->red_part, c->red_ticket, c->scal);
+    const double sb = 1.0 / std::sqrt(bnorm2);  // FP32 vectors live in units of ||b||
+    const double target2 = rtol2 * bnorm2;
+    launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+                        c->cg_r, nullptr, nullptr, 0.0, c->red_part, c->red_ticket, c->scal);
+    NNPP_HIP(c, hipGetLastError());
+    n64++;
+    st = read_scalars(c);
+    if (st) return st;
+    rr = c->h_scal[SC_RR];
+    if (rr > target2) {
+      k_m_replace<true><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, c->pinv32, c->f_e, c->f_r,
+                                                 c->f_z, c->f_p, rtol2, c->red_part,
+                                                 c->red_ticket, c->scal);
       NNPP_HIP(c, hipGetLastError());
-      n64++;
-      st = read_scalars(c);
-      if (st) return st;
-      rr = c->h_scal[SC_RR];
-      if (!std::isfinite(rr))
-        return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
-      if (rr <= target2) break;
-      if (total >= c->maxit) {
-        status = NNPP_ERR_NOCONV;
-        break;
-      }
-      const double sc = std::sqrt(rr);
-      k_m_start<<<nb, NNPP_BLOCK, 0, s>>>(n, c->cg_r, 1.0 / sc, c->pinv32, c->f_e, c->f_r, c->f_z,
-                                          c->f_p, irt2, c->red_part, c->red_ticket, c->scal);
-      NNPP_HIP(c, hipGetLastError());
-      int inner = 0;
-      bool done = false;
-      while (!done) {
-        const int left = c->maxit - total - inner;
-        if (left <= 0) break;
+      while (true) {
+        const int left = c->maxit - total;
+        if (left <= 0) {
+          status = NNPP_ERR_NOCONV;
+          break;
+        }
         const int batch = left < CG_BATCH ? left : CG_BATCH;
         for (int it = 0; it < batch; it++) {
-          k_m_spmv<<<nb, NNPP_BLOCK, 0, s>>>(n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q,
-                                             c->red_part, c->red_ticket, c->scal);
-          k_m_update<<<nb, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
-                                               c->f_z, c->red_part, c->red_ticket, c->scal);
-          k_m_dir<<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, c->scal);
+          launch_m_spmv(wide, g, s, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
+                        c->red_ticket, c->scal);
+          k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
+                                              c->f_z, RELIABLE_DROP2, c->red_part,
+                                              c->red_ticket, c->scal);
+          k_m_dir<false><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, c->scal);
         }
         NNPP_HIP(c, hipGetLastError());
         st = read_scalars(c);
         if (st) return st;
-        inner = (int)c->h_scal[SC_ITERS];
-        done = c->h_scal[SC_DONE] != 0.0;
+        total = (int)c->h_scal[SC_ITERS];
+        if (!std::isfinite(c->h_scal[SC_RR]))
+          return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
+        if (c->h_scal[SC_DONE] == 0.0) continue;
+        // reliable update: fold the correction into x, recompute the residual in FP64
+        k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
+        launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+                            c->cg_r, nullptr, nullptr, 0.0, c->red_part, c->red_ticket,
+                            c->scal);
+        NNPP_HIP(c, hipGetLastError());
+        n64++;
+        st = read_scalars(c);
+        if (st) return st;
+        rr = c->h_scal[SC_RR];
+        if (!std::isfinite(rr))
+          return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
+        if (rr <= target2) break;
+        k_m_dir<true><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, c->scal);
+        k_m_replace<false><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, c->pinv32, c->f_e, c->f_r,
+                                                    c->f_z, c->f_p, rtol2, c->red_part,
+                                                    c->red_ticket, c->scal);
+        NNPP_HIP(c, hipGetLastError());
       }
-      total += inner;
-      n32 += inner;
-      k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, sc, c->f_e, c->cg_x);
-      NNPP_HIP(c, hipGetLastError());
-      if (inner == 0) {  // no progress possible (budget spent)
-        status = NNPP_ERR_NOCONV;
-        break;
+      n32 += total;
+      if (status != NNPP_OK) {  // keep the progress made so far
+        k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
+        NNPP_HIP(c, hipGetLastError());
       }
     }
   }
-  k_scatter_phi<<<nb, NNPP_BLOCK, 0, s>>>(n, S, c->pfree, c->bcmask, bcphi, c->cg_x, phi, phi);
+  k_scatter_phi<<<nb, NNPP_BLOCK, 0, s>>>(n, S, c->pfree, c->bcmask, bcphi, c->cg_x, phi);
   NNPP_HIP(c, hipGetLastError());
   NNPP_HIP(c, hipStreamSynchronize(s));
   if (iters) *iters = total;
@@ -445,14 +552,15 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const int32_t n = c->n;
   if (!n) return NNPP_OK;
-  const int64_t nb = nnpp_nblocks(n);
-  Slices sl{c->pot.off, c->pot.len, c->pot.col};
+  const int64_t g = nnpp_red_grid(nnpp_nblocks(n));
+  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first};
+  const bool wide = c->pot.wide != 0;
   if (precision == NNPP_MIXED)
-    k_m_spmv<<<nb, NNPP_BLOCK, 0, c->stream>>>(n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q,
-                                               c->red_part, c->red_ticket, c->scal);
+    launch_m_spmv(wide, g, c->stream, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
+                  c->red_ticket, c->scal);
   else
-    k_cg_spmv<<<nb, NNPP_BLOCK, 0, c->stream>>>(n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q,
-                                                c->red_part, c->red_ticket, c->scal);
+    launch_cg_spmv(wide, g, c->stream, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
+                   c->red_ticket, c->scal);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }

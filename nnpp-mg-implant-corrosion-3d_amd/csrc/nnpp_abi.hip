@@ -30,6 +30,8 @@ void free_slices(nnpp_slices& s) {
   dfree(s.col);
   dfree(s.w64);
   dfree(s.w32);
+  dfree(s.first);
+  dfree(s.d16);
   s = nnpp_slices();
 }
 

@@ -31,11 +31,19 @@ struct nnpp_slices {
   int64_t slots = 0;       // total entries (incl. padding)
   int64_t nnz = 0;         // real entries
   int64_t* off = nullptr;  // [nslice + 1]
-  int32_t* len = nullptr;  // [nslice] entries per lane
+  int32_t* len = nullptr;  // [nslice] entries per lane (quad layout: groups of 4 per lane)
   int32_t* col = nullptr;  // [slots]
   double* w64 = nullptr;   // [slots]
   float* w32 = nullptr;    // [slots]
   int64_t cap_slots = 0;
+  // quad layout (potential operator): entry t of row i sits at
+  //   off[i/64] + 256 (t/4) + 4 (i%64) + t%4,
+  // so one lane reads its next 4 entries with one 8/16/32-byte load; padding has weight 0
+  // and repeats the last column. Columns are also kept as 16-bit steps from the previous
+  // entry of the row (first entry: from first[i]) unless a step exceeds 65535 (wide = 1).
+  int32_t* first = nullptr;  // [n] column before entry 0 (the row's first column, or i)
+  uint16_t* d16 = nullptr;   // [slots]
+  int wide = 0;
 };
 
 struct nnpp_ctx {
@@ -113,7 +121,7 @@ struct nnpp_ctx {
 // Device scalars used by the CG kernels and reductions.
 enum CgScalar {
   SC_RHO = 0, SC_ALPHA, SC_BETA, SC_PQ, SC_RR, SC_TARGET2, SC_DONE, SC_ITERS, SC_BNORM2,
-  SC_R0, SC_R1, SC_R2, SC_R3, SC_COUNT
+  SC_R0, SC_R1, SC_R2, SC_R3, SC_RRMAX, SC_COUNT
 };
 
 int nnpp_set_error(nnpp_ctx* c, int code, const std::string& msg);
@@ -171,9 +179,38 @@ __device__ __forceinline__ double nnpp_ld_coherent(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Reductions run on a fixed grid of at most NNPP_RED_GRID blocks (a multiple of 8); every
+// block walks several 256-node blocks. XCD x (= physical block % 8) owns the contiguous range
+// [x*per, (x+1)*per) of node blocks, so neighbour data stays in its L2.
+#define NNPP_RED_GRID 2048
+
+struct nnpp_block_iter {
+  int64_t cur, end, stride;
+};
+
+__device__ __forceinline__ nnpp_block_iter nnpp_blocks_of(int64_t nb) {
+  const int64_t g = blockIdx.x, G = gridDim.x;
+  nnpp_block_iter it;
+  if (G >= nb) {  // one node block per physical block
+    it.cur = nnpp_xcd_block(g, nb);
+    it.end = it.cur + 1;
+    it.stride = 1;
+  } else {
+    const int64_t per = (nb + NNPP_NXCD - 1) / NNPP_NXCD, x = g % NNPP_NXCD;
+    it.cur = x * per + g / NNPP_NXCD;
+    it.end = (x + 1) * per < nb ? (x + 1) * per : nb;
+    it.stride = G / NNPP_NXCD;
+  }
+  return it;
+}
+
+static inline int64_t nnpp_red_grid(int64_t nb) {
+  return nb <= NNPP_RED_GRID ? nb : NNPP_RED_GRID;
+}
+
 // Deterministic grid reduction: each block writes K partials; the last block to arrive sums
 // them in block order and returns true (in thread 0) with the totals in v. The ticket resets
-// itself, so no memset is needed between launches.
+// itself, so no memset is needed between launches. At most NNPP_RED_GRID blocks.
 template <int K>
 __device__ __forceinline__ bool nnpp_grid_sum(double (&v)[K], double* part, unsigned* ticket,
                                               int64_t nblk, int64_t bid) {
@@ -191,13 +228,23 @@ __device__ __forceinline__ bool nnpp_grid_sum(double (&v)[K], double* part, unsi
   __syncthreads();
   if (!last) return false;
   __threadfence();
-  // the last block sums the partials; lanes stride over blocks, then a fixed-order tree
+  // the last block sums the partials: every thread loads its (at most 8) partials at once,
+  // then a fixed-order tree
+  constexpr int PER = NNPP_RED_GRID / NNPP_BLOCK;
   double acc[K];
 #pragma unroll
-  for (int k = 0; k < K; k++) acc[k] = 0.0;
-  for (int64_t b = threadIdx.x; b < nblk; b += NNPP_BLOCK)
+  for (int k = 0; k < K; k++) {
+    double t[PER];
 #pragma unroll
-    for (int k = 0; k < K; k++) acc[k] += nnpp_ld_coherent(&part[b * K + k]);
+    for (int q = 0; q < PER; q++) {
+      const int64_t b = threadIdx.x + (int64_t)q * NNPP_BLOCK;
+      t[q] = (b < nblk) ? nnpp_ld_coherent(&part[b * K + k]) : 0.0;
+    }
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < PER; q++) a += t[q];
+    acc[k] = a;
+  }
   nnpp_block_sum<K>(acc, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -207,7 +254,7 @@ __device__ __forceinline__ bool nnpp_grid_sum(double (&v)[K], double* part, unsi
   return threadIdx.x == 0;
 }
 
-static inline int64_t nnpp_nblocks(int64_t n) { return (n + NNPP_BLOCK - 1) / NNPP_BLOCK; }
+__host__ __device__ static inline int64_t nnpp_nblocks(int64_t n) { return (n + NNPP_BLOCK - 1) / NNPP_BLOCK; }
 
 // ---------------------------------------------------------------- launchers (host) -----
 int nnpp_build_neighbors_dev(nnpp_ctx* c, double r, double cdx, double cdy, double cdz,
