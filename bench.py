@@ -93,14 +93,13 @@ def cpu_baseline(nn, case, rp, col, budget_s=20.0):
 
 
 def kernels_only(nn, case, reps):
-    """Time the hot kernels alone (for rocprofv3 --pmc runs): one mixed step first so the
-    solver vectors hold real data, then `reps` launches of each kernel."""
+    """Time the hot kernels alone (for rocprofv3 runs): no solve runs first, so every traced
+    launch of these kernels is a full one (their cost does not depend on the vector values)."""
     pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
     nnz, _ = pb.build_clouds_hash()
     pb.set_bc(case.bctypes, case.bcvalues)
     pb.init_state(case.icvalues)
-    st = pb.step(1, nn.MIXED)
-    out = {"nodes": case.n, "bonds": nnz, "liquid_bonds": st["nnz_ll"]}
+    out = {"nodes": case.n, "bonds": nnz}
     for name, k, prec in (("pot_spmv_fp32", nn._abi.KERNEL_POT_SPMV, nn.MIXED),
                           ("pot_spmv_fp64", nn._abi.KERNEL_POT_SPMV, nn.FP64),
                           ("flux_fp64", nn._abi.KERNEL_FLUX, nn.FP64),
@@ -122,6 +121,88 @@ def copy_probe(mib):
                       "launches": 3}))
 
 
+KERNEL_SOURCES = ["nnpp-mg-implant-corrosion-3d_amd/csrc/k_potential.hip",
+                  "nnpp-mg-implant-corrosion-3d_amd/csrc/k_flux.hip",
+                  "nnpp-mg-implant-corrosion-3d_amd/csrc/nnpp_internal.hpp"]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_regular128.json")
+
+
+def sources_digest():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def kernel_short_name(full):
+    n = full.replace("(anonymous namespace)::", "")
+    if n.startswith("void "):
+        n = n[5:]
+    return n.split("(")[0]
+
+
+def _db_rows(db, sql):
+    import sqlite3
+    con = sqlite3.connect(db)
+    try:
+        return con.execute(sql).fetchall()
+    finally:
+        con.close()
+
+
+def summarize_rocprof(trace_db, fetch_db=None, write_db=None, copyf_db=None, copyw_db=None,
+                      copy_bytes=None, out_json=None):
+    """Text summary of rocprofv3 results (kernel trace + FETCH_SIZE/WRITE_SIZE counter runs).
+
+    Counter runs are scaled by a plain copy of known size: scale = bytes moved / counter KiB."""
+    lines = ["| kernel | calls | total ms | avg us |", "|---|---|---|---|"]  # top_kernels: us
+    rows = _db_rows(trace_db, "select name, total_calls, total_duration, average from top_kernels "
+                              "order by total_duration desc limit 16")
+    for name, calls, tot, avg in rows:
+        lines.append(f"| `{kernel_short_name(name)[:70]}` | {calls} | {tot / 1e3:.3f} | {avg:.1f} |")
+    out = {"digest": sources_digest(), "kernels": {}}
+    if fetch_db and write_db and copyf_db and copyw_db and copy_bytes:
+        def per_kernel(db):
+            r = _db_rows(db, "select kernel_name, avg(value), count(*), avg(duration) from "
+                             "counters_collection group by kernel_name")
+            return {kernel_short_name(k): (v, c, d) for k, v, c, d in r}
+        cf = max(v for v, c, d in per_kernel(copyf_db).values())
+        cw = max(v for v, c, d in per_kernel(copyw_db).values())
+        fs, ws = copy_bytes / cf, copy_bytes / cw  # bytes per counter unit
+        out.update({"fetch_scale": fs, "write_scale": ws})
+        fk, wk = per_kernel(fetch_db), per_kernel(write_db)
+        lines += ["", f"Counter scale from a {copy_bytes / 2**30:.0f} GiB copy: FETCH_SIZE x{fs:.1f}"
+                      f" B/unit, WRITE_SIZE x{ws:.1f} B/unit.", "",
+                  "| kernel | dispatches | FETCH bytes/dispatch | WRITE bytes/dispatch | avg us |",
+                  "|---|---|---|---|---|"]
+        for k in sorted(fk, key=lambda k: -fk[k][0] * fk[k][1])[:10]:
+            f_ = fk[k][0] * fs
+            w_ = wk.get(k, (0, 0, 0))[0] * ws
+            out["kernels"][k] = {"fetch": f_, "write": w_, "avg_ns": fk[k][2]}
+            lines.append(f"| `{k[:70]}` | {fk[k][1]} | {f_:.4g} | {w_:.4g} | {fk[k][2] / 1e3:.1f} |")
+    if out_json:
+        with open(out_json, "w") as fh:
+            json.dump(out, fh, indent=1)
+    print("\n".join(lines))
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per dispatch from the committed counter summary, if it matches these sources."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("digest") != sources_digest():
+        return None
+    for k, v in t.get("kernels", {}).items():
+        if kernel_prefix in k:
+            return v["fetch"] + v["write"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +216,17 @@ def main():
                     help="only time the hot kernels (for counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter scale)")
+    ap.add_argument("--summarize-rocprof", nargs="+", metavar="DB",
+                    help="trace.db [fetch.db write.db copyfetch.db copywrite.db copy_MiB]")
+    ap.add_argument("--traffic-out", default=None)
     a = ap.parse_args()
+    if a.summarize_rocprof:
+        d = a.summarize_rocprof
+        if len(d) >= 6:
+            summarize_rocprof(d[0], d[1], d[2], d[3], d[4], int(d[5]) * 2 ** 20, a.traffic_out)
+        else:
+            summarize_rocprof(d[0])
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -238,7 +329,7 @@ def main():
         "neighbor_build_s": t_nb,
         "kernels": kern,
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_m_spmv"),
                      "kernel": "k_m_spmv (FP32 potential SpMV + dot)",
                      "bytes_per_launch": bytes32},
         "cpu_baseline": cpu,

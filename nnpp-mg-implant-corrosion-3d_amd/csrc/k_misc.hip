@@ -217,8 +217,7 @@ __global__ void k_pot_fill(int32_t n, int64_t nrows_pad, int S, const int64_t* _
 // main.cpp:915-922 front update, then the volume sum of main.cpp:944-949 over the new phase.
 __global__ __launch_bounds__(NNPP_BLOCK) void k_front(int32_t n, const double* __restrict__ U0,
                                                       uint8_t* __restrict__ phase, double thr,
-                                                      double sat, double* part, unsigned* ticket,
-                                                      double* scal) {
+                                                      double sat, double* __restrict__ part) {
   double v[3] = {0.0, 0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * NNPP_BLOCK) {
@@ -234,11 +233,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_front(int32_t n, const double* _
       v[2] += 1.0;
     }
   }
-  if (nnpp_grid_sum<3>(v, part, ticket, gridDim.x, blockIdx.x)) {
-    scal[SC_R0] = v[0];
-    scal[SC_R1] = v[1];
-    scal[SC_R2] = v[2];
-  }
+  nnpp_block_partial<3>(v, part);
 }
 
 // main.cpp:676-711: F = U1 - bc on Dirichlet rows/fields (bc already divided by CSolid for
@@ -255,13 +250,84 @@ __global__ void k_bc_rows(int32_t n, int F, const uint16_t* __restrict__ bcmask,
 }
 
 __global__ __launch_bounds__(NNPP_BLOCK) void k_nonfinite(int64_t m, const double* __restrict__ U,
-                                                          double* part, unsigned* ticket,
-                                                          double* scal) {
+                                                          double* __restrict__ part) {
   double v[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * NNPP_BLOCK)
     if (!isfinite(U[i])) v[0] += 1.0;
-  if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) scal[SC_R3] = v[0];
+  nnpp_block_partial<1>(v, part);
+}
+
+// One block: sum the partials of the previous kernel and update the device scalars.
+__global__ __launch_bounds__(NNPP_BLOCK) void k_finalize(int op, const double* __restrict__ part,
+                                                         int64_t nblk, double* __restrict__ scal,
+                                                         double a0) {
+  if ((op == FIN_PQ || op == FIN_UPDATE || op == FIN_M_UPDATE) && scal[SC_DONE] != 0.0) return;
+  if (op == FIN_BNORM2 || op == FIN_PQ || op == FIN_NONFINITE) {
+    double v[1];
+    nnpp_final_sum<1>(v, part, nblk);
+    if (threadIdx.x != 0) return;
+    if (op == FIN_BNORM2) scal[SC_BNORM2] = v[0];
+    else if (op == FIN_NONFINITE) scal[SC_R3] = v[0];
+    else {
+      scal[SC_PQ] = v[0];
+      scal[SC_ALPHA] = scal[SC_RHO] / v[0];
+    }
+  } else if (op == FIN_FRONT) {
+    double v[3];
+    nnpp_final_sum<3>(v, part, nblk);
+    if (threadIdx.x != 0) return;
+    scal[SC_R0] = v[0];
+    scal[SC_R1] = v[1];
+    scal[SC_R2] = v[2];
+  } else {
+    double v[2];
+    nnpp_final_sum<2>(v, part, nblk);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+      case FIN_RESID:
+        scal[SC_RR] = v[1];
+        break;
+      case FIN_RESID_START: {
+        scal[SC_RHO] = v[0];
+        scal[SC_RR] = v[1];
+        const double t2 = a0 * scal[SC_BNORM2];
+        scal[SC_TARGET2] = t2;
+        scal[SC_ITERS] = 0.0;
+        scal[SC_DONE] = (v[1] <= t2) ? 1.0 : 0.0;
+        break;
+      }
+      case FIN_UPDATE:
+        scal[SC_BETA] = v[0] / scal[SC_RHO];
+        scal[SC_RHO] = v[0];
+        scal[SC_RR] = v[1];
+        scal[SC_ITERS] += 1.0;
+        if (v[1] <= scal[SC_TARGET2]) scal[SC_DONE] = 1.0;
+        break;
+      case FIN_M_REPLACE:
+      case FIN_M_REPLACE_START:
+        scal[SC_RHO] = v[0];
+        scal[SC_RR] = v[1];
+        scal[SC_RRMAX] = v[1];
+        scal[SC_TARGET2] = a0;
+        if (op == FIN_M_REPLACE_START) scal[SC_ITERS] = 0.0;
+        // after a replacement the FP64 test has just failed: run at least one more iteration
+        scal[SC_DONE] = (op == FIN_M_REPLACE_START && v[1] <= a0) ? 1.0 : 0.0;
+        break;
+      case FIN_M_UPDATE: {
+        scal[SC_BETA] = v[0] / scal[SC_RHO];
+        scal[SC_RHO] = v[0];
+        scal[SC_RR] = v[1];
+        scal[SC_ITERS] += 1.0;
+        const double mx = scal[SC_RRMAX] > v[1] ? scal[SC_RRMAX] : v[1];
+        scal[SC_RRMAX] = mx;
+        if (v[1] <= scal[SC_TARGET2] || v[1] <= a0 * mx) scal[SC_DONE] = 1.0;
+        break;
+      }
+      default:
+        break;
+    }
+  }
 }
 
 // Poisson operator action for nnpp_poisson_apply: K11 rows of utils.h:1794-1848 without the
@@ -465,10 +531,11 @@ int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flip
   const int32_t n = c->n;
   hipStream_t s = c->stream;
   if (n) {
-    k_front<<<nnpp_red_grid(nnpp_nblocks(n)), NNPP_BLOCK, 0, s>>>(n, U, c->phase, thr,
-                                                                  c->p.CSat / c->p.CSolid,
-                                                   c->red_part, c->red_ticket, c->scal);
+    const int64_t g = nnpp_red_grid(nnpp_nblocks(n));
+    k_front<<<g, NNPP_BLOCK, 0, s>>>(n, U, c->phase, thr, c->p.CSat / c->p.CSolid, c->red_part);
     NNPP_HIP(c, hipGetLastError());
+    const int st = nnpp_finalize(c, FIN_FRONT, g, 0.0);
+    if (st) return st;
     NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT,
                                hipMemcpyDeviceToHost, s));
   }
@@ -494,8 +561,10 @@ int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad) {
   *bad = 0;
   if (!m) return NNPP_OK;
   const int64_t nb = nnpp_red_grid(nnpp_nblocks(m));
-  k_nonfinite<<<nb, NNPP_BLOCK, 0, c->stream>>>(m, U, c->red_part, c->red_ticket, c->scal);
+  k_nonfinite<<<nb, NNPP_BLOCK, 0, c->stream>>>(m, U, c->red_part);
   NNPP_HIP(c, hipGetLastError());
+  const int st = nnpp_finalize(c, FIN_NONFINITE, nb, 0.0);
+  if (st) return st;
   NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost,
                              c->stream));
   NNPP_HIP(c, hipStreamSynchronize(c->stream));
@@ -508,6 +577,12 @@ int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out) {
   k_pot_apply_csr<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(
       c->n, c->row_ptr, c->col, c->phase, c->x, c->y, c->z,
       weight_args(c, 3.0 + 2.0 * c->p.s_phi), in, out);
+  NNPP_HIP(c, hipGetLastError());
+  return NNPP_OK;
+}
+
+int nnpp_finalize(nnpp_ctx* c, int op, int64_t nblk, double a0) {
+  k_finalize<<<1, NNPP_BLOCK, 0, c->stream>>>(op, c->red_part, nblk, c->scal, a0);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }

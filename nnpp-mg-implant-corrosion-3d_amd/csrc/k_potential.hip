@@ -38,7 +38,7 @@ struct Slices {
 };
 
 __device__ __forceinline__ bool cg_done(const double* scal) {
-  return nnpp_ld_coherent(scal + SC_DONE) != 0.0;
+  return nnpp_ld_scalar(scal + SC_DONE) != 0.0;
 }
 
 // Columns of the next 4 entries of a row: one 16-byte load (wide) or one 8-byte load of
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
                                                     const double* __restrict__ pbdir,
                                                     const double* __restrict__ phi,
                                                     double* __restrict__ b, double* __restrict__ x,
-                                                    double* part, unsigned* ticket, double* scal) {
+                                                    double* __restrict__ part) {
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
     x[i] = xi;
     v[0] += bi * bi;
   }
-  if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) scal[SC_BNORM2] = v[0];
+  nnpp_block_partial<1>(v, part);
 }
 
 // r = b - M x in FP64 (RR = ||r||^2). START: also z = D^-1 r, p = z, rho, the target from
@@ -153,9 +153,8 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
                                                       const double* __restrict__ b,
                                                       const double* __restrict__ x,
                                                       double* __restrict__ r, double* __restrict__ zv,
-                                                      double* __restrict__ pv, double rtol2,
-                                                      double* part, unsigned* ticket,
-                                                      double* scal) {
+                                                      double* __restrict__ pv,
+                                                      double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
@@ -171,16 +170,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
       v[0] += ri * zi;
     }
   }
-  if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
-    scal[SC_RR] = v[1];
-    if (START) {
-      scal[SC_RHO] = v[0];
-      const double t2 = rtol2 * scal[SC_BNORM2];
-      scal[SC_TARGET2] = t2;
-      scal[SC_ITERS] = 0.0;
-      scal[SC_DONE] = (v[1] <= t2) ? 1.0 : 0.0;
-    }
-  }
+  nnpp_block_partial<2>(v, part);
 }
 
 template <bool WIDE>
@@ -188,8 +178,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
                                                         const double* __restrict__ w,
                                                         const double* __restrict__ dd,
                                                         const double* __restrict__ p,
-                                                        double* __restrict__ q, double* part,
-                                                        unsigned* ticket, double* scal) {
+                                                        double* __restrict__ q, double* __restrict__ part, const double* __restrict__ scal) {
   if (cg_done(scal)) return;
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
@@ -200,10 +189,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
     q[i] = qi;
     v[0] += p[i] * qi;
   }
-  if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) {
-    scal[SC_PQ] = v[0];
-    scal[SC_ALPHA] = scal[SC_RHO] / v[0];
-  }
+  nnpp_block_partial<1>(v, part);
 }
 
 __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_update(int32_t n, const double* __restrict__ pinv,
@@ -211,10 +197,9 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_update(int32_t n, const doubl
                                                           const double* __restrict__ q,
                                                           double* __restrict__ x,
                                                           double* __restrict__ r,
-                                                          double* __restrict__ zv, double* part,
-                                                          unsigned* ticket, double* scal) {
+                                                          double* __restrict__ zv, double* __restrict__ part, const double* __restrict__ scal) {
   if (cg_done(scal)) return;
-  const double alpha = nnpp_ld_coherent(scal + SC_ALPHA);
+  const double alpha = nnpp_ld_scalar(scal + SC_ALPHA);
   double v[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * NNPP_BLOCK) {
@@ -226,19 +211,13 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_update(int32_t n, const doubl
     v[0] += ri * zi;
     v[1] += ri * ri;
   }
-  if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
-    scal[SC_BETA] = v[0] / scal[SC_RHO];
-    scal[SC_RHO] = v[0];
-    scal[SC_RR] = v[1];
-    scal[SC_ITERS] += 1.0;
-    if (v[1] <= scal[SC_TARGET2]) scal[SC_DONE] = 1.0;
-  }
+  nnpp_block_partial<2>(v, part);
 }
 
 __global__ void k_cg_dir(int32_t n, const double* __restrict__ zv, double* __restrict__ p,
                          const double* scal) {
   if (cg_done(scal)) return;
-  const double beta = nnpp_ld_coherent(scal + SC_BETA);
+  const double beta = nnpp_ld_scalar(scal + SC_BETA);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = zv[i] + beta * p[i];
 }
@@ -253,9 +232,8 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_replace(int32_t n, const doubl
                                                           float* __restrict__ e,
                                                           float* __restrict__ r,
                                                           float* __restrict__ zv,
-                                                          float* __restrict__ p, double target2,
-                                                          double* part, unsigned* ticket,
-                                                          double* scal) {
+                                                          float* __restrict__ p,
+                                                          double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * NNPP_BLOCK) {
@@ -270,15 +248,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_replace(int32_t n, const doubl
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
   }
-  if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
-    scal[SC_RHO] = v[0];
-    scal[SC_RR] = v[1];
-    scal[SC_RRMAX] = v[1];
-    scal[SC_TARGET2] = target2;
-    if (START) scal[SC_ITERS] = 0.0;
-    // after a replacement the FP64 test has just failed: run at least one more iteration
-    scal[SC_DONE] = (START && v[1] <= target2) ? 1.0 : 0.0;
-  }
+  nnpp_block_partial<2>(v, part);
 }
 
 template <bool WIDE>
@@ -286,8 +256,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
                                                        const float* __restrict__ w,
                                                        const float* __restrict__ dd,
                                                        const float* __restrict__ p,
-                                                       float* __restrict__ q, double* part,
-                                                       unsigned* ticket, double* scal) {
+                                                       float* __restrict__ q, double* __restrict__ part, const double* __restrict__ scal) {
   if (cg_done(scal)) return;
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
@@ -298,10 +267,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
     q[i] = qi;
     v[0] += (double)p[i] * (double)qi;
   }
-  if (nnpp_grid_sum<1>(v, part, ticket, gridDim.x, blockIdx.x)) {
-    scal[SC_PQ] = v[0];
-    scal[SC_ALPHA] = scal[SC_RHO] / v[0];
-  }
+  nnpp_block_partial<1>(v, part);
 }
 
 // e += alpha p, r -= alpha q, z = D^-1 r; stops the batch (DONE) at the target or when the
@@ -311,11 +277,11 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_update(int32_t n, const float*
                                                          const float* __restrict__ q,
                                                          float* __restrict__ e,
                                                          float* __restrict__ r,
-                                                         float* __restrict__ zv, double trig2,
-                                                         double* part, unsigned* ticket,
-                                                         double* scal) {
+                                                         float* __restrict__ zv,
+                                                         double* __restrict__ part,
+                                                         const double* __restrict__ scal) {
   if (cg_done(scal)) return;
-  const float alpha = (float)nnpp_ld_coherent(scal + SC_ALPHA);
+  const float alpha = (float)nnpp_ld_scalar(scal + SC_ALPHA);
   double v[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * NNPP_BLOCK) {
@@ -327,15 +293,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_update(int32_t n, const float*
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
   }
-  if (nnpp_grid_sum<2>(v, part, ticket, gridDim.x, blockIdx.x)) {
-    scal[SC_BETA] = v[0] / scal[SC_RHO];
-    scal[SC_RHO] = v[0];
-    scal[SC_RR] = v[1];
-    scal[SC_ITERS] += 1.0;
-    const double mx = scal[SC_RRMAX] > v[1] ? scal[SC_RRMAX] : v[1];
-    scal[SC_RRMAX] = mx;
-    if (v[1] <= scal[SC_TARGET2] || v[1] <= trig2 * mx) scal[SC_DONE] = 1.0;
-  }
+  nnpp_block_partial<2>(v, part);
 }
 
 // p = z + beta p. The iteration that stops a batch skips it (the flag is already set), so the
@@ -346,7 +304,7 @@ template <bool FORCE>
 __global__ void k_m_dir(int32_t n, const float* __restrict__ zv, float* __restrict__ p,
                         const double* scal) {
   if (!FORCE && cg_done(scal)) return;
-  const float beta = (float)nnpp_ld_coherent(scal + SC_BETA);
+  const float beta = (float)nnpp_ld_scalar(scal + SC_BETA);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = zv[i] + beta * p[i];
 }
@@ -411,18 +369,22 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   if (relres) *relres = 0.0;
   if (!n) return NNPP_OK;
   const int64_t nb = nnpp_nblocks(n);
-  const int64_t g = nnpp_red_grid(nb);  // grid of every reduction kernel
+  const int64_t g = nnpp_red_grid(nb);  // grid of every kernel with a reduction
   Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first};
   const bool wide = c->pot.wide != 0;
+  double* part = c->red_part;
+  const double* scal = c->scal;
   double zs[NNPP_MAX_SPECIES] = {0};
   for (int k = 0; k < S; k++) zs[k] = c->p.z[k];
   NNPP_HIP(c, hipMemcpyAsync(c->scal + SC_COUNT, zs, sizeof(double) * NNPP_MAX_SPECIES,
                              hipMemcpyHostToDevice, s));
   const double cq = (1. / (c->p.K_phi * c->p.sigma1)) * c->p.F * c->p.CSolid;
   k_rhs<<<g, NNPP_BLOCK, 0, s>>>(n, S, Uq, c->scal + SC_COUNT, cq, c->pfree, c->psc, c->pbdir,
-                                 phi, c->cg_b, c->cg_x, c->red_part, c->red_ticket, c->scal);
+                                 phi, c->cg_b, c->cg_x, part);
   NNPP_HIP(c, hipGetLastError());
-  int st = read_scalars(c);
+  int st = nnpp_finalize(c, FIN_BNORM2, g, 0.0);
+  if (st) return st;
+  st = read_scalars(c);
   if (st) return st;
   const double bnorm2 = c->h_scal[SC_BNORM2];
   if (!std::isfinite(bnorm2))
@@ -433,15 +395,24 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   int total = 0;
   int status = NNPP_OK;
   double rr = 0.0;
+  // r = b - M x in FP64 (START: also z, p, rho, target, converged flag)
+  auto resid = [&](bool start) -> int {
+    if (start)
+      launch_resid<true>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+                         c->cg_r, c->cg_z, c->cg_p, part);
+    else
+      launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+                          c->cg_r, nullptr, nullptr, part);
+    NNPP_HIP(c, hipGetLastError());
+    n64++;
+    int e = nnpp_finalize(c, start ? FIN_RESID_START : FIN_RESID, g, rtol2);
+    if (e) return e;
+    return read_scalars(c);
+  };
   if (bnorm2 == 0.0) {  // zero right-hand side: the solution is 0 on the free nodes
     NNPP_HIP(c, hipMemsetAsync(c->cg_x, 0, sizeof(double) * n, s));
   } else if (precision != NNPP_MIXED) {
-    launch_resid<true>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x, c->cg_r,
-                       c->cg_z, c->cg_p, rtol2, c->red_part, c->red_ticket, c->scal);
-    NNPP_HIP(c, hipGetLastError());
-    n64++;
-    st = read_scalars(c);
-    if (st) return st;
+    if ((st = resid(true))) return st;
     bool done = c->h_scal[SC_DONE] != 0.0;
     while (!done) {
       const int left = c->maxit - total;
@@ -451,44 +422,33 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       }
       const int batch = left < CG_BATCH ? left : CG_BATCH;
       for (int it = 0; it < batch; it++) {
-        launch_cg_spmv(wide, g, s, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
-                       c->red_ticket, c->scal);
+        launch_cg_spmv(wide, g, s, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, part, scal);
+        if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
         k_cg_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
-                                             c->cg_z, c->red_part, c->red_ticket, c->scal);
-        k_cg_dir<<<nb, NNPP_BLOCK, 0, s>>>(n, c->cg_z, c->cg_p, c->scal);
+                                             c->cg_z, part, scal);
+        if ((st = nnpp_finalize(c, FIN_UPDATE, g, 0.0))) return st;
+        k_cg_dir<<<nb, NNPP_BLOCK, 0, s>>>(n, c->cg_z, c->cg_p, scal);
       }
       NNPP_HIP(c, hipGetLastError());
-      st = read_scalars(c);
-      if (st) return st;
+      if ((st = read_scalars(c))) return st;
       total = (int)c->h_scal[SC_ITERS];
       done = c->h_scal[SC_DONE] != 0.0;
       if (!std::isfinite(c->h_scal[SC_RR]))
         return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
     }
     n64 += total;
-    // FP64 residual check of the final iterate
-    launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
-                        c->cg_r, nullptr, nullptr, 0.0, c->red_part, c->red_ticket, c->scal);
-    NNPP_HIP(c, hipGetLastError());
-    n64++;
-    st = read_scalars(c);
-    if (st) return st;
+    if ((st = resid(false))) return st;  // FP64 residual check of the final iterate
     rr = c->h_scal[SC_RR];
   } else {
     const double sb = 1.0 / std::sqrt(bnorm2);  // FP32 vectors live in units of ||b||
     const double target2 = rtol2 * bnorm2;
-    launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
-                        c->cg_r, nullptr, nullptr, 0.0, c->red_part, c->red_ticket, c->scal);
-    NNPP_HIP(c, hipGetLastError());
-    n64++;
-    st = read_scalars(c);
-    if (st) return st;
+    if ((st = resid(false))) return st;
     rr = c->h_scal[SC_RR];
     if (rr > target2) {
       k_m_replace<true><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, c->pinv32, c->f_e, c->f_r,
-                                                 c->f_z, c->f_p, rtol2, c->red_part,
-                                                 c->red_ticket, c->scal);
+                                                 c->f_z, c->f_p, part);
       NNPP_HIP(c, hipGetLastError());
+      if ((st = nnpp_finalize(c, FIN_M_REPLACE_START, g, rtol2))) return st;
       while (true) {
         const int left = c->maxit - total;
         if (left <= 0) {
@@ -497,38 +457,32 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         }
         const int batch = left < CG_BATCH ? left : CG_BATCH;
         for (int it = 0; it < batch; it++) {
-          launch_m_spmv(wide, g, s, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
-                        c->red_ticket, c->scal);
+          launch_m_spmv(wide, g, s, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, part, scal);
+          if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
           k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
-                                              c->f_z, RELIABLE_DROP2, c->red_part,
-                                              c->red_ticket, c->scal);
-          k_m_dir<false><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, c->scal);
+                                              c->f_z, part, scal);
+          if ((st = nnpp_finalize(c, FIN_M_UPDATE, g, RELIABLE_DROP2))) return st;
+          k_m_dir<false><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, scal);
         }
         NNPP_HIP(c, hipGetLastError());
-        st = read_scalars(c);
-        if (st) return st;
+        if ((st = read_scalars(c))) return st;
         total = (int)c->h_scal[SC_ITERS];
         if (!std::isfinite(c->h_scal[SC_RR]))
           return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
         if (c->h_scal[SC_DONE] == 0.0) continue;
         // reliable update: fold the correction into x, recompute the residual in FP64
         k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
-        launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
-                            c->cg_r, nullptr, nullptr, 0.0, c->red_part, c->red_ticket,
-                            c->scal);
         NNPP_HIP(c, hipGetLastError());
-        n64++;
-        st = read_scalars(c);
-        if (st) return st;
+        if ((st = resid(false))) return st;
         rr = c->h_scal[SC_RR];
         if (!std::isfinite(rr))
           return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
         if (rr <= target2) break;
-        k_m_dir<true><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, c->scal);
+        k_m_dir<true><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, scal);
         k_m_replace<false><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, c->pinv32, c->f_e, c->f_r,
-                                                    c->f_z, c->f_p, rtol2, c->red_part,
-                                                    c->red_ticket, c->scal);
+                                                    c->f_z, c->f_p, part);
         NNPP_HIP(c, hipGetLastError());
+        if ((st = nnpp_finalize(c, FIN_M_REPLACE, g, rtol2))) return st;
       }
       n32 += total;
       if (status != NNPP_OK) {  // keep the progress made so far
@@ -548,7 +502,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
 }
 
 // One potential-operator application (with its dot product) on the solver's direction vector;
-// used by the kernel timing hook. The converged flag is cleared first so the launch runs.
+// used by the kernel timing hook. The converged flag is cleared first by the caller.
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const int32_t n = c->n;
   if (!n) return NNPP_OK;
@@ -557,10 +511,10 @@ int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const bool wide = c->pot.wide != 0;
   if (precision == NNPP_MIXED)
     launch_m_spmv(wide, g, c->stream, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
-                  c->red_ticket, c->scal);
+                  (const double*)c->scal);
   else
     launch_cg_spmv(wide, g, c->stream, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
-                   c->red_ticket, c->scal);
+                   (const double*)c->scal);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }

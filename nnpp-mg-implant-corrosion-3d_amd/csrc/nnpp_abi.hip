@@ -117,10 +117,8 @@ int nnpp_create(nnpp_ctx** out, int32_t device) {
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void**)&c->scal, sizeof(double) * (SC_COUNT + NNPP_MAX_SPECIES)) != hipSuccess ||
-      hipMalloc((void**)&c->red_ticket, sizeof(unsigned)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_scal, sizeof(double) * (SC_COUNT + NNPP_MAX_SPECIES)) !=
           hipSuccess ||
-      hipMemset(c->red_ticket, 0, sizeof(unsigned)) != hipSuccess ||
       hipMemset(c->scal, 0, sizeof(double) * (SC_COUNT + NNPP_MAX_SPECIES)) != hipSuccess) {
     nnpp_destroy(c);
     return NNPP_ERR_HIP;
@@ -135,7 +133,6 @@ int nnpp_destroy(nnpp_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   free_nodes(c);
   dfree(c->scal);
-  dfree(c->red_ticket);
   if (c->h_scal) hipHostFree(c->h_scal);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

@@ -102,9 +102,8 @@ struct nnpp_ctx {
   float* f_q = nullptr;
   int64_t pot_nfree = 0;
 
-  // reductions: partials + ticket + scalars; pinned host mirror of the scalars
-  double* red_part = nullptr;   // [nblk_max * 4]
-  unsigned* red_ticket = nullptr;
+  // reductions: per-block partials + scalars; pinned host mirror of the scalars
+  double* red_part = nullptr;   // [NNPP_RED_GRID * 4]
   double* scal = nullptr;       // device scalars (see enum CgScalar)
   double* h_scal = nullptr;     // pinned host copy
 
@@ -175,9 +174,15 @@ __device__ __forceinline__ void nnpp_block_sum(double (&v)[K], double* lds /*[K*
     }
 }
 
+// Load of a value another block of the SAME kernel may have written (grid reductions).
 __device__ __forceinline__ double nnpp_ld_coherent(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Load of a device scalar written by an EARLIER kernel on the same stream: caches are
+// invalidated at every kernel start, so a plain load is coherent; with a uniform address it
+// becomes one scalar-cache load per wavefront instead of a per-lane L2 access to one line.
+__device__ __forceinline__ double nnpp_ld_scalar(const double* __restrict__ p) { return *p; }
 
 // Reductions run on a fixed grid of at most NNPP_RED_GRID blocks (a multiple of 8); every
 // block walks several 256-node blocks. XCD x (= physical block % 8) owns the contiguous range
@@ -208,51 +213,57 @@ static inline int64_t nnpp_red_grid(int64_t nb) {
   return nb <= NNPP_RED_GRID ? nb : NNPP_RED_GRID;
 }
 
-// Deterministic grid reduction: each block writes K partials; the last block to arrive sums
-// them in block order and returns true (in thread 0) with the totals in v. The ticket resets
-// itself, so no memset is needed between launches. At most NNPP_RED_GRID blocks.
+// Grid reductions in two launches. The producing kernel calls nnpp_block_partial (plain
+// stores of one partial per block) and a one-block finalize kernel (nnpp_finalize) sums them in
+// block order and updates the device scalars. No block waits for or reads another block's data
+// inside a kernel: on gfx950 the 8 XCDs have separate L2s, and an in-kernel agent-scope fence
+// per block would write back and invalidate its XCD's L2. The kernel boundary orders the partial
+// stores before the finalize kernel at the cost of one launch.
 template <int K>
-__device__ __forceinline__ bool nnpp_grid_sum(double (&v)[K], double* part, unsigned* ticket,
-                                              int64_t nblk, int64_t bid) {
+__device__ __forceinline__ void nnpp_block_partial(double (&v)[K], double* __restrict__ part) {
   __shared__ double lds[K * (NNPP_BLOCK / NNPP_WAVE)];
-  __shared__ bool last;
   nnpp_block_sum<K>(v, lds);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < K; k++)
-      __hip_atomic_store(&part[bid * K + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == (unsigned)(nblk - 1));
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // the last block sums the partials: every thread loads its (at most 8) partials at once,
-  // then a fixed-order tree
+    for (int k = 0; k < K; k++) part[(int64_t)blockIdx.x * K + k] = v[k];
+}
+
+// Sum of the partials of nblk (<= NNPP_RED_GRID) blocks in a fixed order; one block of
+// NNPP_BLOCK threads; the totals are valid in thread 0.
+template <int K>
+__device__ __forceinline__ void nnpp_final_sum(double (&v)[K], const double* __restrict__ part,
+                                               int64_t nblk) {
+  __shared__ double lds[K * (NNPP_BLOCK / NNPP_WAVE)];
   constexpr int PER = NNPP_RED_GRID / NNPP_BLOCK;
-  double acc[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
     double t[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       const int64_t b = threadIdx.x + (int64_t)q * NNPP_BLOCK;
-      t[q] = (b < nblk) ? nnpp_ld_coherent(&part[b * K + k]) : 0.0;
+      t[q] = (b < nblk) ? part[b * K + k] : 0.0;
     }
     double a = 0.0;
 #pragma unroll
     for (int q = 0; q < PER; q++) a += t[q];
-    acc[k] = a;
+    v[k] = a;
   }
-  nnpp_block_sum<K>(acc, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; k++) v[k] = acc[k];
-    *ticket = 0u;
-  }
-  return threadIdx.x == 0;
+  nnpp_block_sum<K>(v, lds);
 }
+
+// Finalize operations (what the one-block kernel does with the summed partials).
+enum NnppFin {
+  FIN_BNORM2 = 0,     // K=1: BNORM2
+  FIN_RESID,          // K=2: RR
+  FIN_RESID_START,    // K=2: RHO, RR, TARGET2 = a0 * BNORM2, ITERS = 0, DONE
+  FIN_PQ,             // K=1: PQ, ALPHA = RHO / PQ            (skipped when DONE)
+  FIN_UPDATE,         // K=2: BETA, RHO, RR, ITERS++, DONE at TARGET2 (skipped when DONE)
+  FIN_M_REPLACE,      // K=2: RHO, RR, RRMAX, TARGET2 = a0, DONE = 0
+  FIN_M_REPLACE_START,// K=2: as above, ITERS = 0, DONE = (RR <= a0)
+  FIN_M_UPDATE,       // K=2: as FIN_UPDATE, RRMAX, DONE also at RR <= a0 * RRMAX
+  FIN_FRONT,          // K=3: R0, R1, R2
+  FIN_NONFINITE       // K=1: R3
+};
 
 __host__ __device__ static inline int64_t nnpp_nblocks(int64_t n) { return (n + NNPP_BLOCK - 1) / NNPP_BLOCK; }
 
@@ -271,3 +282,4 @@ int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flip
 int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo);
 int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision);
+int nnpp_finalize(nnpp_ctx* c, int op, int64_t nblk, double a0);
