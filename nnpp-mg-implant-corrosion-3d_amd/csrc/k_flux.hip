@@ -2,14 +2,18 @@
 // one pass over the bonds).
 //
 // One thread per node, 64 consecutive nodes per wavefront reading their bonds from the
-// wave-height slices (coalesced 256 B column reads per bond step; on a lattice the neighbour
-// gathers of the 64 lanes are 64 consecutive nodes too). Per node and species the kernel keeps
+// wave-height slices in the quad layout (one 8-byte load of four 16-bit column steps and one
+// 16/32-byte weight load per four bonds). The fields a bond gathers are staged once per call in
+// an AoS record per node (k_pack), so a neighbour costs R/2 16-byte loads instead of S+1
+// separate 8-byte gathers; on a lattice the 64 lanes gather 64 consecutive records. Per node
+// and species the kernel keeps
 //   sd_k = sum_b w_b (c_k,j - c_k,i)            (diffusion, coefficient applied afterwards)
 //   sm_k = sum_b w_b c_k,j (phi_j - phi_i)      (electromigration)
 // with w_b = Beta(xi) V_j / xi^(3+2s). The diffusivity and BV factor depend only on the bond's
 // phase pair (utils.h:100-159, 211-238), so bonds to nodes of the same phase are summed in a
-// first pass and the few solid-liquid bonds of interface nodes in a second pass, each pass
-// applying one coefficient per species at the end instead of one select per bond.
+// first, branch-free pass (other bonds and padding weigh 0) and the few solid-liquid bonds of
+// interface nodes in a second pass, each pass applying one coefficient per species at the end
+// instead of one select per bond.
 //
 // RESID: F_k = (U1_k - Un_k)/dt + (sum_KR1 + sum_KR2) - R_k, phi row of utils.h:798-841.
 // !RESID: explicit update U_k^{n+1} = U_k^n - dt (sum_KR1 + sum_KR2 - R_k), Dirichlet rows,
@@ -34,10 +38,14 @@ struct FluxArgs {
   int32_t n;
   int32_t chemistry;
   const int64_t* off;
-  const int32_t* len;
+  const int32_t* len;  // groups of 4 entries per lane
   const int32_t* col;
+  const uint16_t* d16;
+  const int32_t* first;
   const double* w64;
   const float* w32;
+  const double* rec;   // AoS gather records (k_pack)
+  double* rec_out;
   const uint8_t* phase;
   const uint16_t* bcmask;
   const double* bcval;
@@ -128,82 +136,151 @@ __device__ double ref_reaction(int k, const RxConst& c, int phase, int intf, con
   return R;
 }
 
-template <int S, bool RESID, bool MIXED>
+// Gather record of node j: [c_0 .. c_{S-1} (iterate), phi^n (lagged), phi^{n+1} (RESID)],
+// padded to an even number of doubles so every record is 16-byte aligned.
+template <int S, bool RESID>
+struct Rec {
+  static constexpr int used = RESID ? S + 2 : S + 1;
+  static constexpr int len = (used + 1) & ~1;
+};
+
+// AoS staging copy of the fields the bonds gather (one pass over the SoA state).
+template <int S, bool RESID>
+__global__ void k_pack(int32_t n, const double* __restrict__ U1, const double* __restrict__ Un,
+                       double* __restrict__ rec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int R = Rec<S, RESID>::len;
+  double v[R];
+#pragma unroll
+  for (int k = 0; k < S; k++) v[k] = U1[(int64_t)k * n + i];
+  v[S] = Un[(int64_t)S * n + i];
+  if (RESID) v[S + 1] = U1[(int64_t)S * n + i];
+#pragma unroll
+  for (int k = Rec<S, RESID>::used; k < R; k++) v[k] = 0.0;
+  double2* out = reinterpret_cast<double2*>(rec + (int64_t)i * R);
+#pragma unroll
+  for (int k = 0; k < R / 2; k++) out[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
+
+template <int S, bool RESID, bool MIXED, bool WIDE>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
   const int64_t i = nnpp_xcd_block(blockIdx.x, gridDim.x) * NNPP_BLOCK + threadIdx.x;
   if (i >= a.n) return;
+  constexpr int R = Rec<S, RESID>::len;
   const int64_t n = a.n;
-  const int64_t base = a.off[i / NNPP_WAVE] + (i % NNPP_WAVE);
-  const int L = a.len[i / NNPP_WAVE];
+  const int64_t base = a.off[i / NNPP_WAVE] + 4 * (i % NNPP_WAVE);
+  const int G = a.len[i / NNPP_WAVE];
   const int pi = a.phase[i];
-  const double* __restrict__ U1 = a.U1;
-  const double* __restrict__ Un = a.Un;
-  const double* __restrict__ phin = Un + (int64_t)S * n;
+  const double* __restrict__ rec = a.rec;
 
   double ci[S];
+  double phii, phi1i = 0.0;
+  {
+    const double2* ri = reinterpret_cast<const double2*>(rec + i * R);
+    double v[R];
 #pragma unroll
-  for (int k = 0; k < S; k++) ci[k] = U1[(int64_t)k * n + i];
-  const double phii = phin[i];
-  const double phi1i = RESID ? U1[(int64_t)S * n + i] : 0.0;
+    for (int k = 0; k < R / 2; k++) {
+      const double2 t = ri[k];
+      v[2 * k] = t.x;
+      v[2 * k + 1] = t.y;
+    }
+#pragma unroll
+    for (int k = 0; k < S; k++) ci[k] = v[k];
+    phii = v[S];
+    if (RESID) phi1i = v[S + 1];
+  }
 
   double acc[S];
   int mixed = 0;
   double sphi = 0.0;
   int flagLL = 0;
 
-  // pass 1: same-phase bonds; pass 2 (interface nodes only): solid-liquid bonds
+  // pass 0: same-phase bonds (branch-free: other bonds get weight 0); pass 1 (interface nodes
+  // only): solid-liquid bonds
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1 && !mixed) break;
     double sd[S], sm[S];
     float fd[S], fm[S], cd[S], cm[S];
 #pragma unroll
     for (int k = 0; k < S; k++) {
-      sd[k] = 0.0;
-      sm[k] = 0.0;
-      fd[k] = 0.0f;
-      fm[k] = 0.0f;
-      cd[k] = 0.0f;
-      cm[k] = 0.0f;
+      sd[k] = sm[k] = 0.0;
+      fd[k] = fm[k] = cd[k] = cm[k] = 0.0f;
     }
-    for (int t = 0; t < L; t++) {
-      const int64_t slot = base + (int64_t)t * NNPP_WAVE;
-      const int32_t j = a.col[slot];
-      if (j < 0) break;
-      const int pj = a.phase[j];
-      if (pass == 0) {
-        if (pj != pi) {
-          mixed = 1;
-          continue;
-        }
-      } else if (pj == pi) {
-        continue;
-      }
-      const double phij = phin[j];
-      if (MIXED) {
-        const float w = a.w32[slot];
-        const float dphi = (float)(phij - phii);
-#pragma unroll
-        for (int k = 0; k < S; k++) {
-          const double cj = U1[(int64_t)k * n + j];
-          kahan_add(fd[k], cd[k], w * (float)(cj - ci[k]));
-          kahan_add(fm[k], cm[k], (w * (float)cj) * dphi);
-        }
+    int32_t jr = WIDE ? 0 : a.first[i];
+    for (int gq = 0; gq < G; gq++) {
+      const int64_t s0 = base + 256 * (int64_t)gq;
+      int32_t cq[4];
+      if (WIDE) {
+        const int4 q = *reinterpret_cast<const int4*>(a.col + s0);
+        cq[0] = q.x;
+        cq[1] = q.y;
+        cq[2] = q.z;
+        cq[3] = q.w;
       } else {
-        const double w = a.w64[slot];
-        const double wdphi = w * (phij - phii);
-#pragma unroll
-        for (int k = 0; k < S; k++) {
-          const double cj = U1[(int64_t)k * n + j];
-          sd[k] += w * (cj - ci[k]);
-          sm[k] += wdphi * cj;
-        }
+        const uint2 d = *reinterpret_cast<const uint2*>(a.d16 + s0);
+        cq[0] = jr + (int32_t)(d.x & 0xffffu);
+        cq[1] = cq[0] + (int32_t)(d.x >> 16);
+        cq[2] = cq[1] + (int32_t)(d.y & 0xffffu);
+        cq[3] = cq[2] + (int32_t)(d.y >> 16);
+        jr = cq[3];
       }
-      if (RESID && pass == 0 && pi == 0) {
-        if (MIXED)
-          sphi += (double)(a.w32[slot] * (float)(U1[(int64_t)S * n + j] - phi1i));
-        else
-          sphi += a.w64[slot] * (U1[(int64_t)S * n + j] - phi1i);
-        flagLL = 1;
+      double wq[4];
+      if (MIXED) {
+        const float4 t = *reinterpret_cast<const float4*>(a.w32 + s0);
+        wq[0] = t.x;
+        wq[1] = t.y;
+        wq[2] = t.z;
+        wq[3] = t.w;
+      } else {
+        const double2 t0 = *reinterpret_cast<const double2*>(a.w64 + s0);
+        const double2 t1 = *reinterpret_cast<const double2*>(a.w64 + s0 + 2);
+        wq[0] = t0.x;
+        wq[1] = t0.y;
+        wq[2] = t1.x;
+        wq[3] = t1.y;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int32_t j = cq[e];
+        const int pj = a.phase[j];
+        const bool same = (pj == pi);
+        double w;
+        if (pass == 0) {
+          mixed |= (!same && wq[e] != 0.0);
+          w = same ? wq[e] : 0.0;
+        } else {
+          if (same || wq[e] == 0.0) continue;  // rare: interface nodes only
+          w = wq[e];
+        }
+        const double2* rj = reinterpret_cast<const double2*>(rec + (int64_t)j * R);
+        double v[R];
+#pragma unroll
+        for (int k = 0; k < R / 2; k++) {
+          const double2 t = rj[k];
+          v[2 * k] = t.x;
+          v[2 * k + 1] = t.y;
+        }
+        const double dphi = v[S] - phii;
+        if (MIXED) {
+          const float wf = (float)w, dpf = (float)dphi;
+#pragma unroll
+          for (int k = 0; k < S; k++) {
+            kahan_add(fd[k], cd[k], wf * (float)(v[k] - ci[k]));
+            kahan_add(fm[k], cm[k], (wf * (float)v[k]) * dpf);
+          }
+        } else {
+          const double wdphi = w * dphi;
+#pragma unroll
+          for (int k = 0; k < S; k++) {
+            sd[k] += w * (v[k] - ci[k]);
+            sm[k] += wdphi * v[k];
+          }
+        }
+        if (RESID && pass == 0 && pi == 0) {  // liquid-liquid bond of the phi row
+          sphi += w * (v[S + 1] - phi1i);
+          flagLL |= (w != 0.0);
+        }
       }
     }
 #pragma unroll
@@ -212,10 +289,9 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
         sd[k] = (double)fd[k];
         sm[k] = (double)fm[k];
       }
-      double A, M;
       if (pass == 0) {
-        A = pi ? a.A_SS[k] : a.A_LL[k];
-        M = pi ? a.M_SS[k] : a.M_LL[k];
+        const double A = pi ? a.A_SS[k] : a.A_LL[k];
+        const double M = pi ? a.M_SS[k] : a.M_LL[k];
         acc[k] = A * sd[k] + M * sm[k];
       } else {
         acc[k] += a.A_SL[k] * sd[k] + a.M_SL[k] * sm[k];
@@ -224,23 +300,26 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
   }
 
   const int intf = (pi == 1 && mixed) ? 1 : 0;  // utils.h:82-89
+  const double* __restrict__ U1 = a.U1;
+  const double* __restrict__ Un = a.Un;
   double un[S];
 #pragma unroll
   for (int k = 0; k < S; k++) un[k] = RESID ? Un[(int64_t)k * n + i] : ci[k];
   const unsigned bm = a.bcmask[i];
+  (void)U1;
 
 #pragma unroll
   for (int k = 0; k < S; k++) {
-    double R = 0.0;
+    double Rk = 0.0;
     if constexpr (S == 10) {
-      if (a.chemistry == NNPP_CHEM_REFERENCE) R = ref_reaction(k, a.rx, pi, intf, ci, un, phii);
+      if (a.chemistry == NNPP_CHEM_REFERENCE) Rk = ref_reaction(k, a.rx, pi, intf, ci, un, phii);
     }
-    if (a.chemistry == NNPP_CHEM_GENERIC && k == 0 && intf) R = -a.r_diss * ci[0];
+    if (a.chemistry == NNPP_CHEM_GENERIC && k == 0 && intf) Rk = -a.r_diss * ci[0];
     const double flux = -acc[k];  // sum_KR_k1 + sum_KR_k2
     if (RESID) {
-      a.out[(int64_t)k * n + i] = ((ci[k] - un[k]) / a.dt) + flux - R;
+      a.out[(int64_t)k * n + i] = ((ci[k] - un[k]) / a.dt) + flux - Rk;
     } else {
-      double v = un[k] - a.dt * (flux - R);
+      double v = un[k] - a.dt * (flux - Rk);
       if ((bm >> k) & 1u) v = a.bcval[(int64_t)k * n + i];
       const bool cl = ((a.clamp_always >> k) & 1u) || (((a.clamp_liquid >> k) & 1u) && pi == 0);
       if (cl && v < a.clamp_floor) v = a.clamp_floor;
@@ -260,14 +339,20 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
 }
 
 template <int S>
-hipError_t launch_s(const FluxArgs& a, int resid, int mixed, int64_t nb, hipStream_t s) {
+hipError_t launch_s(const FluxArgs& a, int resid, int mixed, int wide, int64_t nb, hipStream_t s) {
+  // stage the gathered fields, then the bond pass
+  const int64_t npk = (a.n + NNPP_BLOCK - 1) / NNPP_BLOCK;
+  if (resid) k_pack<S, true><<<npk, NNPP_BLOCK, 0, s>>>(a.n, a.U1, a.Un, a.rec_out);
+  else k_pack<S, false><<<npk, NNPP_BLOCK, 0, s>>>(a.n, a.U1, a.Un, a.rec_out);
+#define NNPP_FLUX(R_, M_, W_) k_flux<S, R_, M_, W_><<<nb, NNPP_BLOCK, 0, s>>>(a)
   if (resid) {
-    if (mixed) k_flux<S, true, true><<<nb, NNPP_BLOCK, 0, s>>>(a);
-    else k_flux<S, true, false><<<nb, NNPP_BLOCK, 0, s>>>(a);
+    if (mixed) { if (wide) NNPP_FLUX(true, true, true); else NNPP_FLUX(true, true, false); }
+    else { if (wide) NNPP_FLUX(true, false, true); else NNPP_FLUX(true, false, false); }
   } else {
-    if (mixed) k_flux<S, false, true><<<nb, NNPP_BLOCK, 0, s>>>(a);
-    else k_flux<S, false, false><<<nb, NNPP_BLOCK, 0, s>>>(a);
+    if (mixed) { if (wide) NNPP_FLUX(false, true, true); else NNPP_FLUX(false, true, false); }
+    else { if (wide) NNPP_FLUX(false, false, true); else NNPP_FLUX(false, false, false); }
   }
+#undef NNPP_FLUX
   return hipGetLastError();
 }
 
@@ -283,8 +368,12 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   a.off = c->bonds.off;
   a.len = c->bonds.len;
   a.col = c->bonds.col;
+  a.d16 = c->bonds.d16;
+  a.first = c->bonds.first;
   a.w64 = c->bonds.w64;
   a.w32 = c->bonds.w32;
+  a.rec = c->aos;
+  a.rec_out = c->aos;
   a.phase = c->phase;
   a.bcmask = c->bcmask;
   a.bcval = c->bcval;
@@ -346,7 +435,7 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   hipError_t e = hipSuccess;
   switch (S) {
 #define NNPP_CASE(SS) \
-  case SS: e = launch_s<SS>(a, resid, mixed, nb, c->stream); break;
+  case SS: e = launch_s<SS>(a, resid, mixed, c->bonds.wide, nb, c->stream); break;
     NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
     NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
     NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)

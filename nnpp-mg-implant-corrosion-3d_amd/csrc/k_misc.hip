@@ -58,7 +58,7 @@ __global__ void k_slice_len(int32_t n, const int64_t* __restrict__ row_ptr,
                             int32_t* __restrict__ len) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  atomicMax(&len[i / NNPP_WAVE], (int32_t)(row_ptr[i + 1] - row_ptr[i]));
+  atomicMax(&len[i / NNPP_WAVE], (int32_t)((row_ptr[i + 1] - row_ptr[i] + 3) / 4));
 }
 
 __global__ void k_times(int64_t ns, const int32_t* __restrict__ len, int32_t mult,
@@ -81,34 +81,48 @@ __device__ __forceinline__ double bond_weight(const WeightArgs& wa, const double
   return ((1.0 / pow(xi, wa.expo)) * nnpp_beta(xi, wa.delta, wa.dx)) * V;
 }
 
-// Fill the bond slices from CSR: entry t of row i at off[i/64] + 64 t + i%64.
+// Fill the bond slices from CSR in the quad layout (entry t of row i at
+// off[i/64] + 256 (t/4) + 4 (i%64) + t%4), with 16-bit column steps from first[i]; padding has
+// weight 0 and repeats the last column. Steps above 65535 raise *maxstep.
 __global__ void k_slice_fill(int32_t n, int64_t nrows_pad, const int64_t* __restrict__ row_ptr,
                              const int32_t* __restrict__ col, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
                              WeightArgs wa, const int64_t* __restrict__ off,
-                             const int32_t* __restrict__ len, int32_t* __restrict__ scol,
-                             double* __restrict__ w64, float* __restrict__ w32) {
+                             const int32_t* __restrict__ len, int32_t* __restrict__ first,
+                             int32_t* __restrict__ scol, uint16_t* __restrict__ d16,
+                             double* __restrict__ w64, float* __restrict__ w32,
+                             int* __restrict__ maxstep) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows_pad) return;
   const int64_t c = i / NNPP_WAVE;
-  const int64_t base = off[c] + (i % NNPP_WAVE);
-  const int L = len[c];
+  const int64_t base = off[c] + 4 * (i % NNPP_WAVE);
+  const int L = 4 * len[c];
   const int64_t b0 = (i < n) ? row_ptr[i] : 0;
   const int rl = (i < n) ? (int)(row_ptr[i + 1] - b0) : 0;
+  int32_t prev = (int32_t)(i < n ? i : 0);
+  if (rl > 0) prev = col[b0];
+  if (i < n) first[i] = prev;
+  int big = 0;
   for (int t = 0; t < L; t++) {
-    const int64_t slot = base + (int64_t)t * NNPP_WAVE;
+    const int64_t slot = base + 256 * (int64_t)(t >> 2) + (t & 3);
     if (t < rl) {
       const int32_t j = col[b0 + t];
       const double w = bond_weight(wa, x, y, z, i, j);
+      const int step = j - prev;
+      big = step > big ? step : big;
       scol[slot] = j;
+      d16[slot] = (uint16_t)(step & 0xffff);
       w64[slot] = w;
       w32[slot] = (float)w;
+      prev = j;
     } else {
-      scol[slot] = -1;
+      scol[slot] = prev;
+      d16[slot] = 0;
       w64[slot] = 0.0;
       w32[slot] = 0.0f;
     }
   }
+  if (big > 65535) atomicMax(maxstep, big);
 }
 
 // Potential operator, step 1: free flags (liquid, not Dirichlet for phi, >= 1 liquid bond).
@@ -452,15 +466,25 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
   NNPP_HIP(c, hipMemsetAsync(c->bonds.len, 0, sizeof(int32_t) * (ns ? ns : 1), s));
   if (n) k_slice_len<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->row_ptr, c->bonds.len);
   NNPP_HIP(c, hipGetLastError());
-  st = slices_offsets(c, c->bonds, NNPP_WAVE);
+  st = slices_offsets(c, c->bonds, 4 * NNPP_WAVE);
   if (st) return st;
+  if (!c->bonds.first)
+    NNPP_HIP(c, hipMalloc((void**)&c->bonds.first, sizeof(int32_t) * (n ? n : 1)));
+  int* dmax = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&dmax, sizeof(int)));
+  NNPP_HIP(c, hipMemsetAsync(dmax, 0, sizeof(int), s));
   const int64_t npad = ns * NNPP_WAVE;
   if (npad)
     k_slice_fill<<<nnpp_nblocks(npad), NNPP_BLOCK, 0, s>>>(
         n, npad, c->row_ptr, c->col, c->x, c->y, c->z, weight_args(c, c->w_exponent),
-        c->bonds.off, c->bonds.len, c->bonds.col, c->bonds.w64, c->bonds.w32);
+        c->bonds.off, c->bonds.len, c->bonds.first, c->bonds.col, c->bonds.d16, c->bonds.w64,
+        c->bonds.w32, dmax);
   NNPP_HIP(c, hipGetLastError());
+  int hmax = 0;
+  NNPP_HIP(c, hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, s));
   NNPP_HIP(c, hipStreamSynchronize(s));
+  hipFree(dmax);
+  c->bonds.wide = hmax > 65535 ? 1 : 0;
   c->bonds.nnz = c->nnz;
   c->geom_dirty = false;
   return NNPP_OK;

@@ -53,6 +53,7 @@ void free_nodes(nnpp_ctx* c) {
   dfree(c->U[1]);
   dfree(c->Unew);
   dfree(c->Fres);
+  dfree(c->aos);
   dfree(c->pfree);
   dfree(c->pdd);
   dfree(c->pdd32);
@@ -195,6 +196,7 @@ int nnpp_set_nodes(nnpp_ctx* c, int32_t n, const double* x, const double* y, con
   NNPP_HIP(c, hipMalloc((void**)&c->U[1], sizeof(double) * m * F));
   NNPP_HIP(c, hipMalloc((void**)&c->Unew, sizeof(double) * m * F));
   NNPP_HIP(c, hipMalloc((void**)&c->Fres, sizeof(double) * m * F));
+  NNPP_HIP(c, hipMalloc((void**)&c->aos, sizeof(double) * m * ((F + 2) & ~(size_t)1)));
   NNPP_HIP(c, hipMalloc((void**)&c->pfree, m));
   NNPP_HIP(c, hipMalloc((void**)&c->pdd, sizeof(double) * m));
   NNPP_HIP(c, hipMalloc((void**)&c->pdd32, sizeof(float) * m));

@@ -80,6 +80,7 @@ struct nnpp_ctx {
   double* U[2] = {nullptr, nullptr};  // slots N, N1
   double* Unew = nullptr;             // F*n scratch of the explicit step
   double* Fres = nullptr;             // F*n residual output
+  double* aos = nullptr;              // n * (S+3 rounded to even) AoS gather records
 
   // potential solve
   uint8_t* pfree = nullptr;
