@@ -209,6 +209,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--case", choices=["regular", "screw"], default="regular",
+                    help="regular lattice with a Mg sphere (default) or the jittered screw "
+                         "geometry with per-node volumes")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -239,8 +242,8 @@ def main():
     if a.kernels_only:
         kernels_only(nn, nn.synth.regular(ng, S=11, seed=0), 20)
         return
-    log(f"[bench] building {ng}^3 case")
-    case = nn.synth.regular(ng, S=11, seed=0)
+    log(f"[bench] building {a.case} {ng}^3 case")
+    case = (nn.synth.regular if a.case == "regular" else nn.synth.screw)(ng, S=11, seed=0)
     # neighbour list built on the GPU once, shared by both runs
     pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
     t0 = time.perf_counter()
@@ -278,12 +281,14 @@ def main():
     t_spmv64 = pbm.time_kernel(nn._abi.KERNEL_POT_SPMV, nn.FP64, reps)
     t_flux64 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.FP64, 10)
     t_flux32 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.MIXED, 10)
-    # algorithmic bytes per launch of the potential SpMV: a 16-bit column step + the weight per
-    # liquid bond; per node the first column (4), p_i (gathered p_j counted once), dd_i, pinv
-    # not read, q_i written
-    cb = 4 if mixed["stats"].get("wide", 0) else 2
-    bytes32 = nll * (cb + 4) + n * (4 + 3 * 4)
-    bytes64 = nll * (cb + 8) + n * (4 + 3 * 8)
+    # algorithmic bytes per launch of the potential SpMV: per liquid bond the stored column (a
+    # 16-bit step, or 4 bytes) and the weight (a 1-byte dictionary class, or the FP32/FP64
+    # value); per node the first column (4), p_i (gathered p_j counted once), dd_i, q_i written
+    cb = mixed["stats"]["pot_col_bytes"]
+    wb32 = mixed["stats"]["pot_weight_bytes"]
+    wb64 = 1 if wb32 == 1 else 8
+    bytes32 = nll * (cb + wb32) + n * (4 + 3 * 4)
+    bytes64 = nll * (cb + wb64) + n * (4 + 3 * 8)
     S = case.S
     fbytes = lambda wb: nnz * (4 + wb) + n * (8 * (S + 1) + 1 + 2 + 8 * S)  # noqa: E731
     ach = bytes32 / (t_spmv32 * 1e-3) / 1e9
@@ -314,8 +319,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 bond sums (Kahan) / f64 state, dots, residual checks",
         "data": "synthetic (seeded lattice case in the reference's input layout)",
-        "config": {"workload": f"regular {ng}^3 lattice, 11 species + phi, delta=3dx, Mg sphere "
-                               f"r=0.3, Dirichlet faces, CG rtol {a.rtol:g}",
+        "config": {"workload": (f"regular {ng}^3 lattice, 11 species + phi, delta=3dx, Mg sphere "
+                                f"r=0.3, Dirichlet faces, CG rtol {a.rtol:g}" if a.case == "regular"
+                                else f"jittered {ng}^3 lattice in a fluid cylinder around a screw-"
+                                     f"like solid, per-node volumes, 11 species + phi, "
+                                     f"CG rtol {a.rtol:g}"),
                    "nodes": n, "bonds": nnz, "liquid_bonds": nll, "species": S},
         "fp64": {"value": value64, "ms_per_step": fp64["time_s"] / a.steps * 1e3,
                  "cg_iterations": fp64["stats"]["cg_iterations"],
@@ -331,7 +339,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_m_spmv"),
                      "kernel": "k_m_spmv (FP32 potential SpMV + dot)",
-                     "bytes_per_launch": bytes32},
+                     "bytes_per_launch": bytes32,
+                     "operator_format": {"col_bytes": cb, "weight_bytes": wb32}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -121,6 +121,9 @@ typedef struct nnpp_step_stats {
   double voln1rel;          /* current relative Mg volume feeding BV */
   int64_t nnz_flux;         /* directed bonds per flux evaluation */
   int64_t nnz_ll;           /* liquid-liquid directed bonds in the last Poisson operator */
+  int32_t pot_col_bytes;    /* bytes per stored column of that operator: 2 (steps) or 4 */
+  int32_t pot_weight_bytes; /* bytes per stored weight read in mixed mode: 1 (dictionary
+                               class) or 4 (FP32 value) */
 } nnpp_step_stats;
 
 typedef struct nnpp_ctx nnpp_ctx;
@@ -248,6 +251,17 @@ int nnpp_poisson_apply(nnpp_ctx* ctx, const double* x, double* y);
  * relative residual rtol (default 1e-10), maxit (default 20000); for NNPP_MIXED the inner
  * FP32 solve reduces the residual by inner_rtol per refinement pass (default 1e-4). */
 int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol);
+
+/* Storage format of the bond slices and the potential operator. NNPP_FORMAT_AUTO (default):
+ * 16-bit column steps unless a row needs a larger step, a 1-byte weight-class dictionary for
+ * the potential operator when it has at most 255 distinct nonzero weights. The flags force
+ * 32-bit columns and/or plain FP64/FP32 weights. Every format gives bit-identical results;
+ * they differ in bytes read per bond. No reference counterpart (the reference stores an
+ * Eigen::SparseMatrix, utils.h:1794-1848). Takes effect at the next operator build. */
+#define NNPP_FORMAT_AUTO 0
+#define NNPP_FORMAT_WIDE_COLUMNS 1
+#define NNPP_FORMAT_NO_DICTIONARY 2
+int nnpp_set_format(nnpp_ctx* ctx, int32_t flags);
 
 /* Solve the potential for the charge of slot N: phi^{n+1} on free nodes (liquid, with a
  * liquid-liquid bond, not Dirichlet), Dirichlet phi = bc, other nodes keep their value.

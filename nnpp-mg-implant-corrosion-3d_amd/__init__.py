@@ -4,12 +4,14 @@ Python host mirror over libnnpp.so's C ABI (include/nnpp.h). The compute path is
 library; importing this package loads it and fails if it is missing.
 """
 from . import _abi
-from ._abi import (CHEM_GENERIC, CHEM_REFERENCE, FP64, MIXED, SLOT_N, SLOT_N1, NnppError,
-                   Params, StepStats)
+from ._abi import (CHEM_GENERIC, CHEM_REFERENCE, FORMAT_AUTO, FORMAT_NO_DICTIONARY,
+                   FORMAT_WIDE_COLUMNS, FP64, MIXED, SLOT_N, SLOT_N1, NnppError, Params,
+                   StepStats)
 from .problem import Problem
 from . import synth, formats
 
 lib = _abi.load()
 
-__all__ = ["CHEM_GENERIC", "CHEM_REFERENCE", "FP64", "MIXED", "SLOT_N", "SLOT_N1", "NnppError",
+__all__ = ["CHEM_GENERIC", "CHEM_REFERENCE", "FORMAT_AUTO", "FORMAT_NO_DICTIONARY",
+           "FORMAT_WIDE_COLUMNS", "FP64", "MIXED", "SLOT_N", "SLOT_N1", "NnppError",
            "Params", "StepStats", "Problem", "synth", "formats", "lib"]

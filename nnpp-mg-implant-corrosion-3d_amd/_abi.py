@@ -18,6 +18,7 @@ FP64, MIXED = 0, 1
 CLAMP_NONE, CLAMP_ALWAYS, CLAMP_LIQUID = 0, 1, 2
 SLOT_N, SLOT_N1 = 0, 1
 KERNEL_FLUX, KERNEL_POT_SPMV = 0, 1
+FORMAT_AUTO, FORMAT_WIDE_COLUMNS, FORMAT_NO_DICTIONARY = 0, 1, 2
 
 # .data2 field names in file order (utils.h:2348-2398); pKH2O is the reference's pKH20
 DATA2_FIELDS = [
@@ -72,7 +73,7 @@ class StepStats(C.Structure):
         ("steps", C.c_int64), ("cg_iterations", C.c_int64), ("cg_spmv_fp64", C.c_int64),
         ("cg_spmv_fp32", C.c_int64), ("last_relres", C.c_double), ("flipped", C.c_int64),
         ("solid_count", C.c_int64), ("voln1rel", C.c_double), ("nnz_flux", C.c_int64),
-        ("nnz_ll", C.c_int64),
+        ("nnz_ll", C.c_int64), ("pot_col_bytes", C.c_int32), ("pot_weight_bytes", C.c_int32),
     ]
 
     def as_dict(self):
@@ -121,6 +122,7 @@ SIGNATURES = {
     "nnpp_residuals": (C.c_int, [_p, _d, _i32, _i32, _dp]),
     "nnpp_poisson_apply": (C.c_int, [_p, _dp, _dp]),
     "nnpp_set_solver": (C.c_int, [_p, _d, _i32, _d]),
+    "nnpp_set_format": (C.c_int, [_p, _i32]),
     "nnpp_poisson_solve": (C.c_int, [_p, _i32, _ip, _dp]),
     "nnpp_step": (C.c_int, [_p, _i32, _i32, _i32, C.POINTER(StepStats)]),
     "nnpp_front_update": (C.c_int, [_p, _d, _ip]),

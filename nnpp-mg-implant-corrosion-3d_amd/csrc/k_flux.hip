@@ -212,13 +212,13 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
       const int64_t s0 = base + 256 * (int64_t)gq;
       int32_t cq[4];
       if (WIDE) {
-        const int4 q = *reinterpret_cast<const int4*>(a.col + s0);
+        const int4 q = nnpp_ld_stream(reinterpret_cast<const int4*>(a.col + s0));
         cq[0] = q.x;
         cq[1] = q.y;
         cq[2] = q.z;
         cq[3] = q.w;
       } else {
-        const uint2 d = *reinterpret_cast<const uint2*>(a.d16 + s0);
+        const uint2 d = nnpp_ld_stream(reinterpret_cast<const uint2*>(a.d16 + s0));
         cq[0] = jr + (int32_t)(d.x & 0xffffu);
         cq[1] = cq[0] + (int32_t)(d.x >> 16);
         cq[2] = cq[1] + (int32_t)(d.y & 0xffffu);
@@ -227,14 +227,14 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
       }
       double wq[4];
       if (MIXED) {
-        const float4 t = *reinterpret_cast<const float4*>(a.w32 + s0);
+        const float4 t = nnpp_ld_stream(reinterpret_cast<const float4*>(a.w32 + s0));
         wq[0] = t.x;
         wq[1] = t.y;
         wq[2] = t.z;
         wq[3] = t.w;
       } else {
-        const double2 t0 = *reinterpret_cast<const double2*>(a.w64 + s0);
-        const double2 t1 = *reinterpret_cast<const double2*>(a.w64 + s0 + 2);
+        const double2 t0 = nnpp_ld_stream(reinterpret_cast<const double2*>(a.w64 + s0));
+        const double2 t1 = nnpp_ld_stream(reinterpret_cast<const double2*>(a.w64 + s0 + 2));
         wq[0] = t0.x;
         wq[1] = t0.y;
         wq[2] = t1.x;

@@ -1,6 +1,8 @@
 // k_misc.hip — device exclusive scan, bond-slice construction, front update (main.cpp:915-922
 // plus the volume sum of main.cpp:944-949), Dirichlet residual rows (main.cpp:676-711) and
 // the non-finite check.
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "nnpp_internal.hpp"
@@ -364,6 +366,64 @@ __global__ void k_pot_apply_csr(int32_t n, const int64_t* __restrict__ row_ptr,
   out[i] = acc;
 }
 
+constexpr int DICT_HASH = 4096;
+
+// Collect the distinct nonzero FP64 weights in a small open-addressing hash set; gives up
+// (flag) as soon as a probe chain gets long or too many keys arrive.
+__global__ void k_dict_insert(int64_t slots, const double* __restrict__ w,
+                              unsigned long long* __restrict__ table, int* __restrict__ count,
+                              int* __restrict__ fail) {
+  unsigned long long last = 0;
+  int64_t seen = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < slots;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    if ((++seen & 63) == 0 && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
+    const double v = w[k];
+    if (v == 0.0) continue;
+    const unsigned long long key = (unsigned long long)__double_as_longlong(v);
+    if (key == last) continue;
+    last = key;
+    unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 52);
+    int probe = 0;
+    for (; probe < 32; probe++, h = (h + 1) & (DICT_HASH - 1)) {
+      const unsigned long long cur = table[h];
+      if (cur == key) break;
+      if (cur == 0ull) {
+        const unsigned long long prev = atomicCAS(&table[h], 0ull, key);
+        if (prev == 0ull) {
+          if (atomicAdd(count, 1) >= NNPP_DICT_MAX - 1) atomicExch(fail, 1);
+          break;
+        }
+        if (prev == key) break;
+      }
+    }
+    if (probe == 32) {
+      atomicExch(fail, 1);
+      return;
+    }
+  }
+}
+
+// Class of every entry: index of its weight in the sorted table (class 0 = weight 0).
+__global__ void k_dict_map(int64_t slots, const double* __restrict__ w,
+                           const double* __restrict__ tab, int ntab, uint8_t* __restrict__ cls) {
+  __shared__ double t[NNPP_DICT_MAX];
+  for (int k = threadIdx.x; k < NNPP_DICT_MAX; k += blockDim.x) t[k] = k < ntab ? tab[k] : 0.0;
+  __syncthreads();
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < slots;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double v = w[k];
+    int lo = 0, hi = ntab - 1;  // t is sorted ascending and contains v
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (t[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    cls[k] = (uint8_t)lo;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host launchers ---------
@@ -418,6 +478,8 @@ static int slices_reserve(nnpp_ctx* c, nnpp_slices& sl, int64_t slots) {
     if (sl.w64) hipFree(sl.w64);
     if (sl.w32) hipFree(sl.w32);
     if (sl.d16) hipFree(sl.d16);
+    if (sl.cls) hipFree(sl.cls);
+    sl.cls = nullptr;
     sl.col = nullptr;
     sl.w64 = nullptr;
     sl.w32 = nullptr;
@@ -484,7 +546,7 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
   NNPP_HIP(c, hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, s));
   NNPP_HIP(c, hipStreamSynchronize(s));
   hipFree(dmax);
-  c->bonds.wide = hmax > 65535 ? 1 : 0;
+  c->bonds.wide = (hmax > 65535 || (c->format & NNPP_FORMAT_WIDE_COLUMNS)) ? 1 : 0;
   c->bonds.nnz = c->nnz;
   c->geom_dirty = false;
   return NNPP_OK;
@@ -537,7 +599,12 @@ int nnpp_build_potential(nnpp_ctx* c) {
   NNPP_HIP(c, hipStreamSynchronize(s));
   hipFree(rcount);
   hipFree(dmax);
-  c->pot.wide = hmax > 65535 ? 1 : 0;
+  c->pot.wide = (hmax > 65535 || (c->format & NNPP_FORMAT_WIDE_COLUMNS)) ? 1 : 0;
+  c->pot.ncls = 0;
+  if (!(c->format & NNPP_FORMAT_NO_DICTIONARY)) {
+    st = nnpp_build_dictionary(c, c->pot);
+    if (st) return st;
+  }
   int64_t nnz = 0, nf = 0;
   for (int32_t i = 0; i < n; i++) {
     nnz += rc[i];
@@ -608,5 +675,52 @@ int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out) {
 int nnpp_finalize(nnpp_ctx* c, int op, int64_t nblk, double a0) {
   k_finalize<<<1, NNPP_BLOCK, 0, c->stream>>>(op, c->red_part, nblk, c->scal, a0);
   NNPP_HIP(c, hipGetLastError());
+  return NNPP_OK;
+}
+
+int nnpp_build_dictionary(nnpp_ctx* c, nnpp_slices& sl) {
+  sl.ncls = 0;
+  if (!sl.slots) return NNPP_OK;
+  hipStream_t s = c->stream;
+  unsigned long long* table = nullptr;
+  int* flags = nullptr;  // [0] = count, [1] = fail
+  NNPP_HIP(c, hipMalloc((void**)&table, sizeof(unsigned long long) * DICT_HASH));
+  NNPP_HIP(c, hipMalloc((void**)&flags, sizeof(int) * 2));
+  NNPP_HIP(c, hipMemsetAsync(table, 0, sizeof(unsigned long long) * DICT_HASH, s));
+  NNPP_HIP(c, hipMemsetAsync(flags, 0, sizeof(int) * 2, s));
+  k_dict_insert<<<1024, NNPP_BLOCK, 0, s>>>(sl.slots, sl.w64, table, flags, flags + 1);
+  NNPP_HIP(c, hipGetLastError());
+  std::vector<unsigned long long> ht(DICT_HASH);
+  int hf[2] = {0, 0};
+  NNPP_HIP(c, hipMemcpyAsync(ht.data(), table, sizeof(unsigned long long) * DICT_HASH,
+                             hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(hf, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  hipFree(table);
+  hipFree(flags);
+  if (hf[1] || hf[0] > NNPP_DICT_MAX - 1) return NNPP_OK;  // not a small set: keep FP weights
+  std::vector<double> tab(1, 0.0);
+  for (unsigned long long k : ht) {
+    if (!k) continue;
+    double v;
+    std::memcpy(&v, &k, sizeof(v));
+    tab.push_back(v);
+  }
+  std::sort(tab.begin(), tab.end());
+  const int ntab = (int)tab.size();
+  if (ntab > NNPP_DICT_MAX || tab[1] <= 0.0) return NNPP_OK;  // weights are positive
+  std::vector<float> tab32(ntab);
+  for (int k = 0; k < ntab; k++) tab32[k] = (float)tab[k];
+  if (!sl.tab64) NNPP_HIP(c, hipMalloc((void**)&sl.tab64, sizeof(double) * NNPP_DICT_MAX));
+  if (!sl.tab32) NNPP_HIP(c, hipMalloc((void**)&sl.tab32, sizeof(float) * NNPP_DICT_MAX));
+  if (!sl.cls) NNPP_HIP(c, hipMalloc((void**)&sl.cls, sl.cap_slots));
+  NNPP_HIP(c, hipMemsetAsync(sl.tab64, 0, sizeof(double) * NNPP_DICT_MAX, s));
+  NNPP_HIP(c, hipMemsetAsync(sl.tab32, 0, sizeof(float) * NNPP_DICT_MAX, s));
+  NNPP_HIP(c, hipMemcpyAsync(sl.tab64, tab.data(), sizeof(double) * ntab, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(sl.tab32, tab32.data(), sizeof(float) * ntab, hipMemcpyHostToDevice, s));
+  k_dict_map<<<2048, NNPP_BLOCK, 0, s>>>(sl.slots, sl.w64, sl.tab64, ntab, sl.cls);
+  NNPP_HIP(c, hipGetLastError());
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  sl.ncls = ntab;
   return NNPP_OK;
 }

@@ -35,6 +35,9 @@ struct Slices {
   const int32_t* col;
   const uint16_t* d16;
   const int32_t* first;
+  const uint8_t* cls;   // weight classes (dictionary variant)
+  const double* tab64;  // dictionary, FP64 / FP32
+  const float* tab32;
 };
 
 __device__ __forceinline__ bool cg_done(const double* scal) {
@@ -43,16 +46,18 @@ __device__ __forceinline__ bool cg_done(const double* scal) {
 
 // Columns of the next 4 entries of a row: one 16-byte load (wide) or one 8-byte load of
 // 16-bit steps accumulated onto the running column j.
-template <bool WIDE>
+template <bool WIDE, bool NT>
 __device__ __forceinline__ void quad_cols(const Slices& sl, int64_t s0, int32_t& j, int32_t (&c)[4]) {
   if (WIDE) {
-    const int4 q = *reinterpret_cast<const int4*>(sl.col + s0);
+    const int4* p = reinterpret_cast<const int4*>(sl.col + s0);
+    const int4 q = NT ? nnpp_ld_stream(p) : *p;
     c[0] = q.x;
     c[1] = q.y;
     c[2] = q.z;
     c[3] = q.w;
   } else {
-    const uint2 d = *reinterpret_cast<const uint2*>(sl.d16 + s0);
+    const uint2* p = reinterpret_cast<const uint2*>(sl.d16 + s0);
+    const uint2 d = NT ? nnpp_ld_stream(p) : *p;
     c[0] = j + (int32_t)(d.x & 0xffffu);
     c[1] = c[0] + (int32_t)(d.x >> 16);
     c[2] = c[1] + (int32_t)(d.y & 0xffffu);
@@ -61,9 +66,16 @@ __device__ __forceinline__ void quad_cols(const Slices& sl, int64_t s0, int32_t&
   }
 }
 
-template <bool WIDE>
+// Dictionary table copied to LDS at kernel start (all threads of the block take part).
+template <typename T>
+__device__ __forceinline__ void load_tab(T* lds, const T* __restrict__ tab) {
+  for (int k = threadIdx.x; k < NNPP_DICT_MAX; k += NNPP_BLOCK) lds[k] = tab[k];
+  __syncthreads();
+}
+
+template <bool WIDE, bool CLS>
 __device__ __forceinline__ double spmv64_row(const Slices& sl, const double* __restrict__ w,
-                                             const double* __restrict__ dd,
+                                             const double* tab, const double* __restrict__ dd,
                                              const double* __restrict__ v, int64_t i) {
   const int64_t base = sl.off[i / NNPP_WAVE] + 4 * (i % NNPP_WAVE);
   const int G = sl.len[i / NNPP_WAVE];
@@ -73,14 +85,28 @@ __device__ __forceinline__ double spmv64_row(const Slices& sl, const double* __r
   for (int g = 0; g < G; g++) {
     const int64_t s0 = base + 256 * (int64_t)g;
     int32_t c[4];
-    quad_cols<WIDE>(sl, s0, j, c);
-    const double2 wa = *reinterpret_cast<const double2*>(w + s0);
-    const double2 wb = *reinterpret_cast<const double2*>(w + s0 + 2);
+    quad_cols<WIDE, false>(sl, s0, j, c);
+    double wq[4];
+    if (CLS) {
+      const uint32_t k = *reinterpret_cast<const uint32_t*>(sl.cls + s0);
+      wq[0] = tab[k & 0xffu];
+      wq[1] = tab[(k >> 8) & 0xffu];
+      wq[2] = tab[(k >> 16) & 0xffu];
+      wq[3] = tab[k >> 24];
+    } else {
+      // plain loads: for the FP64 weights they measure faster than non-temporal ones
+      const double2 wa = *reinterpret_cast<const double2*>(w + s0);
+      const double2 wb = *reinterpret_cast<const double2*>(w + s0 + 2);
+      wq[0] = wa.x;
+      wq[1] = wa.y;
+      wq[2] = wb.x;
+      wq[3] = wb.y;
+    }
     const double p0 = v[c[0]], p1 = v[c[1]], p2 = v[c[2]], p3 = v[c[3]];
-    acc += wa.x * (vi - p0);
-    acc += wa.y * (vi - p1);
-    acc += wb.x * (vi - p2);
-    acc += wb.y * (vi - p3);
+    acc += wq[0] * (vi - p0);
+    acc += wq[1] * (vi - p1);
+    acc += wq[2] * (vi - p2);
+    acc += wq[3] * (vi - p3);
   }
   return dd[i] * vi + acc;
 }
@@ -92,9 +118,9 @@ __device__ __forceinline__ void kahan(float& s, float& cmp, float v) {
   s = t;
 }
 
-template <bool WIDE>
+template <bool WIDE, bool CLS>
 __device__ __forceinline__ float spmv32_row(const Slices& sl, const float* __restrict__ w,
-                                            const float* __restrict__ dd,
+                                            const float* tab, const float* __restrict__ dd,
                                             const float* __restrict__ v, int64_t i) {
   const int64_t base = sl.off[i / NNPP_WAVE] + 4 * (i % NNPP_WAVE);
   const int G = sl.len[i / NNPP_WAVE];
@@ -104,13 +130,29 @@ __device__ __forceinline__ float spmv32_row(const Slices& sl, const float* __res
   for (int g = 0; g < G; g++) {
     const int64_t s0 = base + 256 * (int64_t)g;
     int32_t c[4];
-    quad_cols<WIDE>(sl, s0, j, c);
-    const float4 wq = *reinterpret_cast<const float4*>(w + s0);
+    quad_cols<WIDE, true>(sl, s0, j, c);
+    float wq[4];
+    if (CLS) {
+      const uint32_t k =
+          __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(sl.cls + s0));
+      wq[0] = tab[k & 0xffu];
+      wq[1] = tab[(k >> 8) & 0xffu];
+      wq[2] = tab[(k >> 16) & 0xffu];
+      wq[3] = tab[k >> 24];
+    } else {
+      // the slice streams are read once per application: non-temporal, so the gathered
+      // vector keeps its place in L2
+      const float4 t = nnpp_ld_stream(reinterpret_cast<const float4*>(w + s0));
+      wq[0] = t.x;
+      wq[1] = t.y;
+      wq[2] = t.z;
+      wq[3] = t.w;
+    }
     const float p0 = v[c[0]], p1 = v[c[1]], p2 = v[c[2]], p3 = v[c[3]];
-    kahan(acc, cmp, wq.x * (vi - p0));
-    kahan(acc, cmp, wq.y * (vi - p1));
-    kahan(acc, cmp, wq.z * (vi - p2));
-    kahan(acc, cmp, wq.w * (vi - p3));
+    kahan(acc, cmp, wq[0] * (vi - p0));
+    kahan(acc, cmp, wq[1] * (vi - p1));
+    kahan(acc, cmp, wq[2] * (vi - p2));
+    kahan(acc, cmp, wq[3] * (vi - p3));
   }
   return dd[i] * vi + acc;
 }
@@ -145,7 +187,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
 
 // r = b - M x in FP64 (RR = ||r||^2). START: also z = D^-1 r, p = z, rho, the target from
 // ||b|| and the converged flag.
-template <bool START, bool WIDE>
+template <bool START, bool WIDE, bool CLS>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
                                                       const double* __restrict__ w,
                                                       const double* __restrict__ dd,
@@ -155,12 +197,15 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
                                                       double* __restrict__ r, double* __restrict__ zv,
                                                       double* __restrict__ pv,
                                                       double* __restrict__ part) {
+  __shared__ double tab[CLS ? NNPP_DICT_MAX : 1];
+  if (CLS) load_tab(tab, sl.tab64);
   double v[2] = {0.0, 0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-    const double ri = (pinv[i] != 0.0) ? b[i] - spmv64_row<WIDE>(sl, w, dd, x, i) : 0.0;
+    const double ri =
+        (pinv[i] != 0.0) ? b[i] - spmv64_row<WIDE, CLS>(sl, w, tab, dd, x, i) : 0.0;
     r[i] = ri;
     v[1] += ri * ri;
     if (START) {
@@ -173,19 +218,21 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
   nnpp_block_partial<2>(v, part);
 }
 
-template <bool WIDE>
+template <bool WIDE, bool CLS>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
                                                         const double* __restrict__ w,
                                                         const double* __restrict__ dd,
                                                         const double* __restrict__ p,
                                                         double* __restrict__ q, double* __restrict__ part, const double* __restrict__ scal) {
   if (cg_done(scal)) return;
+  __shared__ double tab[CLS ? NNPP_DICT_MAX : 1];
+  if (CLS) load_tab(tab, sl.tab64);
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-    const double qi = spmv64_row<WIDE>(sl, w, dd, p, i);
+    const double qi = spmv64_row<WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
     v[0] += p[i] * qi;
   }
@@ -251,19 +298,21 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_replace(int32_t n, const doubl
   nnpp_block_partial<2>(v, part);
 }
 
-template <bool WIDE>
+template <bool WIDE, bool CLS>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
                                                        const float* __restrict__ w,
                                                        const float* __restrict__ dd,
                                                        const float* __restrict__ p,
                                                        float* __restrict__ q, double* __restrict__ part, const double* __restrict__ scal) {
   if (cg_done(scal)) return;
+  __shared__ float tab[CLS ? NNPP_DICT_MAX : 1];
+  if (CLS) load_tab(tab, sl.tab32);
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-    const float qi = spmv32_row<WIDE>(sl, w, dd, p, i);
+    const float qi = spmv32_row<WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
     v[0] += (double)p[i] * (double)qi;
   }
@@ -329,22 +378,35 @@ __global__ void k_scatter_phi(int32_t n, int S, const uint8_t* __restrict__ pfre
   else if ((bcmask[i] >> S) & 1u) phi[i] = bcphi[i];
 }
 
+// variant = 2 * wide + dictionary
 template <bool START, typename... A>
-void launch_resid(bool wide, int64_t g, hipStream_t s, A... a) {
-  if (wide) k_resid<START, true><<<g, NNPP_BLOCK, 0, s>>>(a...);
-  else k_resid<START, false><<<g, NNPP_BLOCK, 0, s>>>(a...);
+void launch_resid(int var, int64_t g, hipStream_t s, A... a) {
+  switch (var) {
+    case 0: k_resid<START, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 1: k_resid<START, false, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 2: k_resid<START, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    default: k_resid<START, true, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+  }
 }
 
 template <typename... A>
-void launch_cg_spmv(bool wide, int64_t g, hipStream_t s, A... a) {
-  if (wide) k_cg_spmv<true><<<g, NNPP_BLOCK, 0, s>>>(a...);
-  else k_cg_spmv<false><<<g, NNPP_BLOCK, 0, s>>>(a...);
+void launch_cg_spmv(int var, int64_t g, hipStream_t s, A... a) {
+  switch (var) {
+    case 0: k_cg_spmv<false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 1: k_cg_spmv<false, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 2: k_cg_spmv<true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    default: k_cg_spmv<true, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+  }
 }
 
 template <typename... A>
-void launch_m_spmv(bool wide, int64_t g, hipStream_t s, A... a) {
-  if (wide) k_m_spmv<true><<<g, NNPP_BLOCK, 0, s>>>(a...);
-  else k_m_spmv<false><<<g, NNPP_BLOCK, 0, s>>>(a...);
+void launch_m_spmv(int var, int64_t g, hipStream_t s, A... a) {
+  switch (var) {
+    case 0: k_m_spmv<false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 1: k_m_spmv<false, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 2: k_m_spmv<true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    default: k_m_spmv<true, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+  }
 }
 
 }  // namespace
@@ -370,8 +432,9 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   if (!n) return NNPP_OK;
   const int64_t nb = nnpp_nblocks(n);
   const int64_t g = nnpp_red_grid(nb);  // grid of every kernel with a reduction
-  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first};
-  const bool wide = c->pot.wide != 0;
+  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first,
+            c->pot.cls, c->pot.tab64, c->pot.tab32};
+  const int var = 2 * (c->pot.wide != 0) + (c->pot.ncls > 0);  // kernel variant
   double* part = c->red_part;
   const double* scal = c->scal;
   double zs[NNPP_MAX_SPECIES] = {0};
@@ -398,10 +461,10 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   // r = b - M x in FP64 (START: also z, p, rho, target, converged flag)
   auto resid = [&](bool start) -> int {
     if (start)
-      launch_resid<true>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+      launch_resid<true>(var, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
                          c->cg_r, c->cg_z, c->cg_p, part);
     else
-      launch_resid<false>(wide, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+      launch_resid<false>(var, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
                           c->cg_r, nullptr, nullptr, part);
     NNPP_HIP(c, hipGetLastError());
     n64++;
@@ -422,7 +485,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       }
       const int batch = left < CG_BATCH ? left : CG_BATCH;
       for (int it = 0; it < batch; it++) {
-        launch_cg_spmv(wide, g, s, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, part, scal);
+        launch_cg_spmv(var, g, s, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, part, scal);
         if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
         k_cg_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
                                              c->cg_z, part, scal);
@@ -457,7 +520,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         }
         const int batch = left < CG_BATCH ? left : CG_BATCH;
         for (int it = 0; it < batch; it++) {
-          launch_m_spmv(wide, g, s, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, part, scal);
+          launch_m_spmv(var, g, s, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, part, scal);
           if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
           k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
                                               c->f_z, part, scal);
@@ -507,13 +570,14 @@ int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const int32_t n = c->n;
   if (!n) return NNPP_OK;
   const int64_t g = nnpp_red_grid(nnpp_nblocks(n));
-  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first};
-  const bool wide = c->pot.wide != 0;
+  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first,
+            c->pot.cls, c->pot.tab64, c->pot.tab32};
+  const int var = 2 * (c->pot.wide != 0) + (c->pot.ncls > 0);  // kernel variant
   if (precision == NNPP_MIXED)
-    launch_m_spmv(wide, g, c->stream, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
+    launch_m_spmv(var, g, c->stream, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
                   (const double*)c->scal);
   else
-    launch_cg_spmv(wide, g, c->stream, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
+    launch_cg_spmv(var, g, c->stream, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
                    (const double*)c->scal);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;

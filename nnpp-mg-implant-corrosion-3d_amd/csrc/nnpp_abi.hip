@@ -32,6 +32,9 @@ void free_slices(nnpp_slices& s) {
   dfree(s.w32);
   dfree(s.first);
   dfree(s.d16);
+  dfree(s.cls);
+  dfree(s.tab64);
+  dfree(s.tab32);
   s = nnpp_slices();
 }
 
@@ -454,6 +457,13 @@ int nnpp_set_solver(nnpp_ctx* c, double rtol, int32_t maxit, double inner_rtol) 
   return NNPP_OK;
 }
 
+int nnpp_set_format(nnpp_ctx* c, int32_t flags) {
+  if (!c || (flags & ~(NNPP_FORMAT_WIDE_COLUMNS | NNPP_FORMAT_NO_DICTIONARY))) return NNPP_ERR_ARG;
+  if (flags != c->format) c->geom_dirty = c->pot_dirty = true;
+  c->format = flags;
+  return NNPP_OK;
+}
+
 int nnpp_poisson_solve(nnpp_ctx* c, int32_t precision, int32_t* iters, double* relres) {
   if (!c) return NNPP_ERR_ARG;
   NNPP_HIP(c, hipSetDevice(c->device));
@@ -522,6 +532,8 @@ int nnpp_step(nnpp_ctx* c, int32_t nsteps, int32_t precision, int32_t check_fini
     loc.flipped += flips;
     loc.nnz_flux = c->bonds.nnz;
     loc.nnz_ll = c->pot.nnz;
+    loc.pot_col_bytes = c->pot.wide ? 4 : 2;
+    loc.pot_weight_bytes = c->pot.ncls > 0 ? 1 : 4;
     if (c->solid_count <= 0) break;  // main.cpp:924-932
   }
   loc.solid_count = c->solid_count;

@@ -44,7 +44,15 @@ struct nnpp_slices {
   int32_t* first = nullptr;  // [n] column before entry 0 (the row's first column, or i)
   uint16_t* d16 = nullptr;   // [slots]
   int wide = 0;
+  // Lossless weight dictionary: when the operator has at most NNPP_DICT_MAX - 1 distinct FP64
+  // weights (a regular lattice has a handful), entry weights are stored as 1-byte classes into
+  // tab64/tab32 (class 0 = padding, weight 0). ncls = 0: no dictionary, w64/w32 are read.
+  uint8_t* cls = nullptr;    // [slots]
+  double* tab64 = nullptr;   // [NNPP_DICT_MAX]
+  float* tab32 = nullptr;    // [NNPP_DICT_MAX]
+  int ncls = 0;
 };
+#define NNPP_DICT_MAX 256
 
 struct nnpp_ctx {
   int device = 0;
@@ -70,6 +78,7 @@ struct nnpp_ctx {
   nnpp_slices pot;         // free-free liquid bonds of the potential operator
   bool geom_dirty = true;  // bond slices need a rebuild
   bool pot_dirty = true;   // potential operator needs a rebuild (phase/bc changed)
+  int32_t format = 0;      // NNPP_FORMAT_* flags
   double w_exponent = 3.0; // 3 + 2 s shared by every field
 
   uint8_t* phase = nullptr;
@@ -142,6 +151,29 @@ __device__ __forceinline__ int64_t nnpp_xcd_block(int64_t b, int64_t nb) {
   const int64_t q = nb / NNPP_NXCD, r = nb % NNPP_NXCD;
   const int64_t xcd = b % NNPP_NXCD, k = b / NNPP_NXCD;
   return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// Non-temporal loads of streams read once per kernel (slice columns and weights), so they do
+// not push gathered vectors out of L2.
+typedef unsigned int nnpp_u32x2 __attribute__((ext_vector_type(2)));
+typedef double nnpp_f64x2 __attribute__((ext_vector_type(2)));
+typedef float nnpp_f32x4 __attribute__((ext_vector_type(4)));
+typedef int nnpp_i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 nnpp_ld_stream(const uint2* p) {
+  const nnpp_u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const nnpp_u32x2*>(p));
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ double2 nnpp_ld_stream(const double2* p) {
+  const nnpp_f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const nnpp_f64x2*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float4 nnpp_ld_stream(const float4* p) {
+  const nnpp_f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const nnpp_f32x4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 nnpp_ld_stream(const int4* p) {
+  const nnpp_i32x4 v = __builtin_nontemporal_load(reinterpret_cast<const nnpp_i32x4*>(p));
+  return make_int4(v.x, v.y, v.z, v.w);
 }
 
 // utils.h:17-25
@@ -284,3 +316,4 @@ int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo);
 int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision);
 int nnpp_finalize(nnpp_ctx* c, int op, int64_t nblk, double a0);
+int nnpp_build_dictionary(nnpp_ctx* c, nnpp_slices& sl);

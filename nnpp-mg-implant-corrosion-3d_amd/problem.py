@@ -123,6 +123,10 @@ class Problem:
     def set_solver(self, rtol=1e-10, maxit=20000, inner_rtol=1e-4):
         self._check(self.lib.nnpp_set_solver(self.ctx, rtol, maxit, inner_rtol))
 
+    def set_format(self, flags=0):
+        """Operator storage format (FORMAT_* flags); all formats give bit-identical results."""
+        self._check(self.lib.nnpp_set_format(self.ctx, int(flags)))
+
     # -- operators -----------------------------------------------------------------------
     def residuals(self, voln1rel=0.0, apply_bc=False, precision=FP64):
         """utils::Residuals on (slot N1 = U^{n+1}, slot N = U^n); (F, n) array."""

@@ -235,6 +235,42 @@ def test_mixed_step_close_to_fp64():
         assert rel_err(out[nn.MIXED][f], out[nn.FP64][f]) < 1e-6, f
 
 
+@pytest.mark.parametrize("kind", ["regular", "screw"])
+def test_storage_formats_bit_identical(kind):
+    """16-bit column steps vs 32-bit columns, dictionary weight classes vs stored weights:
+    the same sums in the same order, so the trajectories agree bit for bit."""
+    say("case storage formats", kind)
+    # 16^3 regular: spacing 1/16 makes every bond vector exact, so the dictionary applies
+    case = nn.synth.regular(16, S=11, seed=9) if kind == "regular" else nn.synth.screw(16, S=11, seed=9)
+    rp, col, _ = oracle_nb(case)
+    for prec in (nn.FP64, nn.MIXED):
+        out = {}
+        for fmt in (nn.FORMAT_AUTO, nn.FORMAT_WIDE_COLUMNS, nn.FORMAT_NO_DICTIONARY,
+                    nn.FORMAT_WIDE_COLUMNS | nn.FORMAT_NO_DICTIONARY):
+            with problem(case) as pb:
+                pb.set_format(fmt)
+                pb.set_neighbors(rp, col)
+                pb.set_bc(case.bctypes, case.bcvalues)
+                pb.init_state(case.icvalues)
+                pb.set_solver(rtol=1e-12, maxit=100000)
+                st = pb.step(4, prec)
+                assert st["status"] == 0
+                out[fmt] = (pb.get_state(), st)
+        U0, st0 = out[nn.FORMAT_AUTO]
+        assert st0["pot_col_bytes"] == 2
+        if kind == "regular":
+            assert st0["pot_weight_bytes"] == 1
+        for fmt, (U, st) in out.items():
+            assert st["pot_col_bytes"] == (4 if fmt & nn.FORMAT_WIDE_COLUMNS else 2)
+            if fmt & nn.FORMAT_NO_DICTIONARY:
+                assert st["pot_weight_bytes"] == 4
+            assert st["cg_iterations"] == st0["cg_iterations"], fmt
+            assert np.array_equal(U, U0), (prec, fmt)
+    with problem(case) as pb:
+        with pytest.raises(nn.NnppError):
+            pb.set_format(4)
+
+
 def test_deterministic_and_error_paths():
     say("case determinism / errors")
     case = nn.synth.regular(12, S=5, seed=6)
