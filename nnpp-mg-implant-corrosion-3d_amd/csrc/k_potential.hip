@@ -44,71 +44,81 @@ __device__ __forceinline__ bool cg_done(const double* scal) {
   return nnpp_ld_scalar(scal + SC_DONE) != 0.0;
 }
 
-// Columns of the next 4 entries of a row: one 16-byte load (wide) or one 8-byte load of
-// 16-bit steps accumulated onto the running column j.
-template <bool WIDE, bool NT>
-__device__ __forceinline__ void quad_cols(const Slices& sl, int64_t s0, int32_t& j, int32_t (&c)[4]) {
-  if (WIDE) {
-    const int4* p = reinterpret_cast<const int4*>(sl.col + s0);
-    const int4 q = NT ? nnpp_ld_stream(p) : *p;
-    c[0] = q.x;
-    c[1] = q.y;
-    c[2] = q.z;
-    c[3] = q.w;
-  } else {
-    const uint2* p = reinterpret_cast<const uint2*>(sl.d16 + s0);
-    const uint2 d = NT ? nnpp_ld_stream(p) : *p;
-    c[0] = j + (int32_t)(d.x & 0xffffu);
-    c[1] = c[0] + (int32_t)(d.x >> 16);
-    c[2] = c[1] + (int32_t)(d.y & 0xffffu);
-    c[3] = c[2] + (int32_t)(d.y >> 16);
-    j = c[3];
+// Stream words of one quad (4 consecutive entries of a row): the columns (four int32, or four
+// 16-bit steps from the previous column) and the weights (FP64/FP32 values, or four 1-byte
+// dictionary classes). The FP32 variants load them non-temporally so the gathered vector keeps
+// its place in L2; for the FP64 variants plain loads measure faster.
+template <typename T, bool WIDE, bool CLS>
+struct Quad {
+  static constexpr bool NT = sizeof(T) == 4;
+  int4 col;
+  uint2 d;
+  uint32_t k;
+  T w[4];
+
+  __device__ __forceinline__ void load(const Slices& sl, const T* __restrict__ wp, int64_t s0) {
+    if constexpr (WIDE) {
+      const int4* p = reinterpret_cast<const int4*>(sl.col + s0);
+      col = NT ? nnpp_ld_stream(p) : *p;
+    } else {
+      const uint2* p = reinterpret_cast<const uint2*>(sl.d16 + s0);
+      d = NT ? nnpp_ld_stream(p) : *p;
+    }
+    if constexpr (CLS) {
+      const uint32_t* p = reinterpret_cast<const uint32_t*>(sl.cls + s0);
+      k = NT ? __builtin_nontemporal_load(p) : *p;
+    } else if constexpr (NT) {
+      const float4 t = nnpp_ld_stream(reinterpret_cast<const float4*>(wp + s0));
+      w[0] = t.x;
+      w[1] = t.y;
+      w[2] = t.z;
+      w[3] = t.w;
+    } else {
+      const double2 a = *reinterpret_cast<const double2*>(wp + s0);
+      const double2 b = *reinterpret_cast<const double2*>(wp + s0 + 2);
+      w[0] = a.x;
+      w[1] = a.y;
+      w[2] = b.x;
+      w[3] = b.y;
+    }
   }
-}
+
+  // columns, advancing the running column j in the step format
+  __device__ __forceinline__ void cols(int32_t& j, int32_t (&c)[4]) const {
+    if constexpr (WIDE) {
+      c[0] = col.x;
+      c[1] = col.y;
+      c[2] = col.z;
+      c[3] = col.w;
+    } else {
+      c[0] = j + (int32_t)(d.x & 0xffffu);
+      c[1] = c[0] + (int32_t)(d.x >> 16);
+      c[2] = c[1] + (int32_t)(d.y & 0xffffu);
+      c[3] = c[2] + (int32_t)(d.y >> 16);
+      j = c[3];
+    }
+  }
+
+  __device__ __forceinline__ void weights(const T* tab, T (&wq)[4]) const {
+    if constexpr (CLS) {
+      wq[0] = tab[k & 0xffu];
+      wq[1] = tab[(k >> 8) & 0xffu];
+      wq[2] = tab[(k >> 16) & 0xffu];
+      wq[3] = tab[k >> 24];
+    } else {
+      wq[0] = w[0];
+      wq[1] = w[1];
+      wq[2] = w[2];
+      wq[3] = w[3];
+    }
+  }
+};
 
 // Dictionary table copied to LDS at kernel start (all threads of the block take part).
 template <typename T>
 __device__ __forceinline__ void load_tab(T* lds, const T* __restrict__ tab) {
   for (int k = threadIdx.x; k < NNPP_DICT_MAX; k += NNPP_BLOCK) lds[k] = tab[k];
   __syncthreads();
-}
-
-template <bool WIDE, bool CLS>
-__device__ __forceinline__ double spmv64_row(const Slices& sl, const double* __restrict__ w,
-                                             const double* tab, const double* __restrict__ dd,
-                                             const double* __restrict__ v, int64_t i) {
-  const int64_t base = sl.off[i / NNPP_WAVE] + 4 * (i % NNPP_WAVE);
-  const int G = sl.len[i / NNPP_WAVE];
-  const double vi = v[i];
-  int32_t j = WIDE ? 0 : sl.first[i];
-  double acc = 0.0;
-  for (int g = 0; g < G; g++) {
-    const int64_t s0 = base + 256 * (int64_t)g;
-    int32_t c[4];
-    quad_cols<WIDE, false>(sl, s0, j, c);
-    double wq[4];
-    if (CLS) {
-      const uint32_t k = *reinterpret_cast<const uint32_t*>(sl.cls + s0);
-      wq[0] = tab[k & 0xffu];
-      wq[1] = tab[(k >> 8) & 0xffu];
-      wq[2] = tab[(k >> 16) & 0xffu];
-      wq[3] = tab[k >> 24];
-    } else {
-      // plain loads: for the FP64 weights they measure faster than non-temporal ones
-      const double2 wa = *reinterpret_cast<const double2*>(w + s0);
-      const double2 wb = *reinterpret_cast<const double2*>(w + s0 + 2);
-      wq[0] = wa.x;
-      wq[1] = wa.y;
-      wq[2] = wb.x;
-      wq[3] = wb.y;
-    }
-    const double p0 = v[c[0]], p1 = v[c[1]], p2 = v[c[2]], p3 = v[c[3]];
-    acc += wq[0] * (vi - p0);
-    acc += wq[1] * (vi - p1);
-    acc += wq[2] * (vi - p2);
-    acc += wq[3] * (vi - p3);
-  }
-  return dd[i] * vi + acc;
 }
 
 __device__ __forceinline__ void kahan(float& s, float& cmp, float v) {
@@ -118,43 +128,69 @@ __device__ __forceinline__ void kahan(float& s, float& cmp, float v) {
   s = t;
 }
 
-template <bool WIDE, bool CLS>
-__device__ __forceinline__ float spmv32_row(const Slices& sl, const float* __restrict__ w,
-                                            const float* tab, const float* __restrict__ dd,
-                                            const float* __restrict__ v, int64_t i) {
+// Per-row accumulator: plain FP64 sum, or Kahan-compensated FP32 sum.
+template <typename T>
+struct RowAcc;
+template <>
+struct RowAcc<double> {
+  double s = 0.0;
+  __device__ __forceinline__ void add(double x) { s += x; }
+};
+template <>
+struct RowAcc<float> {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x) { kahan(s, c, x); }
+};
+
+// Quads per batch: with stored weights (6-12 bytes per entry) deep batches keep enough bytes in
+// flight; with dictionary classes (3 bytes per entry) small batches keep occupancy and measure
+// faster (MI355X, 128^3 lattice and jittered screw cases).
+template <typename T, bool CLS>
+constexpr int spmv_batch() {
+  return !CLS ? 8 : (sizeof(T) == 4 ? 2 : 1);
+}
+
+// y_i = dd_i v_i + sum_t w_t (v_i - v_{c_t}) over the row's slice entries (padding weighs 0).
+// Quads are taken in batches: all stream words of a batch are loaded first, then all its
+// gathers are issued, then summed, so each wave keeps a batch's worth of loads in flight per
+// memory round trip. G is uniform over the wave, so the batch tail is a uniform branch; the
+// tail's loads are clamped to the row's last quad and not summed.
+template <typename T, bool WIDE, bool CLS>
+__device__ __forceinline__ T spmv_row(const Slices& sl, const T* __restrict__ w, const T* tab,
+                                      const T* __restrict__ dd, const T* __restrict__ v,
+                                      int64_t i) {
+  constexpr int B = spmv_batch<T, CLS>();
   const int64_t base = sl.off[i / NNPP_WAVE] + 4 * (i % NNPP_WAVE);
   const int G = sl.len[i / NNPP_WAVE];
-  const float vi = v[i];
+  const T vi = v[i];
   int32_t j = WIDE ? 0 : sl.first[i];
-  float acc = 0.0f, cmp = 0.0f;
-  for (int g = 0; g < G; g++) {
-    const int64_t s0 = base + 256 * (int64_t)g;
-    int32_t c[4];
-    quad_cols<WIDE, true>(sl, s0, j, c);
-    float wq[4];
-    if (CLS) {
-      const uint32_t k =
-          __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(sl.cls + s0));
-      wq[0] = tab[k & 0xffu];
-      wq[1] = tab[(k >> 8) & 0xffu];
-      wq[2] = tab[(k >> 16) & 0xffu];
-      wq[3] = tab[k >> 24];
-    } else {
-      // the slice streams are read once per application: non-temporal, so the gathered
-      // vector keeps its place in L2
-      const float4 t = nnpp_ld_stream(reinterpret_cast<const float4*>(w + s0));
-      wq[0] = t.x;
-      wq[1] = t.y;
-      wq[2] = t.z;
-      wq[3] = t.w;
+  RowAcc<T> acc;
+  for (int g = 0; g < G; g += B) {
+    Quad<T, WIDE, CLS> q[B];
+#pragma unroll
+    for (int b = 0; b < B; b++)
+      q[b].load(sl, w, base + 256 * (int64_t)(g + b < G ? g + b : G - 1));
+    int32_t c[B][4];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      if (g + b < G) q[b].cols(j, c[b]);
+      else c[b][0] = c[b][1] = c[b][2] = c[b][3] = j;
     }
-    const float p0 = v[c[0]], p1 = v[c[1]], p2 = v[c[2]], p3 = v[c[3]];
-    kahan(acc, cmp, wq[0] * (vi - p0));
-    kahan(acc, cmp, wq[1] * (vi - p1));
-    kahan(acc, cmp, wq[2] * (vi - p2));
-    kahan(acc, cmp, wq[3] * (vi - p3));
+    T pv[B][4];
+#pragma unroll
+    for (int b = 0; b < B; b++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) pv[b][e] = v[c[b][e]];
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+      if (g + b >= G) break;
+      T wq[4];
+      q[b].weights(tab, wq);
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc.add(wq[e] * (vi - pv[b][e]));
+    }
   }
-  return dd[i] * vi + acc;
+  return dd[i] * vi + acc.s;
 }
 
 // b (free rows), warm start x = phi, ||b||^2.
@@ -205,7 +241,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
     const double ri =
-        (pinv[i] != 0.0) ? b[i] - spmv64_row<WIDE, CLS>(sl, w, tab, dd, x, i) : 0.0;
+        (pinv[i] != 0.0) ? b[i] - spmv_row<double, WIDE, CLS>(sl, w, tab, dd, x, i) : 0.0;
     r[i] = ri;
     v[1] += ri * ri;
     if (START) {
@@ -232,7 +268,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-    const double qi = spmv64_row<WIDE, CLS>(sl, w, tab, dd, p, i);
+    const double qi = spmv_row<double, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
     v[0] += p[i] * qi;
   }
@@ -312,7 +348,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-    const float qi = spmv32_row<WIDE, CLS>(sl, w, tab, dd, p, i);
+    const float qi = spmv_row<float, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
     v[0] += (double)p[i] * (double)qi;
   }

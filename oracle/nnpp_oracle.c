@@ -425,11 +425,40 @@ static double dot_(int32_t n, const double* a, const double* b) {
  * ||r||_2 <= rtol ||b||_2. Dirichlet nodes get phi = bc, other non-free nodes keep phi.
  * *relres is recomputed from the final iterate.
  */
+static int poisson_solve_core(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                              const double* z, const double* vol, const int64_t* row_ptr,
+                              const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                              const double* bcvalues, const double* U, const double* rhs,
+                              double* phi, double rtol, int32_t maxit, int32_t* iters,
+                              double* relres);
+
 int or_poisson_solve(int32_t n, const nnpp_params* p, const double* x, const double* y,
                      const double* z, const double* vol, const int64_t* row_ptr,
                      const int32_t* col, const int32_t* phase, const int32_t* bctypes,
                      const double* bcvalues, const double* U, double* phi, double rtol,
                      int32_t maxit, int32_t* iters, double* relres) {
+  return poisson_solve_core(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, U, NULL,
+                            phi, rtol, maxit, iters, relres);
+}
+
+/* The same scaled operator M with a given right-hand side on the free rows (the adjoint
+ * solve: M is symmetric, so M^T psi = rhs is M psi = rhs). psi holds the warm start on entry
+ * and the solution on the free nodes on exit; other entries are left as they are. */
+int or_poisson_solve_rhs(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                         const double* z, const double* vol, const int64_t* row_ptr,
+                         const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                         const double* bcvalues, const double* rhs, double* psi, double rtol,
+                         int32_t maxit, int32_t* iters, double* relres) {
+  return poisson_solve_core(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, NULL, rhs,
+                            psi, rtol, maxit, iters, relres);
+}
+
+static int poisson_solve_core(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                              const double* z, const double* vol, const int64_t* row_ptr,
+                              const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                              const double* bcvalues, const double* U, const double* rhs,
+                              double* phi, double rtol, int32_t maxit, int32_t* iters,
+                              double* relres) {
   const int S = p->n_species;
   const double V0 = p->dx * p->dy * p->dz;
   const double cb = p->F * p->CSolid / (p->K_phi * p->sigma1);
@@ -445,7 +474,7 @@ int or_poisson_solve(int32_t n, const nnpp_params* p, const double* x, const dou
   for (int32_t i = 0; i < n; i++)
     if (bctypes && bphi && bctypes[i] == 0 && bphi[i] != -1) {
       dir[i] = 1;
-      phi[i] = bphi[i];
+      if (!rhs) phi[i] = bphi[i];
     }
 #pragma omp parallel for schedule(static)
   for (int32_t i = 0; i < n; i++) {
@@ -461,12 +490,16 @@ int or_poisson_solve(int32_t n, const nnpp_params* p, const double* x, const dou
       flag = 1;
     }
     if (!flag) continue;
-    double qc = 0.0;
-    for (int k = 0; k < S; k++) qc += p->z[k] * U[(size_t)k * n + i];
     const double sc = vol ? vol[i] / V0 : 1.0;
     fre[i] = 1;
     diag[i] = sc * d;
-    b[i] = sc * (cb * qc + bd);
+    if (rhs) {
+      b[i] = rhs[i];
+    } else {
+      double qc = 0.0;
+      for (int k = 0; k < S; k++) qc += p->z[k] * U[(size_t)k * n + i];
+      b[i] = sc * (cb * qc + bd);
+    }
   }
   /* r = b - M x */
 #define APPLY_M(in, out)                                                         \
@@ -626,4 +659,207 @@ int or_step(int32_t n, const nnpp_params* p, const double* x, const double* y, c
   free(phi);
   free(G);
   return cst;
+}
+
+/*
+ * Reverse-mode derivative of nsteps explicit steps (or_step) with respect to the per-species
+ * diffusivities D_liquid[k], D_solid[k], the generic dissolution rate r_diss, and the starting
+ * state. The reference has no adjoint; this differentiates the oracle's own or_step as the
+ * plain reverse of its loops (a scatter over the bonds of every row), so it shares no code
+ * path with the device adjoint (a gather over the transposed bonds). It is pinned by central
+ * finite differences of or_step.
+ *   - the phase trajectory is held fixed (front flips are piecewise constant);
+ *   - Dirichlet and clamped entries pass no derivative;
+ *   - the potential solve is differentiated at convergence (M symmetric: M psi = dL/dphi);
+ *   - BV = 10^(-vol_red (1 - voln1rel)) is differentiated through the voln1rel updates
+ *     (every sub_steps steps, voln1rel = sum over solid nodes of (U_0 - CSat/CSolid) / vol0rel).
+ * Steps stop early, as in or_step's callers, once no solid node is left.
+ * grad[0..S-1] = dL/dD_liquid, grad[S..2S-1] = dL/dD_solid, grad[2S] = dL/dr_diss;
+ * lam0 (optional) = dL/dU^0. Generic chemistry only (NNPP_ERR_UNSUPPORTED otherwise).
+ */
+int or_adjoint(int32_t n, const nnpp_params* p, const double* x, const double* y, const double* z,
+               const double* vol, const int64_t* row_ptr, const int32_t* col,
+               const int32_t* bctypes, const double* bcvalues, const double* U0,
+               const int32_t* phase0, int64_t step0, double voln1rel0, double vol0rel,
+               int32_t nsteps, double rtol, int32_t maxit, const double* lamN, double* grad,
+               double* lam0, int32_t* steps_done) {
+  const int S = p->n_species;
+  if (S < 1 || S > NNPP_MAX_SPECIES || nsteps < 0) return NNPP_ERR_ARG;
+  if (p->chemistry != NNPP_CHEM_GENERIC) return NNPP_ERR_UNSUPPORTED;
+  const size_t nF = (size_t)(S + 1) * n;
+  const double V0 = p->dx * p->dy * p->dz;
+  const double cb = p->F * p->CSolid / (p->K_phi * p->sigma1);
+  double* Ut = (double*)malloc(sizeof(double) * nF * (size_t)(nsteps + 1));
+  int32_t* Pt = (int32_t*)malloc(sizeof(int32_t) * (size_t)n * (size_t)(nsteps + 1));
+  double* Vt = (double*)malloc(sizeof(double) * (size_t)(nsteps + 1));
+  int64_t* St = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nsteps + 1));
+  memcpy(Ut, U0, sizeof(double) * nF);
+  memcpy(Pt, phase0, sizeof(int32_t) * (size_t)n);
+  int64_t step = step0, sc = 0;
+  double vr = voln1rel0;
+  for (int32_t i = 0; i < n; i++) sc += phase0[i] == 1;
+  int32_t done = 0;
+  int status = NNPP_OK;
+  for (int32_t m = 0; m < nsteps; m++) {
+    Vt[m] = vr;
+    St[m] = step;
+    double* Un = Ut + (size_t)(m + 1) * nF;
+    int32_t* Pn = Pt + (size_t)(m + 1) * n;
+    memcpy(Un, Ut + (size_t)m * nF, sizeof(double) * nF);
+    memcpy(Pn, Pt + (size_t)m * n, sizeof(int32_t) * (size_t)n);
+    int32_t its = 0, fl = 0;
+    double rr = 0.0;
+    const int st = or_step(n, p, x, y, z, vol, row_ptr, col, bctypes, bcvalues, Un, Pn, &step,
+                           &vr, vol0rel, &sc, rtol, maxit, &its, &rr, &fl);
+    if (st != NNPP_OK && st != NNPP_ERR_NOCONV) {
+      status = st;
+      break;
+    }
+    if (st == NNPP_ERR_NOCONV) status = st;
+    done = m + 1;
+    if (sc <= 0) break;
+  }
+
+  double* lam = (double*)malloc(sizeof(double) * nF);
+  double* ln = (double*)malloc(sizeof(double) * nF);
+  double* G = (double*)malloc(sizeof(double) * nF);
+  double* g = (double*)malloc(sizeof(double) * nF);
+  double* rhs = (double*)calloc((size_t)n, sizeof(double));
+  double* psi = (double*)calloc((size_t)n, sizeof(double));
+  char* fre = (char*)calloc((size_t)n, 1);
+  char* dir = (char*)calloc((size_t)n, 1);
+  memcpy(lam, lamN, sizeof(double) * nF);
+  for (int q = 0; q <= 2 * S; q++) grad[q] = 0.0;
+  const double* bphi = bcvalues ? bcvalues + (size_t)S * n : NULL;
+  for (int32_t i = 0; i < n; i++) dir[i] = bctypes && bphi && bctypes[i] == 0 && bphi[i] != -1;
+  double mu = 0.0;  /* dL/d(voln1rel in force after the step being reversed) */
+  for (int32_t m = done - 1; m >= 0 && status != NNPP_ERR_ARG; m--) {
+    const double* Um = Ut + (size_t)m * nF;
+    const int32_t* ph = Pt + (size_t)m * n;
+    const int32_t* ph1 = Pt + (size_t)(m + 1) * n;
+    /* voln1rel update at the end of step m */
+    if (p->sub_steps > 0 && (St[m] % p->sub_steps) == 0) {
+      for (int32_t i = 0; i < n; i++)
+        if (ph1[i] == 1) lam[i] += mu / vol0rel;
+      mu = 0.0;
+    }
+    const double BV = pow(10., -p->vol_red * (1. - Vt[m]));
+    or_residuals(n, p, x, y, z, vol, row_ptr, col, ph, Um, Um, Vt[m], G);
+    memset(ln, 0, sizeof(double) * nF);
+    /* explicit update, Dirichlet rows and clamp */
+    for (int k = 0; k < S; k++)
+      for (int32_t i = 0; i < n; i++) {
+        const size_t e = (size_t)k * n + i;
+        double v = Um[e] - p->dt * G[e];
+        int pass = 1;
+        if (bctypes && bcvalues && bctypes[i] == 0 && bcvalues[e] != -1) {
+          v = bcvalues[e] / p->CSolid;
+          pass = 0;
+        }
+        if ((p->clamp[k] == NNPP_CLAMP_ALWAYS || (p->clamp[k] == NNPP_CLAMP_LIQUID && ph[i] == 0)) &&
+            v < p->clamp_floor)
+          pass = 0;
+        g[e] = pass ? -p->dt * lam[e] : 0.0;
+        ln[e] += pass ? lam[e] : 0.0;
+      }
+    /* potential: phi^{m+1} = M^-1 b(U^m) on free nodes, bc on Dirichlet nodes, phi^m elsewhere */
+    for (int32_t i = 0; i < n; i++) {
+      fre[i] = 0;
+      if (dir[i] || ph[i] != 0) continue;
+      for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++)
+        if (ph[col[b]] == 0) {
+          fre[i] = 1;
+          break;
+        }
+    }
+    for (int32_t i = 0; i < n; i++) rhs[i] = fre[i] ? lam[(size_t)S * n + i] : 0.0;
+    int32_t its = 0;
+    double rr = 0.0;
+    const int pst = or_poisson_solve_rhs(n, p, x, y, z, vol, row_ptr, col, ph, bctypes, bcvalues,
+                                         rhs, psi, rtol, maxit, &its, &rr);
+    if (pst == NNPP_ERR_NOCONV) status = pst;
+    for (int32_t i = 0; i < n; i++) {
+      if (fre[i]) {
+        const double s_ = vol ? vol[i] / V0 : 1.0;
+        for (int k = 0; k < S; k++) ln[(size_t)k * n + i] += psi[i] * s_ * cb * p->z[k];
+      } else if (!dir[i]) {
+        ln[(size_t)S * n + i] += lam[(size_t)S * n + i];
+      }
+    }
+    /* bond terms of G (or_residuals, species rows), reversed bond by bond */
+    double gBV = 0.0;
+    const double* phm = Um + (size_t)S * n;
+    for (int k = 0; k < S; k++) {
+      const double* c = Um + (size_t)k * n;
+      const double K1 = p->K1[k], K2 = p->K2[k], Ds = p->D_solid[k], Dl = p->D_liquid[k];
+      const double ex1 = 3.0 + 2.0 * p->s1[k], ex2 = 3.0 + 2.0 * p->s2[k];
+      const double mz = K2 * p->z[k] * p->F / (p->R * p->T);
+      const int sb = p->solid_bonds[k];
+      for (int32_t i = 0; i < n; i++) {
+        const double gi = g[(size_t)k * n + i];
+        int intf = 0;
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+          const int32_t j = col[b];
+          const int pi = ph[i], pj = ph[j];
+          if (pi == 1 && pj == 0) intf = 1;
+          if (gi == 0.0) continue;
+          double D, bv;
+          int liquid = 0, bvdep = 0;
+          if (pi == 1 && pj == 1) {
+            if (!sb) continue;
+            D = Ds;
+            bv = 1.0;
+          } else if (pi != pj) {
+            D = Ds;
+            bv = sb ? BV : 1.0;
+            bvdep = sb;
+          } else {
+            D = Dl;
+            bv = 1.0;
+            liquid = 1;
+          }
+          const double xx = x[j] - x[i], yy = y[j] - y[i], zz = z[j] - z[i];
+          const double xi = sqrt(xx * xx + yy * yy + zz * zz);
+          const double be = or_beta(xi, p->delta, p->dx);
+          const double Vj = vol ? vol[j] : V0;
+          const double a0 = K1 * be * Vj / pow(xi, ex1);          /* per unit D bv */
+          const double m0 = mz * be * Vj / pow(xi, ex2);
+          const double dph = phm[j] - phm[i];
+          const double T0 = -a0 * (c[j] - c[i]) - m0 * c[j] * dph;  /* term = D bv T0 */
+          const double f = gi * D * bv;
+          ln[(size_t)k * n + j] += f * (-a0 - m0 * dph);
+          ln[(size_t)k * n + i] += f * a0;
+          ln[(size_t)S * n + j] += f * (-m0 * c[j]);
+          ln[(size_t)S * n + i] += f * (m0 * c[j]);
+          if (liquid) grad[k] += gi * bv * T0;
+          else grad[S + k] += gi * bv * T0;
+          if (bvdep) gBV += gi * D * T0;
+        }
+        /* generic reaction: G_0 += r_diss U_0 at interface solid nodes */
+        if (k == 0 && intf) {
+          ln[i] += gi * p->r_diss;
+          grad[2 * S] += gi * c[i];
+        }
+      }
+    }
+    mu += gBV * BV * log(10.) * p->vol_red;
+    double* t = lam;
+    lam = ln;
+    ln = t;
+  }
+  if (lam0) memcpy(lam0, lam, sizeof(double) * nF);
+  if (steps_done) *steps_done = done;
+  free(Ut);
+  free(Pt);
+  free(Vt);
+  free(St);
+  free(lam);
+  free(ln);
+  free(G);
+  free(g);
+  free(rhs);
+  free(psi);
+  free(fre);
+  free(dir);
+  return status;
 }

@@ -45,6 +45,11 @@ int or_poisson_solve(int32_t n, const nnpp_params* p, const double* x, const dou
                      const double* bcvalues, const double* U, double* phi, double rtol,
                      int32_t maxit, int32_t* iters, double* relres);
 
+int or_poisson_solve_rhs(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                         const double* z, const double* vol, const int64_t* row_ptr,
+                         const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                         const double* bcvalues, const double* rhs, double* psi, double rtol,
+                         int32_t maxit, int32_t* iters, double* relres);
 int or_init_state(int32_t n, const nnpp_params* p, const int32_t* bctypes,
                   const double* bcvalues, const double* icvalues, double* U, int32_t* phase,
                   int64_t* solid_count, double* vol0rel);

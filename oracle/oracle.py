@@ -51,6 +51,10 @@ class Oracle:
         L.or_init_state.argtypes = [_i32, P, _ip, _dp, _dp, _dp, _ip, _lp, _dp]
         L.or_step.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _dp, _dp, _ip, _lp, _dp,
                               _d, _lp, _d, _i32, _ip, _dp, _ip]
+        L.or_poisson_solve_rhs.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _ip, _dp,
+                                           _dp, _dp, _d, _i32, _ip, _dp]
+        L.or_adjoint.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _dp, _dp, _ip,
+                                 C.c_int64, _d, _d, _i32, _d, _i32, _dp, _dp, _dp, _ip]
 
     def beta(self, xi, delta, dx):
         return self.lib.or_beta(xi, delta, dx)
@@ -113,6 +117,26 @@ class Oracle:
                                _d_(case.bcvalues), _d_(case.icvalues), _d_(U), _i_(ph),
                                C.byref(sc), C.byref(v0))
         return U, ph, int(sc.value), float(v0.value)
+
+    def adjoint(self, case, rp, col, nsteps, lamN, state=None, rtol=1e-13, maxit=100000):
+        """Reverse-mode gradient of nsteps explicit steps from the initial state (or `state`):
+        returns (grad [D_liquid(S), D_solid(S), r_diss], lam0 = dL/dU^0, steps, status) for
+        L with dL/dU^N = lamN."""
+        if state is None:
+            U, ph, sc, v0 = self.init_state(case)
+            state = {"U": U, "phase": ph, "step": 1, "voln1rel": 0.0, "vol0rel": v0, "solid": sc}
+        S = case.params.S
+        grad = np.zeros(2 * S + 1)
+        lam0 = np.zeros_like(state["U"])
+        done = C.c_int32()
+        st = self.lib.or_adjoint(case.n, C.byref(case.params), _d_(case.x), _d_(case.y),
+                                 _d_(case.z), _d_(case.vol), _l_(rp), _i_(col), _i_(case.bctypes),
+                                 _d_(case.bcvalues), _d_(np.ascontiguousarray(state["U"])),
+                                 _i_(np.ascontiguousarray(state["phase"], dtype=np.int32)),
+                                 state["step"], state["voln1rel"], state["vol0rel"], nsteps, rtol,
+                                 maxit, _d_(np.ascontiguousarray(lamN, dtype=np.float64)),
+                                 _d_(grad), _d_(lam0), C.byref(done))
+        return grad, lam0, int(done.value), st
 
     def run(self, case, rp, col, nsteps, rtol=1e-10, maxit=100000, state=None):
         """nsteps explicit steps from the initial state (or from `state`)."""

@@ -264,3 +264,62 @@ def test_init_state_rules():
     m = c.bctypes == 0
     assert np.all(U[:p.S, m] == 1.0) and np.all(U[p.S, m] == 0.0)
     assert abs(v0 - (U[0][ph == 1] - p.CSat / p.CSolid).sum()) < 1e-12 * abs(v0)
+
+
+def _fd_case(ng, S, seed, thr=None):
+    case = nn.synth.regular(ng, S=S, seed=seed)
+    case.params.sub_steps = 2  # several voln1rel (BV) updates inside the run
+    if thr is not None:
+        case.params.front_threshold = thr
+    p = case.params
+    st, rp, col, _ = O.build_neighbors(case.x, case.y, case.z, p.delta * (1 + 1e-5), p.dx * 10.,
+                                       p.dy * 10., p.dz * 10.)
+    assert st == 0
+    return case, rp, col
+
+
+@pytest.mark.parametrize("nsteps,thr", [(6, None), (9, 1.0)])
+def test_adjoint_matches_central_differences(nsteps, thr):
+    """or_adjoint against central differences of or_step, for every parameter and one random
+    direction of the starting state; thr = 1.0 flips about half the solid nodes on the way."""
+    import copy
+    case, rp, col = _fd_case(8, 3, 3, thr)
+    S = case.params.S
+    rng = np.random.default_rng(0)
+    lamN = rng.standard_normal(case.icvalues.shape)
+
+    def loss(c, state=None):
+        return float((lamN * O.run(c, rp, col, nsteps, rtol=1e-14, state=state)["U"]).sum())
+
+    grad, lam0, done, st = O.adjoint(case, rp, col, nsteps, lamN, rtol=1e-14)
+    assert st == 0 and done == nsteps
+    names = [("D_liquid", k) for k in range(S)] + [("D_solid", k) for k in range(S)] + [("r_diss", None)]
+    fd = np.zeros_like(grad)
+    for q, (name, k) in enumerate(names):
+        base = getattr(case.params, name) if k is None else getattr(case.params, name)[k]
+        h = 1e-5 * abs(base)
+        vals = []
+        for sgn in (1, -1):
+            c = copy.deepcopy(case)
+            if k is None:
+                setattr(c.params, name, base + sgn * h)
+            else:
+                getattr(c.params, name)[k] = base + sgn * h
+            vals.append(loss(c))
+        fd[q] = (vals[0] - vals[1]) / (2 * h)
+    scale = np.maximum(np.abs(fd), 1e-2 * np.abs(fd).max())
+    assert np.all(np.abs(grad - fd) <= 2e-6 * scale), (grad, fd)
+    U0, ph0, sc0, v0 = O.init_state(case)
+    d = rng.standard_normal(U0.shape)
+    h = 1e-6
+    Ls = [loss(case, {"U": U0 + s * h * d, "phase": ph0, "step": 1, "voln1rel": 0.0,
+                      "vol0rel": v0, "solid": sc0}) for s in (1, -1)]
+    fdd = (Ls[0] - Ls[1]) / (2 * h)
+    assert abs(float((lam0 * d).sum()) - fdd) <= 1e-7 * abs(fdd)
+
+
+def test_adjoint_rejects_reference_chemistry():
+    case, rp, col = _fd_case(6, 10, 1)
+    case.params.chemistry = nn.CHEM_REFERENCE
+    _, _, done, st = O.adjoint(case, rp, col, 1, np.zeros_like(case.icvalues))
+    assert st == nn._abi.ERR_UNSUPPORTED and done == 0
