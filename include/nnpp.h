@@ -234,6 +234,10 @@ int nnpp_init_state(nnpp_ctx* ctx, const double* icvalues);
 /* Set the relative volume and step counter explicitly (restart, main.cpp:506-542). */
 int nnpp_set_time(nnpp_ctx* ctx, int64_t step, double voln1rel, double vol0rel);
 
+/* Read them back (the step counter, voln1rel and vol0rel variables of main.cpp:474-479,
+ * 944-949); any pointer may be NULL. */
+int nnpp_get_time(nnpp_ctx* ctx, int64_t* step, double* voln1rel, double* vol0rel);
+
 /* Residuals F_k of every field for the iterate U^{n+1} (slot N1) and the previous state
  * (slot N), exactly as utils::Residuals (utils.h:27-842): bond sums of the iterate, lagged
  * potential, reactions, (U1-Un)/dt, phi residual on rows with a liquid-liquid bond.
@@ -300,6 +304,46 @@ int nnpp_stream(nnpp_ctx* ctx, void** stream);
 enum { NNPP_KERNEL_FLUX = 0, NNPP_KERNEL_POT_SPMV = 1 };
 int nnpp_time_kernel(nnpp_ctx* ctx, int32_t kernel, int32_t precision, int32_t reps,
                      double* avg_ms);
+
+/* ------------------------------------------------------------- differentiation ----------
+ * Adjoint of nnpp_step's explicit FP64 step (no reference counterpart: the reference has no
+ * derivative code). A tape records the steps taken between nnpp_tape_begin and nnpp_tape_end;
+ * nnpp_tape_adjoint then propagates dL/dU backwards through them:
+ *   - species update: the bond sums are linear in the state of the neighbours and in the
+ *     diffusivities; their transpose is evaluated as a gather over the same (symmetric) bond
+ *     lists; Dirichlet and clamped entries pass no derivative;
+ *   - potential: phi^{n+1} = M^{-1} b(U^n) on free nodes, so dL/db = M^{-1} dL/dphi^{n+1}
+ *     (M is symmetric) by the same CG;
+ *   - BV = 10^(-vol_red (1 - voln1rel)) is differentiated through the voln1rel updates;
+ *   - the phase trajectory is held fixed (front flips are piecewise constant).
+ * Generic chemistry only (NNPP_ERR_UNSUPPORTED for the reference chemistry). */
+typedef struct nnpp_adjoint_stats {
+  int64_t steps;             /* steps reversed */
+  int64_t recomputed_steps;  /* forward steps recomputed from checkpoints */
+  int64_t cg_iterations;     /* iterations of the adjoint potential solves */
+  int64_t tape_bytes;        /* device bytes held by checkpoints and the segment buffer */
+  double last_relres;        /* relative residual of the last adjoint potential solve */
+} nnpp_adjoint_stats;
+
+/* Start recording at the current state. Every checkpoint_every steps (1 = every step, the full
+ * tape of states) nnpp_step stores a device copy of the state, phase and step scalars; the
+ * parameters and solver settings are frozen into the tape. While a tape exists, calls that
+ * change the parameters, nodes, neighbours, boundary conditions, state, phase or time return
+ * NNPP_ERR_STATE, and nnpp_step with NNPP_MIXED returns NNPP_ERR_UNSUPPORTED. */
+int nnpp_tape_begin(nnpp_ctx* ctx, int32_t checkpoint_every);
+
+/* Reverse every recorded step. lam: dL/dU at the current state (F*n doubles, field-major).
+ * grad (2S+1 doubles): dL/dD_liquid[0..S-1], dL/dD_solid[0..S-1], dL/dr_diss. lam0 (F*n, may
+ * be NULL): dL/dU at the state where the tape began (phase held fixed). Segments between
+ * checkpoints are recomputed forward (bit-identical to the recorded run). The current state is
+ * left as it was; the tape stays valid for further calls. stats may be NULL. */
+int nnpp_tape_adjoint(nnpp_ctx* ctx, const double* lam, double* grad, double* lam0,
+                      nnpp_adjoint_stats* stats);
+
+/* Stop recording and free the tape. */
+int nnpp_tape_end(nnpp_ctx* ctx);
+
+int64_t nnpp_abi_adjoint_stats_size(void);
 
 #ifdef __cplusplus
 }

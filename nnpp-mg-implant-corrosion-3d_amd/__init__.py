@@ -14,4 +14,11 @@ lib = _abi.load()
 
 __all__ = ["CHEM_GENERIC", "CHEM_REFERENCE", "FORMAT_AUTO", "FORMAT_NO_DICTIONARY",
            "FORMAT_WIDE_COLUMNS", "FP64", "MIXED", "SLOT_N", "SLOT_N1", "NnppError",
-           "Params", "StepStats", "Problem", "synth", "formats", "lib"]
+           "Params", "StepStats", "Problem", "synth", "formats", "lib", "autograd"]
+
+
+def __getattr__(name):  # the torch binding is imported on first use
+    if name == "autograd":
+        import importlib
+        return importlib.import_module(__name__ + ".autograd")
+    raise AttributeError(name)

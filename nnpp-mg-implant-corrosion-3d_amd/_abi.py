@@ -80,6 +80,16 @@ class StepStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("recomputed_steps", C.c_int64), ("cg_iterations", C.c_int64),
+        ("tape_bytes", C.c_int64), ("last_relres", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class NnppError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"nnpp error {code}: {msg}")
@@ -119,6 +129,7 @@ SIGNATURES = {
     "nnpp_get_phase": (C.c_int, [_p, _ip]),
     "nnpp_init_state": (C.c_int, [_p, _dp]),
     "nnpp_set_time": (C.c_int, [_p, _i64, _d, _d]),
+    "nnpp_get_time": (C.c_int, [_p, _lp, _dp, _dp]),
     "nnpp_residuals": (C.c_int, [_p, _d, _i32, _i32, _dp]),
     "nnpp_poisson_apply": (C.c_int, [_p, _dp, _dp]),
     "nnpp_set_solver": (C.c_int, [_p, _d, _i32, _d]),
@@ -129,6 +140,10 @@ SIGNATURES = {
     "nnpp_state_device_ptr": (C.c_int, [_p, _i32, _i32, C.POINTER(_p)]),
     "nnpp_stream": (C.c_int, [_p, C.POINTER(_p)]),
     "nnpp_time_kernel": (C.c_int, [_p, _i32, _i32, _i32, _dp]),
+    "nnpp_tape_begin": (C.c_int, [_p, _i32]),
+    "nnpp_tape_adjoint": (C.c_int, [_p, _dp, _dp, _dp, C.POINTER(AdjointStats)]),
+    "nnpp_tape_end": (C.c_int, [_p]),
+    "nnpp_abi_adjoint_stats_size": (C.c_int64, []),
 }
 
 _lib = None

@@ -52,6 +52,8 @@ struct FluxArgs {
   const double* U1;  // iterate (bond sums, reaction n1 terms)
   const double* Un;  // previous state (lagged phi, reaction n terms, time term)
   double* out;
+  uint16_t* passmask;  // explicit update: bit k set where U_k^{n+1} is not a Dirichlet or clamped
+                       // value (the derivative passes); nullptr: not written
   double A_LL[NNPP_MAX_SPECIES], A_SL[NNPP_MAX_SPECIES], A_SS[NNPP_MAX_SPECIES];
   double M_LL[NNPP_MAX_SPECIES], M_SL[NNPP_MAX_SPECIES], M_SS[NNPP_MAX_SPECIES];
   double z[NNPP_MAX_SPECIES];
@@ -306,6 +308,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
 #pragma unroll
   for (int k = 0; k < S; k++) un[k] = RESID ? Un[(int64_t)k * n + i] : ci[k];
   const unsigned bm = a.bcmask[i];
+  unsigned pass_bits = 0;
   (void)U1;
 
 #pragma unroll
@@ -320,12 +323,18 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
       a.out[(int64_t)k * n + i] = ((ci[k] - un[k]) / a.dt) + flux - Rk;
     } else {
       double v = un[k] - a.dt * (flux - Rk);
-      if ((bm >> k) & 1u) v = a.bcval[(int64_t)k * n + i];
+      bool keep = !((bm >> k) & 1u);
+      if (!keep) v = a.bcval[(int64_t)k * n + i];
       const bool cl = ((a.clamp_always >> k) & 1u) || (((a.clamp_liquid >> k) & 1u) && pi == 0);
-      if (cl && v < a.clamp_floor) v = a.clamp_floor;
+      if (cl && v < a.clamp_floor) {
+        v = a.clamp_floor;
+        keep = false;
+      }
       a.out[(int64_t)k * n + i] = v;
+      pass_bits |= keep ? (1u << k) : 0u;
     }
   }
+  if (!RESID && a.passmask) a.passmask[i] = (uint16_t)pass_bits;
   if (RESID) {
     double o = 0.0;
     if (flagLL) {
@@ -359,7 +368,7 @@ hipError_t launch_s(const FluxArgs& a, int resid, int mixed, int wide, int64_t n
 }  // namespace
 
 int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
-                     const double* Un, double* out) {
+                     const double* Un, double* out, uint16_t* passmask) {
   const nnpp_params& p = c->p;
   const int S = p.n_species;
   FluxArgs a;
@@ -380,6 +389,7 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   a.U1 = U1;
   a.Un = Un;
   a.out = out;
+  a.passmask = passmask;
   const double FRT = p.F / (p.R * p.T);
   a.clamp_always = a.clamp_liquid = 0;
   for (int k = 0; k < NNPP_MAX_SPECIES; k++) {

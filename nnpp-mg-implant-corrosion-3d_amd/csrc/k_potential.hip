@@ -200,6 +200,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
                                                     const double* __restrict__ psc,
                                                     const double* __restrict__ pbdir,
                                                     const double* __restrict__ phi,
+                                                    const double* __restrict__ rhs,
                                                     double* __restrict__ b, double* __restrict__ x,
                                                     double* __restrict__ part) {
   double v[1] = {0.0};
@@ -209,9 +210,13 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
     if (i >= n) continue;
     double bi = 0.0, xi = 0.0;
     if (pfree[i]) {
-      double q = 0.0;
-      for (int k = 0; k < S; k++) q += zs[k] * U[(int64_t)k * n + i];
-      bi = psc[i] * (cq * q + pbdir[i]);
+      if (rhs) {
+        bi = rhs[i];
+      } else {
+        double q = 0.0;
+        for (int k = 0; k < S; k++) q += zs[k] * U[(int64_t)k * n + i];
+        bi = psc[i] * (cq * q + pbdir[i]);
+      }
       xi = phi[i];
     }
     b[i] = bi;
@@ -411,7 +416,7 @@ __global__ void k_scatter_phi(int32_t n, int S, const uint8_t* __restrict__ pfre
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (pfree[i]) phi[i] = x[i];
-  else if ((bcmask[i] >> S) & 1u) phi[i] = bcphi[i];
+  else if (bcphi && ((bcmask[i] >> S) & 1u)) phi[i] = bcphi[i];
 }
 
 // variant = 2 * wide + dictionary
@@ -459,7 +464,8 @@ static constexpr double RELIABLE_DROP2 = 1e-4;  // replace the residual every 2 
 // Uq: state whose species give the charge; phi: warm start in, result out (non-free,
 // non-Dirichlet entries are left as they are).
 int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, double* phi,
-                             int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32) {
+                             int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32,
+                             const double* rhs) {
   const int32_t n = c->n;
   const int S = c->p.n_species;
   hipStream_t s = c->stream;
@@ -479,7 +485,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                              hipMemcpyHostToDevice, s));
   const double cq = (1. / (c->p.K_phi * c->p.sigma1)) * c->p.F * c->p.CSolid;
   k_rhs<<<g, NNPP_BLOCK, 0, s>>>(n, S, Uq, c->scal + SC_COUNT, cq, c->pfree, c->psc, c->pbdir,
-                                 phi, c->cg_b, c->cg_x, part);
+                                 phi, rhs, c->cg_b, c->cg_x, part);
   NNPP_HIP(c, hipGetLastError());
   int st = nnpp_finalize(c, FIN_BNORM2, g, 0.0);
   if (st) return st;
@@ -488,7 +494,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   const double bnorm2 = c->h_scal[SC_BNORM2];
   if (!std::isfinite(bnorm2))
     return nnpp_set_error(c, NNPP_ERR_NONFINITE, "non-finite potential right-hand side");
-  const double* bcphi = c->bcval + (int64_t)S * n;
+  const double* bcphi = rhs ? nullptr : c->bcval + (int64_t)S * n;
   const double rtol2 = c->rtol * c->rtol;
   int64_t n64 = 0, n32 = 0;
   int total = 0;

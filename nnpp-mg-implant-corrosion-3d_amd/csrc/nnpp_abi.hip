@@ -106,7 +106,42 @@ int ensure_potential(nnpp_ctx* c) {
 
 int64_t fields(const nnpp_ctx* c) { return (int64_t)c->p.n_species + 1; }
 
+// Calls that would change what a recorded step depends on are refused while a tape exists.
+int tape_guard(nnpp_ctx* c) {
+  if (c->tape.active)
+    return nnpp_set_error(c, NNPP_ERR_STATE, "a differentiation tape is recording: nnpp_tape_end first");
+  return NNPP_OK;
+}
+
+// Device copy of the state before a recorded step.
+int tape_checkpoint(nnpp_ctx* c) {
+  nnpp_tape_ckpt k;
+  const size_t m = c->n ? c->n : 1;
+  const size_t bytes = sizeof(double) * m * fields(c);
+  NNPP_HIP(c, hipMalloc((void**)&k.U, bytes));
+  if (hipMalloc((void**)&k.phase, m) != hipSuccess) {
+    hipFree(k.U);
+    return nnpp_set_error(c, NNPP_ERR_HIP, "tape checkpoint allocation");
+  }
+  NNPP_HIP(c, hipMemcpyAsync(k.U, c->U[NNPP_SLOT_N], bytes, hipMemcpyDeviceToDevice, c->stream));
+  NNPP_HIP(c, hipMemcpyAsync(k.phase, c->phase, m, hipMemcpyDeviceToDevice, c->stream));
+  k.step = c->step;
+  k.solid_count = c->solid_count;
+  k.voln1rel = c->voln1rel;
+  c->tape.ckpts.push_back(k);
+  c->tape.bytes += (int64_t)(bytes + m);
+  return NNPP_OK;
+}
+
 }  // namespace
+
+void nnpp_tape_free(nnpp_ctx* c) {
+  for (nnpp_tape_ckpt& k : c->tape.ckpts) {
+    dfree(k.U);
+    dfree(k.phase);
+  }
+  c->tape = nnpp_tape();
+}
 
 extern "C" {
 
@@ -135,6 +170,7 @@ int nnpp_destroy(nnpp_ctx* c) {
   if (!c) return NNPP_OK;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  nnpp_tape_free(c);
   free_nodes(c);
   dfree(c->scal);
   if (c->h_scal) hipHostFree(c->h_scal);
@@ -147,6 +183,7 @@ const char* nnpp_last_error(const nnpp_ctx* c) { return c ? c->err.c_str() : "nu
 
 int nnpp_set_params(nnpp_ctx* c, const nnpp_params* p) {
   if (!c || !p) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   const int S = p->n_species;
   if (S < 1 || S > NNPP_MAX_SPECIES)
     return nnpp_set_error(c, NNPP_ERR_ARG, "n_species must be 1..16");
@@ -180,6 +217,7 @@ int nnpp_set_params(nnpp_ctx* c, const nnpp_params* p) {
 int nnpp_set_nodes(nnpp_ctx* c, int32_t n, const double* x, const double* y, const double* z,
                    const double* volumes) {
   if (!c || n < 0 || (n && (!x || !y || !z))) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   int st = need_params(c);
   if (st) return st;
   NNPP_HIP(c, hipSetDevice(c->device));
@@ -240,6 +278,7 @@ int nnpp_set_nodes(nnpp_ctx* c, int32_t n, const double* x, const double* y, con
 int nnpp_build_neighbors(nnpp_ctx* c, double r, double cdx, double cdy, double cdz,
                          int32_t max_len, int64_t* nnz_out, int32_t* max_len_out) {
   if (!c) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   int st = need_params(c);
   if (st) return st;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
@@ -260,6 +299,7 @@ int nnpp_get_neighbors(nnpp_ctx* c, int64_t* row_ptr, int32_t* col) {
 
 int nnpp_set_neighbors(nnpp_ctx* c, const int64_t* row_ptr, const int32_t* col) {
   if (!c || !row_ptr) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
   const int32_t n = c->n;
   if (row_ptr[0] != 0) return nnpp_set_error(c, NNPP_ERR_ARG, "row_ptr[0] != 0");
@@ -291,6 +331,7 @@ int nnpp_set_neighbors(nnpp_ctx* c, const int64_t* row_ptr, const int32_t* col) 
 
 int nnpp_set_bc(nnpp_ctx* c, const int32_t* bctypes, const double* bcvalues) {
   if (!c) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   int st = need_params(c);
   if (st) return st;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
@@ -321,6 +362,7 @@ int nnpp_set_bc(nnpp_ctx* c, const int32_t* bctypes, const double* bcvalues) {
 
 int nnpp_set_state(nnpp_ctx* c, int32_t slot, const double* U) {
   if (!c || !U || slot < 0 || slot > 1) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   int st = need_params(c);
   if (st) return st;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
@@ -342,6 +384,7 @@ int nnpp_get_state(nnpp_ctx* c, int32_t slot, double* U) {
 
 int nnpp_set_phase(nnpp_ctx* c, const int32_t* phase) {
   if (!c || !phase) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
   std::vector<uint8_t> h(c->n);
   int64_t sc = 0;
@@ -369,6 +412,7 @@ int nnpp_get_phase(nnpp_ctx* c, int32_t* phase) {
 
 int nnpp_init_state(nnpp_ctx* c, const double* ic) {
   if (!c || !ic) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   int st = need_params(c);
   if (st) return st;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
@@ -410,9 +454,18 @@ int nnpp_init_state(nnpp_ctx* c, const double* ic) {
 
 int nnpp_set_time(nnpp_ctx* c, int64_t step, double voln1rel, double vol0rel) {
   if (!c) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   c->step = step;
   c->voln1rel = voln1rel;
   c->vol0rel = vol0rel;
+  return NNPP_OK;
+}
+
+int nnpp_get_time(nnpp_ctx* c, int64_t* step, double* voln1rel, double* vol0rel) {
+  if (!c) return NNPP_ERR_ARG;
+  if (step) *step = c->step;
+  if (voln1rel) *voln1rel = c->voln1rel;
+  if (vol0rel) *vol0rel = c->vol0rel;
   return NNPP_OK;
 }
 
@@ -451,6 +504,7 @@ int nnpp_poisson_apply(nnpp_ctx* c, const double* x, double* y) {
 
 int nnpp_set_solver(nnpp_ctx* c, double rtol, int32_t maxit, double inner_rtol) {
   if (!c || !(rtol >= 0) || maxit < 0 || !(inner_rtol > 0 && inner_rtol < 1)) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   c->rtol = rtol;
   c->maxit = maxit;
   c->inner_rtol = inner_rtol;
@@ -466,6 +520,7 @@ int nnpp_set_format(nnpp_ctx* c, int32_t flags) {
 
 int nnpp_poisson_solve(nnpp_ctx* c, int32_t precision, int32_t* iters, double* relres) {
   if (!c) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = ensure_potential(c);
   if (st) return st;
@@ -476,6 +531,7 @@ int nnpp_poisson_solve(nnpp_ctx* c, int32_t precision, int32_t* iters, double* r
 
 int nnpp_front_update(nnpp_ctx* c, double threshold, int32_t* n_flipped) {
   if (!c) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = need_params(c);
   if (st) return st;
@@ -483,57 +539,86 @@ int nnpp_front_update(nnpp_ctx* c, double threshold, int32_t* n_flipped) {
   return nnpp_front_dev(c, c->U[NNPP_SLOT_N], threshold, 0, n_flipped, nullptr);
 }
 
+}  // extern "C"
+
+int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_step* rec) {
+  int st = ensure_potential(c);
+  if (st) return st;
+  const int32_t n = c->n;
+  const int S = c->p.n_species;
+  double* U = c->U[NNPP_SLOT_N];
+  if (rec) {
+    rec->step = c->step;
+    rec->voln1rel = c->voln1rel;
+  }
+  // 1. species update from U^n (lagged phi^n)
+  const double BV = std::pow(10., -c->p.vol_red * (1. - c->voln1rel));
+  st = nnpp_launch_flux(c, 0, precision, BV, U, U, c->Unew);
+  if (st) return st;
+  // 2. phi^{n+1} from the charge of U^n, warm-started from phi^n, written into Unew
+  double* phi_new = c->Unew + (int64_t)S * n;
+  if (n)
+    NNPP_HIP(c, hipMemcpyAsync(phi_new, U + (int64_t)S * n, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, c->stream));
+  int32_t its = 0;
+  double rr = 0.0;
+  int status = NNPP_OK;
+  st = nnpp_potential_solve_dev(c, precision, U, phi_new, &its, &rr, &loc.cg_spmv_fp64,
+                                &loc.cg_spmv_fp32);
+  if (st == NNPP_ERR_NOCONV) status = NNPP_ERR_NOCONV;
+  else if (st) return st;
+  std::swap(c->U[NNPP_SLOT_N], c->Unew);
+  // 3. front update and 4. volume cadence
+  int32_t flips = 0;
+  double volsum = 0.0;
+  st = nnpp_front_dev(c, c->U[NNPP_SLOT_N], c->p.front_threshold, 1, &flips, &volsum);
+  if (st) return st;
+  const bool vol_update = c->p.sub_steps > 0 && (c->step % c->p.sub_steps) == 0 &&
+                          c->vol0rel != 0.0;
+  if (vol_update) c->voln1rel = volsum / c->vol0rel;
+  if (rec) {
+    rec->flips = flips;
+    rec->vol_update = vol_update ? 1 : 0;
+  }
+  c->step++;
+  loc.steps++;
+  loc.cg_iterations += its;
+  loc.last_relres = rr;
+  loc.flipped += flips;
+  loc.nnz_flux = c->bonds.nnz;
+  loc.nnz_ll = c->pot.nnz;
+  loc.pot_col_bytes = c->pot.wide ? 4 : 2;
+  loc.pot_weight_bytes = c->pot.ncls > 0 ? 1 : 4;
+  return status;
+}
+
+extern "C" {
+
 int nnpp_step(nnpp_ctx* c, int32_t nsteps, int32_t precision, int32_t check_finite,
               nnpp_step_stats* stats) {
   if (!c || nsteps < 0) return NNPP_ERR_ARG;
   if (precision != NNPP_FP64 && precision != NNPP_MIXED) return NNPP_ERR_ARG;
+  if (c->tape.active && precision != NNPP_FP64)
+    return nnpp_set_error(c, NNPP_ERR_UNSUPPORTED, "a differentiation tape records FP64 steps only");
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = ensure_bonds(c);
   if (st) return st;
   nnpp_step_stats loc;
   std::memset(&loc, 0, sizeof(loc));
-  const int32_t n = c->n;
-  const int S = c->p.n_species;
   int status = NNPP_OK;
   for (int32_t k = 0; k < nsteps; k++) {
-    st = ensure_potential(c);
-    if (st) return st;
-    double* U = c->U[NNPP_SLOT_N];
-    // 1. species update from U^n (lagged phi^n)
-    const double BV = std::pow(10., -c->p.vol_red * (1. - c->voln1rel));
-    st = nnpp_launch_flux(c, 0, precision, BV, U, U, c->Unew);
-    if (st) return st;
-    // 2. phi^{n+1} from the charge of U^n, warm-started from phi^n, written into Unew
-    double* phi_new = c->Unew + (int64_t)S * n;
-    if (n)
-      NNPP_HIP(c, hipMemcpyAsync(phi_new, U + (int64_t)S * n, sizeof(double) * n,
-                                 hipMemcpyDeviceToDevice, c->stream));
-    int32_t its = 0;
-    double rr = 0.0;
-    st = nnpp_potential_solve_dev(c, precision, U, phi_new, &its, &rr, &loc.cg_spmv_fp64,
-                                  &loc.cg_spmv_fp32);
+    nnpp_tape_step rec{};
+    if (c->tape.active && c->tape.steps.size() % (size_t)c->tape.every == 0) {
+      st = tape_checkpoint(c);
+      if (st) return st;
+    }
+    st = nnpp_step_once(c, precision, loc, c->tape.active ? &rec : nullptr);
     if (st == NNPP_ERR_NOCONV) status = NNPP_ERR_NOCONV;
     else if (st) {
       if (stats) *stats = loc;
       return st;
     }
-    std::swap(c->U[NNPP_SLOT_N], c->Unew);
-    // 3. front update and 4. volume cadence
-    int32_t flips = 0;
-    double volsum = 0.0;
-    st = nnpp_front_dev(c, c->U[NNPP_SLOT_N], c->p.front_threshold, 1, &flips, &volsum);
-    if (st) return st;
-    if (c->p.sub_steps > 0 && (c->step % c->p.sub_steps) == 0 && c->vol0rel != 0.0)
-      c->voln1rel = volsum / c->vol0rel;
-    c->step++;
-    loc.steps++;
-    loc.cg_iterations += its;
-    loc.last_relres = rr;
-    loc.flipped += flips;
-    loc.nnz_flux = c->bonds.nnz;
-    loc.nnz_ll = c->pot.nnz;
-    loc.pot_col_bytes = c->pot.wide ? 4 : 2;
-    loc.pot_weight_bytes = c->pot.ncls > 0 ? 1 : 4;
+    if (c->tape.active) c->tape.steps.push_back(rec);
     if (c->solid_count <= 0) break;  // main.cpp:924-932
   }
   loc.solid_count = c->solid_count;
@@ -548,6 +633,40 @@ int nnpp_step(nnpp_ctx* c, int32_t nsteps, int32_t precision, int32_t check_fini
   if (stats) *stats = loc;
   return status;
 }
+
+int nnpp_tape_begin(nnpp_ctx* c, int32_t every) {
+  if (!c || every < 1) return NNPP_ERR_ARG;
+  if (int g_ = tape_guard(c)) return g_;
+  int st = need_graph(c);
+  if (st) return st;
+  if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
+  if (c->p.chemistry != NNPP_CHEM_GENERIC)
+    return nnpp_set_error(c, NNPP_ERR_UNSUPPORTED, "the adjoint covers the generic chemistry");
+  nnpp_tape_free(c);
+  c->tape.active = true;
+  c->tape.every = every;
+  return NNPP_OK;
+}
+
+int nnpp_tape_adjoint(nnpp_ctx* c, const double* lam, double* grad, double* lam0,
+                      nnpp_adjoint_stats* stats) {
+  if (!c || !lam || !grad) return NNPP_ERR_ARG;
+  if (!c->tape.active) return nnpp_set_error(c, NNPP_ERR_STATE, "no tape (nnpp_tape_begin)");
+  NNPP_HIP(c, hipSetDevice(c->device));
+  int st = ensure_bonds(c);
+  if (st) return st;
+  return nnpp_adjoint_run(c, lam, grad, lam0, stats);
+}
+
+int nnpp_tape_end(nnpp_ctx* c) {
+  if (!c) return NNPP_ERR_ARG;
+  NNPP_HIP(c, hipSetDevice(c->device));
+  NNPP_HIP(c, hipStreamSynchronize(c->stream));
+  nnpp_tape_free(c);
+  return NNPP_OK;
+}
+
+int64_t nnpp_abi_adjoint_stats_size(void) { return (int64_t)sizeof(nnpp_adjoint_stats); }
 
 int nnpp_state_device_ptr(nnpp_ctx* c, int32_t slot, int32_t field, void** dptr) {
   if (!c || !dptr || slot < 0 || slot > 1 || field < 0 || field > c->p.n_species)

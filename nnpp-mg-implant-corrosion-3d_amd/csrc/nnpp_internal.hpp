@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/nnpp.h"
 
@@ -53,6 +54,27 @@ struct nnpp_slices {
   int ncls = 0;
 };
 #define NNPP_DICT_MAX 256
+
+// Differentiation tape (nnpp_tape_begin .. nnpp_tape_end).
+struct nnpp_tape_step {  // host record of one recorded step
+  int64_t step;          // step counter before the step
+  double voln1rel;       // voln1rel in force during the step (its BV)
+  int32_t flips;         // solid -> liquid flips at its end
+  int32_t vol_update;    // voln1rel recomputed at its end
+};
+struct nnpp_tape_ckpt {  // state before recorded step `every * index`
+  double* U = nullptr;   // F*n
+  uint8_t* phase = nullptr;
+  int64_t step = 0, solid_count = 0;
+  double voln1rel = 0.0;
+};
+struct nnpp_tape {
+  bool active = false;
+  int32_t every = 1;
+  std::vector<nnpp_tape_step> steps;
+  std::vector<nnpp_tape_ckpt> ckpts;
+  int64_t bytes = 0;
+};
 
 struct nnpp_ctx {
   int device = 0;
@@ -125,6 +147,8 @@ struct nnpp_ctx {
   double voln1rel = 0.0;
   double vol0rel = 0.0;
   int64_t solid_count = 0;
+
+  nnpp_tape tape;
 };
 
 // Device scalars used by the CG kernels and reductions.
@@ -306,9 +330,12 @@ int nnpp_build_neighbors_dev(nnpp_ctx* c, double r, double cdx, double cdy, doub
 int nnpp_build_bond_slices(nnpp_ctx* c);
 int nnpp_build_potential(nnpp_ctx* c);
 int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
-                     const double* Un, double* out);
+                     const double* Un, double* out, uint16_t* passmask = nullptr);
+// rhs == nullptr: b from the charge of Uq, Dirichlet nodes of phi set to their values;
+// rhs given: b = rhs on the free rows (the adjoint solve), only free nodes of phi written.
 int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, double* phi,
-                             int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32);
+                             int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32,
+                             const double* rhs = nullptr);
 int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out);
 int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flips,
                    double* volsum);
@@ -317,3 +344,8 @@ int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision);
 int nnpp_finalize(nnpp_ctx* c, int op, int64_t nblk, double a0);
 int nnpp_build_dictionary(nnpp_ctx* c, nnpp_slices& sl);
+// One explicit step of nnpp_step (flux, potential, front, voln1rel); fills *rec if non-null.
+int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_step* rec);
+int nnpp_adjoint_run(nnpp_ctx* c, const double* lam, double* grad, double* lam0,
+                     nnpp_adjoint_stats* stats);
+void nnpp_tape_free(nnpp_ctx* c);

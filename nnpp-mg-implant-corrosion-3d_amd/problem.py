@@ -120,6 +120,25 @@ class Problem:
     def set_time(self, step, voln1rel, vol0rel):
         self._check(self.lib.nnpp_set_time(self.ctx, step, voln1rel, vol0rel))
 
+    def get_time(self):
+        """(step counter, voln1rel, vol0rel)."""
+        st, vr, v0 = C.c_int64(), C.c_double(), C.c_double()
+        self._check(self.lib.nnpp_get_time(self.ctx, C.byref(st), C.byref(vr), C.byref(v0)))
+        return int(st.value), float(vr.value), float(v0.value)
+
+    def set_params(self, params):
+        self.params = params.copy()
+        self._check(self.lib.nnpp_set_params(self.ctx, C.byref(self.params)))
+
+    def snapshot(self):
+        """Host copy of everything a step depends on besides parameters and geometry."""
+        return {"U": self.get_state(), "phase": self.get_phase(), "time": self.get_time()}
+
+    def restore(self, snap):
+        self.set_state(snap["U"])
+        self.set_phase(snap["phase"])
+        self.set_time(*snap["time"])
+
     def set_solver(self, rtol=1e-10, maxit=20000, inner_rtol=1e-4):
         self._check(self.lib.nnpp_set_solver(self.ctx, rtol, maxit, inner_rtol))
 
@@ -172,6 +191,27 @@ class Problem:
         p = C.c_void_p()
         self._check(self.lib.nnpp_stream(self.ctx, C.byref(p)))
         return p.value
+
+    # -- differentiation -------------------------------------------------------------------
+    def tape_begin(self, checkpoint_every=1):
+        """Record the following FP64 steps for tape_adjoint (state checkpoint every k steps)."""
+        self._check(self.lib.nnpp_tape_begin(self.ctx, int(checkpoint_every)))
+
+    def tape_adjoint(self, lam, want_lam0=False):
+        """Reverse the recorded steps for dL/dU = lam (F, n) at the current state. Returns
+        (grad, lam0, stats): grad = [dL/dD_liquid (S), dL/dD_solid (S), dL/dr_diss]."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        if lam.shape != (self.F, self.n):
+            raise ValueError(f"lam must have shape {(self.F, self.n)}")
+        grad = np.zeros(2 * self.S + 1)
+        lam0 = np.empty((self.F, self.n)) if want_lam0 else None
+        st = _abi.AdjointStats()
+        self._check(self.lib.nnpp_tape_adjoint(self.ctx, dptr(lam), dptr(grad),
+                                               dptr(lam0) if want_lam0 else None, C.byref(st)))
+        return grad, lam0, st.as_dict()
+
+    def tape_end(self):
+        self._check(self.lib.nnpp_tape_end(self.ctx))
 
     def time_kernel(self, kernel, precision=FP64, reps=20):
         """Mean duration (ms) of one launch of a hot kernel, HIP events on the context stream."""

@@ -31,6 +31,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match():
     assert nn.lib.nnpp_abi_params_size() == C.sizeof(nn.Params)
     assert nn.lib.nnpp_abi_stats_size() == C.sizeof(nn.StepStats)
+    assert nn.lib.nnpp_abi_adjoint_stats_size() == C.sizeof(nn._abi.AdjointStats)
 
 
 def test_params_defaults_and_errors():
