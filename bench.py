@@ -109,6 +109,120 @@ def kernels_only(nn, case, reps):
     print(json.dumps(out))
 
 
+def validate(nn, kind, ng, nsteps, rtol):
+    """Mixed against FP64 after nsteps steps from the same initial state: max and L2 relative
+    error per field (species 1..S, then phi)."""
+    case = (nn.synth.regular if kind == "regular" else nn.synth.screw)(ng, S=11, seed=0)
+    pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+    pb.build_clouds_hash()
+    rp, col = pb.neighbors()
+    pb.close()
+    states, info = {}, {}
+    for name, prec in (("fp64", nn.FP64), ("mixed", nn.MIXED)):
+        pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+        pb.set_neighbors(rp, col)
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=rtol, maxit=50000)
+        t0 = time.perf_counter()
+        its = flips = 0
+        done = 0
+        while done < nsteps:
+            k = min(100, nsteps - done)
+            st = pb.step(k, prec)
+            if st["status"] != 0:
+                raise RuntimeError(f"{name}: step status {st['status']}")
+            done += k
+            its += st["cg_iterations"]
+            flips += st["flipped"]
+            log(f"[validate] {name}: {done}/{nsteps} steps, {time.perf_counter() - t0:.1f} s")
+        info[name] = {"s": time.perf_counter() - t0, "cg_iterations": its, "flipped": flips,
+                      "voln1rel": st["voln1rel"]}
+        states[name] = (pb.get_state(), pb.get_phase())
+        pb.close()
+    errs = field_errors(states["mixed"][0], states["fp64"][0])
+    print(json.dumps({"validate": f"{kind} {ng}^3", "steps": nsteps, "rtol": rtol,
+                      "max_rel": max(e["max"] for e in errs),
+                      "max_l2": max(e["l2"] for e in errs), "fields": errs,
+                      "phase_equal": bool(np.array_equal(states["mixed"][1], states["fp64"][1])),
+                      "runs": info}))
+
+
+def adjoint_workload(nn, ng, nsteps, every=20):
+    """Differentiation workload: regular ng^3, 11 species, nsteps FP64 steps. Loss = L2 mismatch
+    of the final Mg2+ and phi fields against a target made with every diffusivity x1.05 (case
+    seed 1). Gradients with respect to D_liquid, D_solid and r_diss through the torch binding,
+    checkpointed (every `every` steps) and with the full tape; central differences (h = 1e-6
+    relative) for a subset of the parameters; device bytes of each tape."""
+    import torch
+    case = nn.synth.regular(ng, S=11, seed=1)
+    S = case.params.S
+    pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+    pb.build_clouds_hash()
+    pb.set_bc(case.bctypes, case.bcvalues)
+    pb.init_state(case.icvalues)
+    pb.set_solver(rtol=1e-13, maxit=100000)
+    start = pb.snapshot()
+    Dl0 = torch.tensor([case.params.D_liquid[k] for k in range(S)], dtype=torch.float64)
+    Ds0 = torch.tensor([case.params.D_solid[k] for k in range(S)], dtype=torch.float64)
+    rd0 = torch.tensor(case.params.r_diss, dtype=torch.float64)
+    sim = nn.autograd.simulate
+    with torch.no_grad():
+        target = sim(pb, Dl0 * 1.05, Ds0 * 1.05, rd0, nsteps, start, every).clone()
+
+    def loss_of(U):
+        return ((U[0] - target[0]) ** 2).sum() + ((U[S] - target[S]) ** 2).sum()
+
+    runs = {}
+    for k_ in (every, 1):
+        Dl, Ds, rd = (t.clone().requires_grad_(True) for t in (Dl0, Ds0, rd0))
+        t0 = time.perf_counter()
+        U = sim(pb, Dl, Ds, rd, nsteps, start, k_)
+        t1 = time.perf_counter()
+        loss = loss_of(U)
+        loss.backward()
+        t2 = time.perf_counter()
+        st = pb.last_adjoint_stats
+        runs[k_] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "loss": float(loss),
+                    "tape_bytes": st["tape_bytes"], "recomputed_steps": st["recomputed_steps"],
+                    "adjoint_cg_iterations": st["cg_iterations"],
+                    "grad": np.concatenate([Dl.grad.numpy(), Ds.grad.numpy(),
+                                            [float(rd.grad)]]).tolist()}
+        log(f"[adjoint] checkpoint every {k_}: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s,"
+            f" tape {st['tape_bytes'] / 2**20:.0f} MiB")
+    pb.tape_end()
+    g = np.array(runs[every]["grad"])
+    same = bool(np.array_equal(g, np.array(runs[1]["grad"])))
+    # central differences of the forward for a subset of the parameters
+    names = [("D_liquid", 0), ("D_liquid", 5), ("D_solid", 0), ("D_solid", 7), ("r_diss", None)]
+    fd = []
+    for name, k in names:
+        base = getattr(case.params, name) if k is None else getattr(case.params, name)[k]
+        h = 1e-6 * abs(base)
+        L = []
+        for sgn in (1, -1):
+            p = case.params.copy()
+            if k is None:
+                setattr(p, name, base + sgn * h)
+            else:
+                getattr(p, name)[k] = base + sgn * h
+            pb.set_params(p)
+            pb.restore(start)
+            pb.step(nsteps, nn.FP64)
+            L.append(float(loss_of(torch.from_numpy(pb.get_state()))))
+        q = k if name == "D_liquid" else (S + k if name == "D_solid" else 2 * S)
+        fdv = (L[0] - L[1]) / (2 * h)
+        fd.append({"param": f"{name}[{k}]" if k is not None else name, "adjoint": float(g[q]),
+                   "fd": fdv, "rel": abs(g[q] - fdv) / max(abs(fdv), 1e-300)})
+        log(f"[adjoint] fd {fd[-1]}")
+    pb.close()
+    print(json.dumps({"adjoint": f"regular {ng}^3, {S} species, {nsteps} FP64 steps",
+                      "loss": runs[every]["loss"], "checkpointed": runs[every],
+                      "full_tape": runs[1], "checkpoint_every": every,
+                      "checkpointed_equals_full_tape": same, "finite_differences": fd,
+                      "max_fd_rel": max(f["rel"] for f in fd)}))
+
+
 def copy_probe(mib):
     """A plain device copy of known size, to scale FETCH_SIZE / WRITE_SIZE counters."""
     import torch
@@ -222,6 +336,14 @@ def main():
     ap.add_argument("--summarize-rocprof", nargs="+", metavar="DB",
                     help="trace.db [fetch.db write.db copyfetch.db copywrite.db copy_MiB]")
     ap.add_argument("--traffic-out", default=None)
+    ap.add_argument("--validate", type=int, default=0, metavar="STEPS",
+                    help="only run the FP64 and mixed steps STEPS times from the same state and "
+                         "report the per-field error of mixed against FP64")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only run the differentiation workload (64^3, 200 FP64 steps, "
+                         "checkpoints every 20; loss against a +5%% diffusivity target)")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-grid", type=int, default=64)
     a = ap.parse_args()
     if a.summarize_rocprof:
         d = a.summarize_rocprof
@@ -241,6 +363,12 @@ def main():
     ng = a.grid
     if a.kernels_only:
         kernels_only(nn, nn.synth.regular(ng, S=11, seed=0), 20)
+        return
+    if a.validate:
+        validate(nn, a.case, ng, a.validate, a.rtol)
+        return
+    if a.adjoint:
+        adjoint_workload(nn, a.adjoint_grid, a.adjoint_steps)
         return
     log(f"[bench] building {a.case} {ng}^3 case")
     case = (nn.synth.regular if a.case == "regular" else nn.synth.screw)(ng, S=11, seed=0)
