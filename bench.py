@@ -302,6 +302,21 @@ def summarize_rocprof(trace_db, fetch_db=None, write_db=None, copyf_db=None, cop
     print("\n".join(lines))
 
 
+def roofline_entry(ach, alg_bytes, fmt_bytes, t_ms, stencil, cb, wb):
+    traffic = measured_traffic("k_m_spmv")
+    out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+           "kernel": "k_m_spmv (FP32 potential SpMV + p.q)",
+           "algorithmic_bytes_per_launch": alg_bytes,
+           "format_bytes_per_launch": fmt_bytes,
+           "format_GBps": fmt_bytes / (t_ms * 1e-3) / 1e9,
+           "operator_format": ({"stencil": True, "mask_bytes_per_row": 16} if stencil else
+                               {"stencil": False, "col_bytes": cb, "weight_bytes": wb})}
+    if traffic:
+        out["hbm_measured_GBps"] = traffic / (t_ms * 1e-3) / 1e9
+    return out
+
+
 def measured_traffic(kernel_prefix):
     """HBM bytes per dispatch from the committed counter summary, if it matches these sources."""
     try:
@@ -409,20 +424,31 @@ def main():
     t_spmv64 = pbm.time_kernel(nn._abi.KERNEL_POT_SPMV, nn.FP64, reps)
     t_flux64 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.FP64, 10)
     t_flux32 = pbm.time_kernel(nn._abi.KERNEL_FLUX, nn.MIXED, 10)
-    # algorithmic bytes per launch of the potential SpMV: per liquid bond the stored column (a
-    # 16-bit step, or 4 bytes) and the weight (a 1-byte dictionary class, or the FP32/FP64
-    # value); per node the first column (4), p_i (gathered p_j counted once), dd_i, q_i written
-    cb = mixed["stats"]["pot_col_bytes"]
-    wb32 = mixed["stats"]["pot_weight_bytes"]
-    wb64 = 1 if wb32 == 1 else 8
-    bytes32 = nll * (cb + wb32) + n * (4 + 3 * 4)
-    bytes64 = nll * (cb + wb64) + n * (4 + 3 * 8)
+    # Algorithmic bytes of one potential-SpMV launch, the per-unit figure of the CSR operator:
+    # a 4-byte column per liquid bond, 8 bytes of row pointer per row, x in and y out (FP64).
+    alg_bytes = 4 * nll + 8 * (n + 1) + 16 * n
+    # Bytes the stored form itself must move per launch (the operator as held on the device):
+    # stencil form: a 16-byte mask per row; column form: per liquid bond the column (2-byte
+    # step or 4 bytes) and the weight (1-byte class, or the value), plus the first column per
+    # row; both: p_i (each gathered p_j counted once), dd_i, q_i.
+    st_ = mixed["stats"]
+    stencil = bool(st_["pot_stencil"])
+    cb, wb32 = st_["pot_col_bytes"], st_["pot_weight_bytes"]
+    wb64 = 1 if wb32 == 1 else (0 if stencil else 8)
+    if stencil:
+        bytes32 = n * (16 + 3 * 4)
+        bytes64 = n * (16 + 3 * 8)
+    else:
+        bytes32 = nll * (cb + wb32) + n * (4 + 3 * 4)
+        bytes64 = nll * (cb + wb64) + n * (4 + 3 * 8)
     S = case.S
     fbytes = lambda wb: nnz * (4 + wb) + n * (8 * (S + 1) + 1 + 2 + 8 * S)  # noqa: E731
-    ach = bytes32 / (t_spmv32 * 1e-3) / 1e9
+    ach = alg_bytes / (t_spmv32 * 1e-3) / 1e9
     kern = {
-        "pot_spmv_fp32": {"ms": t_spmv32, "GBps": bytes32 / (t_spmv32 * 1e-3) / 1e9},
-        "pot_spmv_fp64": {"ms": t_spmv64, "GBps": bytes64 / (t_spmv64 * 1e-3) / 1e9},
+        "pot_spmv_fp32": {"ms": t_spmv32, "alg_GBps": alg_bytes / (t_spmv32 * 1e-3) / 1e9,
+                          "format_GBps": bytes32 / (t_spmv32 * 1e-3) / 1e9},
+        "pot_spmv_fp64": {"ms": t_spmv64, "alg_GBps": alg_bytes / (t_spmv64 * 1e-3) / 1e9,
+                          "format_GBps": bytes64 / (t_spmv64 * 1e-3) / 1e9},
         "flux_fp64": {"ms": t_flux64, "GBps": fbytes(8) / (t_flux64 * 1e-3) / 1e9},
         "flux_mixed": {"ms": t_flux32, "GBps": fbytes(4) / (t_flux32 * 1e-3) / 1e9},
     }
@@ -464,11 +490,7 @@ def main():
         "mixed_vs_fp64_err": {"max_rel": max_err, "fields": errs, "after_steps": a.steps + a.warmup},
         "neighbor_build_s": t_nb,
         "kernels": kern,
-        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("k_m_spmv"),
-                     "kernel": "k_m_spmv (FP32 potential SpMV + dot)",
-                     "bytes_per_launch": bytes32,
-                     "operator_format": {"col_bytes": cb, "weight_bytes": wb32}},
+        "roofline": roofline_entry(ach, alg_bytes, bytes32, t_spmv32, stencil, cb, wb32),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -123,7 +123,10 @@ typedef struct nnpp_step_stats {
   int64_t nnz_ll;           /* liquid-liquid directed bonds in the last Poisson operator */
   int32_t pot_col_bytes;    /* bytes per stored column of that operator: 2 (steps) or 4 */
   int32_t pot_weight_bytes; /* bytes per stored weight read in mixed mode: 1 (dictionary
-                               class) or 4 (FP32 value) */
+                               class) or 4 (FP32 value); 0 for the stencil form */
+  int32_t pot_stencil;      /* the operator used the stencil form (16 bytes per row, no
+                               per-entry stream; pot_col_bytes = 0) */
+  int32_t reserved0;
 } nnpp_step_stats;
 
 typedef struct nnpp_ctx nnpp_ctx;
@@ -251,20 +254,31 @@ int nnpp_residuals(nnpp_ctx* ctx, double voln1rel, int32_t apply_bc, int32_t pre
  * elsewhere; w_ij = Beta(xi) V_j / xi^(3+2 s_phi). Uses the phase on the device. */
 int nnpp_poisson_apply(nnpp_ctx* ctx, const double* x, double* y);
 
+/* The operator the potential solve iterates with, applied once in the solver's precision
+ * (NNPP_FP64, or NNPP_MIXED: FP32 values and Kahan-compensated FP32 row sums) and storage
+ * format: y_i = s_i (sum over liquid bonds of w_ij x_i - sum over free-free bonds of w_ij x_j)
+ * on free rows (s_i = V_i/(dx dy dz)), 0 elsewhere; x and y are n doubles (converted to and
+ * from FP32 in mixed mode). Checks the hot SpMV kernel in isolation. No reference
+ * counterpart (K11 of utils.h:1794-1848 restricted to the free rows and scaled). */
+int nnpp_potential_operator_apply(nnpp_ctx* ctx, int32_t precision, const double* x, double* y);
+
 /* Solver controls for the potential (replaces Eigen GMRES at main.cpp:764-779):
  * relative residual rtol (default 1e-10), maxit (default 20000); for NNPP_MIXED the inner
  * FP32 solve reduces the residual by inner_rtol per refinement pass (default 1e-4). */
 int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol);
 
 /* Storage format of the bond slices and the potential operator. NNPP_FORMAT_AUTO (default):
- * 16-bit column steps unless a row needs a larger step, a 1-byte weight-class dictionary for
- * the potential operator when it has at most 255 distinct nonzero weights. The flags force
- * 32-bit columns and/or plain FP64/FP32 weights. Every format gives bit-identical results;
- * they differ in bytes read per bond. No reference counterpart (the reference stores an
+ * the stencil form of the potential operator when the neighbour lists are a lattice stencil
+ * (every j - i in a table of at most 128 offsets, weights depending on the offset only, no
+ * per-node volumes): one 128-bit mask per row; otherwise 16-bit column steps unless a row
+ * needs a larger step, and a 1-byte weight-class dictionary when the operator has at most 255
+ * distinct nonzero weights. The flags force 32-bit columns, plain FP64/FP32 weights and/or the
+ * column form. Every format gives bit-identical results; they differ in bytes read per bond. No reference counterpart (the reference stores an
  * Eigen::SparseMatrix, utils.h:1794-1848). Takes effect at the next operator build. */
 #define NNPP_FORMAT_AUTO 0
 #define NNPP_FORMAT_WIDE_COLUMNS 1
 #define NNPP_FORMAT_NO_DICTIONARY 2
+#define NNPP_FORMAT_NO_STENCIL 4
 int nnpp_set_format(nnpp_ctx* ctx, int32_t flags);
 
 /* Solve the potential for the charge of slot N: phi^{n+1} on free nodes (liquid, with a

@@ -5,16 +5,16 @@ library; importing this package loads it and fails if it is missing.
 """
 from . import _abi
 from ._abi import (CHEM_GENERIC, CHEM_REFERENCE, FORMAT_AUTO, FORMAT_NO_DICTIONARY,
-                   FORMAT_WIDE_COLUMNS, FP64, MIXED, SLOT_N, SLOT_N1, NnppError, Params,
-                   StepStats)
+                   FORMAT_NO_STENCIL, FORMAT_WIDE_COLUMNS, FP64, MIXED, SLOT_N, SLOT_N1,
+                   NnppError, Params, StepStats)
 from .problem import Problem
 from . import synth, formats
 
 lib = _abi.load()
 
 __all__ = ["CHEM_GENERIC", "CHEM_REFERENCE", "FORMAT_AUTO", "FORMAT_NO_DICTIONARY",
-           "FORMAT_WIDE_COLUMNS", "FP64", "MIXED", "SLOT_N", "SLOT_N1", "NnppError",
-           "Params", "StepStats", "Problem", "synth", "formats", "lib", "autograd"]
+           "FORMAT_NO_STENCIL", "FORMAT_WIDE_COLUMNS", "FP64", "MIXED", "SLOT_N", "SLOT_N1",
+           "NnppError", "Params", "StepStats", "Problem", "synth", "formats", "lib", "autograd"]
 
 
 def __getattr__(name):  # the torch binding is imported on first use

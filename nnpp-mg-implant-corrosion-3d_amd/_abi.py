@@ -18,7 +18,7 @@ FP64, MIXED = 0, 1
 CLAMP_NONE, CLAMP_ALWAYS, CLAMP_LIQUID = 0, 1, 2
 SLOT_N, SLOT_N1 = 0, 1
 KERNEL_FLUX, KERNEL_POT_SPMV = 0, 1
-FORMAT_AUTO, FORMAT_WIDE_COLUMNS, FORMAT_NO_DICTIONARY = 0, 1, 2
+FORMAT_AUTO, FORMAT_WIDE_COLUMNS, FORMAT_NO_DICTIONARY, FORMAT_NO_STENCIL = 0, 1, 2, 4
 
 # .data2 field names in file order (utils.h:2348-2398); pKH2O is the reference's pKH20
 DATA2_FIELDS = [
@@ -74,6 +74,7 @@ class StepStats(C.Structure):
         ("cg_spmv_fp32", C.c_int64), ("last_relres", C.c_double), ("flipped", C.c_int64),
         ("solid_count", C.c_int64), ("voln1rel", C.c_double), ("nnz_flux", C.c_int64),
         ("nnz_ll", C.c_int64), ("pot_col_bytes", C.c_int32), ("pot_weight_bytes", C.c_int32),
+        ("pot_stencil", C.c_int32), ("reserved0", C.c_int32),
     ]
 
     def as_dict(self):
@@ -132,6 +133,7 @@ SIGNATURES = {
     "nnpp_get_time": (C.c_int, [_p, _lp, _dp, _dp]),
     "nnpp_residuals": (C.c_int, [_p, _d, _i32, _i32, _dp]),
     "nnpp_poisson_apply": (C.c_int, [_p, _dp, _dp]),
+    "nnpp_potential_operator_apply": (C.c_int, [_p, _i32, _dp, _dp]),
     "nnpp_set_solver": (C.c_int, [_p, _d, _i32, _d]),
     "nnpp_set_format": (C.c_int, [_p, _i32]),
     "nnpp_poisson_solve": (C.c_int, [_p, _i32, _ip, _dp]),

@@ -424,6 +424,181 @@ __global__ void k_dict_map(int64_t slots, const double* __restrict__ w,
   }
 }
 
+// ---- stencil form of the potential operator ----
+
+// Offsets j - i of every bond into a hash set (keys stored as the 32-bit offset, never 0 since
+// rows exclude themselves), with one row that has each offset; gives up (fail) beyond
+// NNPP_STENCIL_MAX distinct offsets or on a long probe chain.
+__global__ void k_st_collect(int32_t n, const int64_t* __restrict__ row_ptr,
+                             const int32_t* __restrict__ col, unsigned long long* __restrict__ table,
+                             int32_t* __restrict__ rep, int* __restrict__ count,
+                             int* __restrict__ fail) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  unsigned long long last = 0;
+  for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+    const unsigned long long key = (unsigned long long)(uint32_t)(col[b] - (int32_t)i);
+    if (key == last) continue;
+    last = key;
+    unsigned h = (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 52);
+    int probe = 0;
+    for (; probe < 32; probe++, h = (h + 1) & (DICT_HASH - 1)) {
+      const unsigned long long cur = table[h];
+      if (cur == key) break;
+      if (cur == 0ull) {
+        const unsigned long long prev = atomicCAS(&table[h], 0ull, key);
+        if (prev == 0ull) {
+          rep[h] = (int32_t)i;
+          if (atomicAdd(count, 1) >= NNPP_STENCIL_MAX) atomicExch(fail, 1);
+          break;
+        }
+        if (prev == key) break;
+      }
+    }
+    if (probe == 32) {
+      atomicExch(fail, 1);
+      return;
+    }
+  }
+}
+
+// Weight of each offset from one bond that has it (the same arithmetic as k_pot_fill).
+__global__ void k_st_weights(int K, const int32_t* __restrict__ toff,
+                             const int32_t* __restrict__ rep, const double* __restrict__ x,
+                             const double* __restrict__ y, const double* __restrict__ z,
+                             WeightArgs wa, double* __restrict__ tw64,
+                             float* __restrict__ tw32) {
+  const int t = threadIdx.x;
+  if (t >= K) return;
+  const int64_t i = rep[t];
+  const double w = bond_weight(wa, x, y, z, i, (int32_t)(i + toff[t]));
+  tw64[t] = w;
+  tw32[t] = (float)w;
+}
+
+// Row masks of all bonds; fails if an entry's offset is not in the table, the entries of a
+// row are not in ascending order, or a bond's weight differs (in any bit) from its offset's.
+__global__ void k_st_mask(int32_t n, int K, const int64_t* __restrict__ row_ptr,
+                          const int32_t* __restrict__ col, const int32_t* __restrict__ toff,
+                          const double* __restrict__ tw64, const double* __restrict__ x,
+                          const double* __restrict__ y, const double* __restrict__ z,
+                          WeightArgs wa, uint4* __restrict__ bmask, int* __restrict__ fail) {
+  __shared__ int32_t t_off[NNPP_STENCIL_MAX];
+  __shared__ double t_w[NNPP_STENCIL_MAX];
+  for (int t = threadIdx.x; t < K; t += blockDim.x) {
+    t_off[t] = toff[t];
+    t_w[t] = tw64[t];
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t m[4] = {0u, 0u, 0u, 0u};
+  int prev = -1, bad = 0;
+  for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+    const int32_t j = col[b];
+    const int32_t o = j - (int32_t)i;
+    int lo = 0, hi = K - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (t_off[mid] < o) lo = mid + 1;
+      else hi = mid;
+    }
+    if (t_off[lo] != o || lo <= prev) {
+      bad = 1;
+      break;
+    }
+    const double w = bond_weight(wa, x, y, z, i, j);
+    if (__double_as_longlong(w) != __double_as_longlong(t_w[lo])) {
+      bad = 1;
+      break;
+    }
+    m[lo >> 5] |= 1u << (lo & 31);
+    prev = lo;
+  }
+  if (bad) atomicExch(fail, 1);
+  bmask[i] = make_uint4(m[0], m[1], m[2], m[3]);
+}
+
+// Stencil form of k_pot_flags/k_pot_len/k_pot_fill: entries are the bonds of the row mask
+// whose far end is free, in ascending offset order (the neighbour-list order), so the diagonal
+// sums match the column form bit for bit. Counts entries and free rows per block.
+__global__ __launch_bounds__(NNPP_BLOCK) void k_pot_st_fill(
+    int32_t n, int S, int K, const int32_t* __restrict__ toff, const double* __restrict__ tw64,
+    const uint4* __restrict__ bmask, const uint8_t* __restrict__ phase,
+    const uint8_t* __restrict__ pfree, const uint16_t* __restrict__ bcmask,
+    const double* __restrict__ bcphi, uint4* __restrict__ pmask, double* __restrict__ pdd,
+    float* __restrict__ pdd32, double* __restrict__ pinv, float* __restrict__ pinv32,
+    double* __restrict__ pbdir, double* __restrict__ psc, double* __restrict__ part) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK) {
+    uint32_t pm[4] = {0u, 0u, 0u, 0u};
+    double diag = 0.0, ddir = 0.0, bdir = 0.0;
+    int cnt = 0;
+    const bool fr = pfree[i] != 0;
+    if (fr) {
+      const uint4 bm = bmask[i];
+      const uint32_t words[4] = {bm.x, bm.y, bm.z, bm.w};
+      for (int t = 0; t < K; t++) {
+        if (!((words[t >> 5] >> (t & 31)) & 1u)) continue;
+        const int32_t j = (int32_t)i + toff[t];
+        if (phase[j] != 0) continue;
+        const double w = tw64[t];
+        diag += w;
+        if (pfree[j]) {
+          pm[t >> 5] |= 1u << (t & 31);
+          cnt++;
+        } else {  // a liquid neighbour of a free node is free or Dirichlet
+          ddir += w;
+          if ((bcmask[j] >> S) & 1u) bdir += w * bcphi[j];
+        }
+      }
+    }
+    pmask[i] = make_uint4(pm[0], pm[1], pm[2], pm[3]);
+    const double sc = 1.0;  // constant node volume (the stencil form requires it)
+    pdd[i] = sc * ddir;
+    pdd32[i] = (float)(sc * ddir);
+    pinv[i] = fr ? 1.0 / (sc * diag) : 0.0;
+    pinv32[i] = fr ? (float)(1.0 / (sc * diag)) : 0.0f;
+    pbdir[i] = bdir;
+    psc[i] = sc;
+    v[0] += (double)cnt;
+    v[1] += fr ? 1.0 : 0.0;
+  }
+  nnpp_block_partial<3>(v, part);
+}
+
+// Lane order of each 256-row tile of the stencil SpMV: rows whose mask holds every entry first,
+// then the other rows with entries, then rows without any (and rows past n), each group in
+// row order. Waves made of complete rows then run without per-entry mask tests.
+__global__ __launch_bounds__(NNPP_BLOCK) void k_st_perm(int32_t n, int K,
+                                                        const uint4* __restrict__ pmask,
+                                                        uint8_t* __restrict__ perm) {
+  __shared__ int wcount[3][NNPP_BLOCK / NNPP_WAVE];
+  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
+  int cls = 2;
+  if (i < n) {
+    const uint4 m = pmask[i];
+    const int cnt = __popc(m.x) + __popc(m.y) + __popc(m.z) + __popc(m.w);
+    cls = cnt == K ? 0 : (cnt ? 1 : 2);
+  }
+  const int lane = threadIdx.x % NNPP_WAVE, wid = threadIdx.x / NNPP_WAVE;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (NNPP_WAVE - lane));
+  int rank = 0;
+  for (int c = 0; c < 3; c++) {
+    const unsigned long long b = __ballot(cls == c);
+    if (cls == c) rank = __popcll(b & below);
+    if (lane == 0) wcount[c][wid] = __popcll(b);
+  }
+  __syncthreads();
+  int pos = 0;
+  for (int c = 0; c < 3; c++)
+    for (int w = 0; w < NNPP_BLOCK / NNPP_WAVE; w++)
+      if (c < cls || (c == cls && w < wid)) pos += wcount[c][w];
+  perm[(int64_t)blockIdx.x * NNPP_BLOCK + pos + rank] = (uint8_t)threadIdx.x;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host launchers ---------
@@ -552,11 +727,167 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
   return NNPP_OK;
 }
 
+void nnpp_free_stencil(nnpp_stencil& t) {
+  void* ps[] = {t.toff, t.lc, t.run_start, t.run_len, t.run_base, t.tw64, t.tw32, t.bmask,
+                t.pmask, t.perm};
+  for (void* q : ps)
+    if (q) hipFree(q);
+  t = nnpp_stencil();
+}
+
+// Detect the stencil form of the potential operator for the current geometry (sets
+// c->pst.state to 1 or -1).
+static int build_stencil(nnpp_ctx* c) {
+  nnpp_stencil& T = c->pst;
+  const int32_t n = c->n;
+  hipStream_t s = c->stream;
+  nnpp_free_stencil(T);
+  T.state = -1;  // until shown otherwise
+  if (n == 0 || c->nnz <= 0 || c->vol) return NNPP_OK;
+  unsigned long long* table = nullptr;
+  int32_t* rep = nullptr;
+  int* flags = nullptr;  // [0] count, [1] fail
+  NNPP_HIP(c, hipMalloc((void**)&table, sizeof(unsigned long long) * DICT_HASH));
+  NNPP_HIP(c, hipMalloc((void**)&rep, sizeof(int32_t) * DICT_HASH));
+  NNPP_HIP(c, hipMalloc((void**)&flags, sizeof(int) * 2));
+  NNPP_HIP(c, hipMemsetAsync(table, 0, sizeof(unsigned long long) * DICT_HASH, s));
+  NNPP_HIP(c, hipMemsetAsync(flags, 0, sizeof(int) * 2, s));
+  k_st_collect<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->row_ptr, c->col, table, rep, flags,
+                                                       flags + 1);
+  NNPP_HIP(c, hipGetLastError());
+  std::vector<unsigned long long> ht(DICT_HASH);
+  std::vector<int32_t> hr(DICT_HASH);
+  int hf[2] = {0, 0};
+  NNPP_HIP(c, hipMemcpyAsync(ht.data(), table, sizeof(unsigned long long) * DICT_HASH,
+                             hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(hr.data(), rep, sizeof(int32_t) * DICT_HASH, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(hf, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  hipFree(table);
+  hipFree(rep);
+  if (hf[1] || hf[0] > NNPP_STENCIL_MAX || hf[0] == 0) {
+    hipFree(flags);
+    return NNPP_OK;
+  }
+  std::vector<std::pair<int32_t, int32_t>> offs;  // (offset, representative row)
+  for (int h = 0; h < DICT_HASH; h++)
+    if (ht[h]) offs.push_back({(int32_t)(uint32_t)ht[h], hr[h]});
+  std::sort(offs.begin(), offs.end());
+  const int K = (int)offs.size();
+  std::vector<int32_t> toff(K), reps(K), lc(K), rs, rl, rb;
+  for (int t = 0; t < K; t++) {
+    toff[t] = offs[t].first;
+    reps[t] = offs[t].second;
+  }
+  int base = 0;
+  for (int t = 0; t < K;) {  // runs of consecutive offsets
+    int u = t + 1;
+    while (u < K && toff[u] == toff[u - 1] + 1) u++;
+    rs.push_back(toff[t]);
+    rl.push_back(u - t);
+    rb.push_back(base);
+    for (int q = t; q < u; q++) lc[q] = base + (toff[q] - toff[t]);
+    base += NNPP_BLOCK + (u - t) - 1;
+    t = u;
+  }
+  const int R = (int)rs.size();
+  // the FP64 tile (plus the kernels' few static LDS words) must fit one workgroup's LDS; it
+  // also holds the tile's 256 results at the end
+  int max_lds = 65536;
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+      hipSuccess)
+    max_lds = 65536;
+  if (base < NNPP_BLOCK || (size_t)base * sizeof(double) + 1024 > (size_t)max_lds) {
+    hipFree(flags);
+    return NNPP_OK;
+  }
+  T.K = K;
+  T.R = R;
+  T.lds_elems = base;
+  // the SpMV reads lc and the weights in chunks of 8 consecutive entries: both tables are padded
+  // to a multiple of 8 (LDS position 0, weight 0)
+  const int Kp = (K + 7) & ~7;
+  lc.resize(Kp, 0);
+  int32_t* reps_d = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&T.toff, sizeof(int32_t) * K));
+  NNPP_HIP(c, hipMalloc((void**)&T.lc, sizeof(int32_t) * Kp));
+  NNPP_HIP(c, hipMalloc((void**)&reps_d, sizeof(int32_t) * K));
+  NNPP_HIP(c, hipMalloc((void**)&T.run_start, sizeof(int32_t) * R));
+  NNPP_HIP(c, hipMalloc((void**)&T.run_len, sizeof(int32_t) * R));
+  NNPP_HIP(c, hipMalloc((void**)&T.run_base, sizeof(int32_t) * R));
+  NNPP_HIP(c, hipMalloc((void**)&T.tw64, sizeof(double) * Kp));
+  NNPP_HIP(c, hipMalloc((void**)&T.tw32, sizeof(float) * Kp));
+  NNPP_HIP(c, hipMemsetAsync(T.tw64, 0, sizeof(double) * Kp, s));
+  NNPP_HIP(c, hipMemsetAsync(T.tw32, 0, sizeof(float) * Kp, s));
+  NNPP_HIP(c, hipMalloc((void**)&T.bmask, sizeof(uint4) * n));
+  NNPP_HIP(c, hipMalloc((void**)&T.pmask, sizeof(uint4) * n));
+  NNPP_HIP(c, hipMalloc((void**)&T.perm, (size_t)nnpp_nblocks(n) * NNPP_BLOCK));
+  NNPP_HIP(c, hipMemcpyAsync(T.toff, toff.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.lc, lc.data(), sizeof(int32_t) * Kp, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(reps_d, reps.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.run_start, rs.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.run_len, rl.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.run_base, rb.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+  const WeightArgs wa = weight_args(c, 3.0 + 2.0 * c->p.s_phi);
+  k_st_weights<<<1, NNPP_STENCIL_MAX, 0, s>>>(K, T.toff, reps_d, c->x, c->y, c->z, wa, T.tw64,
+                                              T.tw32);
+  NNPP_HIP(c, hipMemsetAsync(flags, 0, sizeof(int) * 2, s));
+  k_st_mask<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, K, c->row_ptr, c->col, T.toff, T.tw64,
+                                                    c->x, c->y, c->z, wa, T.bmask, flags + 1);
+  NNPP_HIP(c, hipGetLastError());
+  NNPP_HIP(c, hipMemcpyAsync(hf, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  hipFree(reps_d);
+  hipFree(flags);
+  if (hf[1]) {
+    nnpp_free_stencil(T);
+    T.state = -1;
+    return NNPP_OK;
+  }
+  T.state = 1;
+  return NNPP_OK;
+}
+
+// Stencil form of the potential operator for the current phase.
+static int build_potential_stencil(nnpp_ctx* c) {
+  const int32_t n = c->n;
+  const int S = c->p.n_species;
+  hipStream_t s = c->stream;
+  nnpp_stencil& T = c->pst;
+  const int64_t g = nnpp_red_grid(nnpp_nblocks(n));
+  k_pot_flags<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, S, c->row_ptr, c->col, c->phase,
+                                                     c->bcmask, c->pfree);
+  k_pot_st_fill<<<g, NNPP_BLOCK, 0, s>>>(n, S, T.K, T.toff, T.tw64, T.bmask, c->phase, c->pfree,
+                                         c->bcmask, c->bcval + (int64_t)S * n, T.pmask, c->pdd,
+                                         c->pdd32, c->pinv, c->pinv32, c->pbdir, c->psc,
+                                         c->red_part);
+  NNPP_HIP(c, hipGetLastError());
+  k_st_perm<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, T.K, T.pmask, T.perm);
+  NNPP_HIP(c, hipGetLastError());
+  int st = nnpp_finalize(c, FIN_FRONT, g, 0.0);
+  if (st) return st;
+  NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  c->pot.nnz = (int64_t)c->h_scal[SC_R0];
+  c->pot_nfree = (int64_t)c->h_scal[SC_R1];
+  T.active = 1;
+  c->pot_dirty = false;
+  return NNPP_OK;
+}
+
 int nnpp_build_potential(nnpp_ctx* c) {
   const int32_t n = c->n;
   const int S = c->p.n_species;
   const int64_t ns = (n + NNPP_WAVE - 1) / NNPP_WAVE;
   hipStream_t s = c->stream;
+  c->pst.active = 0;
+  if (n && !(c->format & NNPP_FORMAT_NO_STENCIL) && !c->vol) {
+    if (c->pst.state == 0) {
+      const int st = build_stencil(c);
+      if (st) return st;
+    }
+    if (c->pst.state == 1) return build_potential_stencil(c);
+  }
   int st = slices_alloc(c, c->pot, ns);
   if (st) return st;
   int32_t* rcount = nullptr;

@@ -189,6 +189,7 @@ int nnpp_build_neighbors_dev(nnpp_ctx* c, double r, double cdx, double cdy, doub
     if (nnz_out) *nnz_out = 0;
     if (max_len_out) *max_len_out = 0;
     c->geom_dirty = c->pot_dirty = true;
+    c->pst.state = 0;
     return NNPP_OK;
   }
   // Step 0 (grid.h:59-71): bounding box of the coordinates; min/max are order independent,
@@ -322,6 +323,7 @@ int nnpp_build_neighbors_dev(nnpp_ctx* c, double r, double cdx, double cdy, doub
     if (nnz_out) *nnz_out = nnz;
     if (max_len_out) *max_len_out = longest + 1;
     c->geom_dirty = c->pot_dirty = true;
+    c->pst.state = 0;
   }
 cleanup:
   hipStreamSynchronize(s);

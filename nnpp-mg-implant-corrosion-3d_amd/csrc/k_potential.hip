@@ -23,6 +23,8 @@
 // blocks (nnpp_blocks_of), so the last-block sum handles at most 2048 partials.
 #include <cmath>
 
+#include <cstring>
+
 #include "nnpp_internal.hpp"
 
 namespace {
@@ -38,7 +40,36 @@ struct Slices {
   const uint8_t* cls;   // weight classes (dictionary variant)
   const double* tab64;  // dictionary, FP64 / FP32
   const float* tab32;
+  // stencil variant
+  const uint4* pmask;  // bit t: entry t of the row present
+  const int32_t* lc;   // LDS position of entry t relative to the lane (padded to 8 entries)
+  const int32_t* run_start;
+  const int32_t* run_len;
+  const int32_t* run_base;
+  const double* tw64;  // weight of entry t (padded with 0 to 8 entries)
+  const float* tw32;
+  const uint8_t* perm;  // row of each lane within its tile
+  int K, R;
 };
+
+// Load from a table that no kernel writes while it reads it, through the constant address
+// space: with a wave-uniform index it becomes a scalar load (batched by the compiler) instead of
+// a vector load of one address.
+template <typename T>
+__device__ __forceinline__ T ld_table(const T* p, int i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ const T* stencil_weights(const Slices& sl);
+template <>
+__device__ __forceinline__ const double* stencil_weights<double>(const Slices& sl) {
+  return sl.tw64;
+}
+template <>
+__device__ __forceinline__ const float* stencil_weights<float>(const Slices& sl) {
+  return sl.tw32;
+}
 
 __device__ __forceinline__ bool cg_done(const double* scal) {
   return nnpp_ld_scalar(scal + SC_DONE) != 0.0;
@@ -121,25 +152,31 @@ __device__ __forceinline__ void load_tab(T* lds, const T* __restrict__ tab) {
   __syncthreads();
 }
 
-__device__ __forceinline__ void kahan(float& s, float& cmp, float v) {
-  const float y = v - cmp;
-  const float t = s + y;
-  cmp = (t - s) - y;
-  s = t;
-}
-
-// Per-row accumulator: plain FP64 sum, or Kahan-compensated FP32 sum.
+// Per-row accumulator of the terms w (v_i - v_j): plain FP64 sum, or Kahan-compensated FP32
+// sum. The fused multiply-adds are written out, so every storage form of the operator (and its
+// row loop) rounds the same way whatever the compiler would contract on its own.
 template <typename T>
 struct RowAcc;
 template <>
 struct RowAcc<double> {
   double s = 0.0;
-  __device__ __forceinline__ void add(double x) { s += x; }
+  __device__ __forceinline__ void add(double w, double d) { s = __fma_rn(w, d, s); }
+  __device__ __forceinline__ double result(double dd, double vi) const {
+    return __fma_rn(dd, vi, s);
+  }
 };
 template <>
 struct RowAcc<float> {
   float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float x) { kahan(s, c, x); }
+  __device__ __forceinline__ void add(float w, float d) {
+    const float y = __fmaf_rn(w, d, -c);
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ float result(float dd, float vi) const {
+    return __fmaf_rn(dd, vi, s);
+  }
 };
 
 // Quads per batch: with stored weights (6-12 bytes per entry) deep batches keep enough bytes in
@@ -187,10 +224,103 @@ __device__ __forceinline__ T spmv_row(const Slices& sl, const T* __restrict__ w,
       T wq[4];
       q[b].weights(tab, wq);
 #pragma unroll
-      for (int e = 0; e < 4; e++) acc.add(wq[e] * (vi - pv[b][e]));
+      for (int e = 0; e < 4; e++) acc.add(wq[e], vi - pv[b][e]);
     }
   }
-  return dd[i] * vi + acc.s;
+  return acc.result(dd[i], vi);
+}
+
+// Stencil variant: one 256-row tile. Every run of consecutive offsets needs a contiguous range
+// of v (256 + run length - 1 values), staged into LDS with coalesced loads; lane l then reads
+// entry t at lds[l + lc[t]]. Entries go in chunks of 8: the chunk's LDS positions and weights
+// are wave-uniform (scalar loads), its 8 LDS reads are issued together, and an absent entry
+// takes weight +0 (its bits cleared), which leaves the sum unchanged, so no lane branches.
+// FP64: one fused multiply-add chain in ascending offset order, the additions of the column
+// form (bit for bit). FP32: a fused multiply-add chain per chunk, the chunk sums
+// Kahan-compensated. The leading barrier orders the staging after the previous tile's reads;
+// tiles without any entry skip the staging. All threads of the block call this.
+template <typename T>
+__device__ __forceinline__ T mask_weight(T w, int keep);
+template <>
+__device__ __forceinline__ float mask_weight<float>(float w, int keep) {
+  return __int_as_float(__float_as_int(w) & keep);
+}
+template <>
+__device__ __forceinline__ double mask_weight<double>(double w, int keep) {
+  return __longlong_as_double(__double_as_longlong(w) & (long long)keep);
+}
+
+template <typename T>
+__device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict__ dd,
+                                          const T* __restrict__ v, int32_t n, int64_t tile,
+                                          T* lds) {
+  const int64_t r0 = tile * NNPP_BLOCK;
+  const int l = sl.perm[r0 + threadIdx.x];  // this lane's row within the tile
+  const int64_t i = r0 + l;
+  const bool in = i < n;
+  const uint4 m = in ? sl.pmask[i] : make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t any = m.x | m.y | m.z | m.w;
+  const T vi = in ? v[i] : T(0);
+  if (__syncthreads_or(any != 0u)) {
+    for (int r = 0; r < sl.R; r++) {
+      const int64_t g0 = r0 + ld_table(sl.run_start, r);
+      const int len = NNPP_BLOCK + ld_table(sl.run_len, r) - 1;
+      T* dst = lds + ld_table(sl.run_base, r);
+      for (int u = threadIdx.x; u < len; u += NNPP_BLOCK) {
+        const int64_t g = g0 + u;
+        dst[u] = (g >= 0 && g < n) ? v[g] : T(0);
+      }
+    }
+    __syncthreads();
+  }
+  constexpr int BT = 8;
+  const T* __restrict__ tw = stencil_weights<T>(sl);
+  const T* row = lds + l;
+  RowAcc<T> acc;
+  const int K = sl.K;
+  if (__any(any != 0u)) {
+    for (int t0 = 0; t0 < K; t0 += BT) {  // a chunk never straddles a 32-bit mask word
+      const int wd = t0 >> 5;
+      const uint32_t bits =
+          (wd == 0 ? m.x : (wd == 1 ? m.y : (wd == 2 ? m.z : m.w))) >> (t0 & 31);
+      const uint32_t valid = K - t0 >= BT ? 0xffu : (1u << (K - t0)) - 1u;
+      if (!__any(bits & valid)) continue;  // no lane of the wave has an entry in this chunk
+      const bool full = __all((bits & valid) == valid);
+      int lv[BT];
+      T wv[BT], pv[BT];
+#pragma unroll
+      for (int k = 0; k < BT; k++) {  // tables padded to a multiple of BT entries (weight 0)
+        lv[k] = ld_table(sl.lc, t0 + k);
+        wv[k] = ld_table(tw, t0 + k);
+      }
+#pragma unroll
+      for (int k = 0; k < BT; k++) pv[k] = row[lv[k]];
+      T part = T(0);
+      if (full) {  // every lane has every entry of the chunk
+#pragma unroll
+        for (int k = 0; k < BT; k++) {
+          if constexpr (sizeof(T) == 8) acc.add(wv[k], vi - pv[k]);
+          else part = __fmaf_rn(wv[k], vi - pv[k], part);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < BT; k++) {
+          const int keep = (int)(bits << (31 - k)) >> 31;  // bit k -> all ones, else 0
+          const T wk = mask_weight<T>(wv[k], keep);
+          if constexpr (sizeof(T) == 8) acc.add(wk, vi - pv[k]);
+          else part = __fmaf_rn(wk, vi - pv[k], part);
+        }
+      }
+      if constexpr (sizeof(T) == 4) acc.add(T(1), part);
+    }
+  }
+  // hand the results back in row order (lane j of the caller owns row r0 + j), through the
+  // start of the staging area once every lane is done reading it
+  const T y = in ? acc.result(dd[i], vi) : T(0);
+  __syncthreads();
+  lds[l] = y;
+  __syncthreads();
+  return lds[threadIdx.x];
 }
 
 // b (free rows), warm start x = phi, ||b||^2.
@@ -228,7 +358,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_rhs(int32_t n, int S, const doub
 
 // r = b - M x in FP64 (RR = ||r||^2). START: also z = D^-1 r, p = z, rho, the target from
 // ||b|| and the converged flag.
-template <bool START, bool WIDE, bool CLS>
+template <bool START, bool WIDE, bool CLS, bool STEN>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
                                                       const double* __restrict__ w,
                                                       const double* __restrict__ dd,
@@ -239,14 +369,18 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
                                                       double* __restrict__ pv,
                                                       double* __restrict__ part) {
   __shared__ double tab[CLS ? NNPP_DICT_MAX : 1];
+  extern __shared__ double st_lds[];
   if (CLS) load_tab(tab, sl.tab64);
   double v[2] = {0.0, 0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    double mx = 0.0;
+    if constexpr (STEN) mx = stencil_tile<double>(sl, dd, x, n, it.cur, st_lds);
     if (i >= n) continue;
-    const double ri =
-        (pinv[i] != 0.0) ? b[i] - spmv_row<double, WIDE, CLS>(sl, w, tab, dd, x, i) : 0.0;
+    if constexpr (!STEN)
+      if (pinv[i] != 0.0) mx = spmv_row<double, WIDE, CLS>(sl, w, tab, dd, x, i);
+    const double ri = (pinv[i] != 0.0) ? b[i] - mx : 0.0;
     r[i] = ri;
     v[1] += ri * ri;
     if (START) {
@@ -259,7 +393,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
   nnpp_block_partial<2>(v, part);
 }
 
-template <bool WIDE, bool CLS>
+template <bool WIDE, bool CLS, bool STEN>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
                                                         const double* __restrict__ w,
                                                         const double* __restrict__ dd,
@@ -267,13 +401,16 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
                                                         double* __restrict__ q, double* __restrict__ part, const double* __restrict__ scal) {
   if (cg_done(scal)) return;
   __shared__ double tab[CLS ? NNPP_DICT_MAX : 1];
+  extern __shared__ double st_lds[];
   if (CLS) load_tab(tab, sl.tab64);
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    double qi = 0.0;
+    if constexpr (STEN) qi = stencil_tile<double>(sl, dd, p, n, it.cur, st_lds);
     if (i >= n) continue;
-    const double qi = spmv_row<double, WIDE, CLS>(sl, w, tab, dd, p, i);
+    if constexpr (!STEN) qi = spmv_row<double, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
     v[0] += p[i] * qi;
   }
@@ -339,7 +476,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_replace(int32_t n, const doubl
   nnpp_block_partial<2>(v, part);
 }
 
-template <bool WIDE, bool CLS>
+template <bool WIDE, bool CLS, bool STEN>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
                                                        const float* __restrict__ w,
                                                        const float* __restrict__ dd,
@@ -347,13 +484,16 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
                                                        float* __restrict__ q, double* __restrict__ part, const double* __restrict__ scal) {
   if (cg_done(scal)) return;
   __shared__ float tab[CLS ? NNPP_DICT_MAX : 1];
+  extern __shared__ float st_lds32[];
   if (CLS) load_tab(tab, sl.tab32);
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    float qi = 0.0f;
+    if constexpr (STEN) qi = stencil_tile<float>(sl, dd, p, n, it.cur, st_lds32);
     if (i >= n) continue;
-    const float qi = spmv_row<float, WIDE, CLS>(sl, w, tab, dd, p, i);
+    if constexpr (!STEN) qi = spmv_row<float, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
     v[0] += (double)p[i] * (double)qi;
   }
@@ -419,35 +559,69 @@ __global__ void k_scatter_phi(int32_t n, int S, const uint8_t* __restrict__ pfre
   else if (bcphi && ((bcmask[i] >> S) & 1u)) phi[i] = bcphi[i];
 }
 
-// variant = 2 * wide + dictionary
+// variant: 2 * wide + dictionary (column form), 4 = stencil form (lds bytes of dynamic LDS)
 template <bool START, typename... A>
-void launch_resid(int var, int64_t g, hipStream_t s, A... a) {
+void launch_resid(int var, size_t lds, int64_t g, hipStream_t s, A... a) {
   switch (var) {
-    case 0: k_resid<START, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    case 1: k_resid<START, false, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    case 2: k_resid<START, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    default: k_resid<START, true, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 0: k_resid<START, false, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 1: k_resid<START, false, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 2: k_resid<START, true, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 3: k_resid<START, true, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    default: k_resid<START, false, false, true><<<g, NNPP_BLOCK, lds, s>>>(a...); break;
   }
 }
 
 template <typename... A>
-void launch_cg_spmv(int var, int64_t g, hipStream_t s, A... a) {
+void launch_cg_spmv(int var, size_t lds, int64_t g, hipStream_t s, A... a) {
   switch (var) {
-    case 0: k_cg_spmv<false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    case 1: k_cg_spmv<false, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    case 2: k_cg_spmv<true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    default: k_cg_spmv<true, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 0: k_cg_spmv<false, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 1: k_cg_spmv<false, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 2: k_cg_spmv<true, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 3: k_cg_spmv<true, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    default: k_cg_spmv<false, false, true><<<g, NNPP_BLOCK, lds, s>>>(a...); break;
   }
 }
 
 template <typename... A>
-void launch_m_spmv(int var, int64_t g, hipStream_t s, A... a) {
+void launch_m_spmv(int var, size_t lds, int64_t g, hipStream_t s, A... a) {
   switch (var) {
-    case 0: k_m_spmv<false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    case 1: k_m_spmv<false, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    case 2: k_m_spmv<true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    default: k_m_spmv<true, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 0: k_m_spmv<false, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 1: k_m_spmv<false, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 2: k_m_spmv<true, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 3: k_m_spmv<true, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    default: k_m_spmv<false, false, true><<<g, NNPP_BLOCK, lds, s>>>(a...); break;
   }
+}
+
+// Operator view for the SpMV kernels: variant, slices/stencil tables, and the weight arrays.
+struct PotOp {
+  Slices sl;
+  int var;
+  size_t lds32, lds64;
+  const double* w64;
+  const float* w32;
+};
+
+PotOp pot_op(const nnpp_ctx* c) {
+  PotOp o;
+  std::memset(&o, 0, sizeof(o));
+  const nnpp_slices& P = c->pot;
+  const nnpp_stencil& T = c->pst;
+  o.sl = Slices{P.off, P.len, P.col, P.d16, P.first, P.cls, P.tab64, P.tab32,
+                T.pmask, T.lc, T.run_start, T.run_len, T.run_base, T.tw64, T.tw32, T.perm,
+                T.K, T.R};
+  if (T.active) {
+    o.var = 4;
+    o.lds32 = sizeof(float) * T.lds_elems;
+    o.lds64 = sizeof(double) * T.lds_elems;
+    o.w64 = T.tw64;
+    o.w32 = T.tw32;
+  } else {
+    o.var = 2 * (P.wide != 0) + (P.ncls > 0);
+    o.w64 = P.w64;
+    o.w32 = P.w32;
+  }
+  return o;
 }
 
 }  // namespace
@@ -474,9 +648,9 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   if (!n) return NNPP_OK;
   const int64_t nb = nnpp_nblocks(n);
   const int64_t g = nnpp_red_grid(nb);  // grid of every kernel with a reduction
-  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first,
-            c->pot.cls, c->pot.tab64, c->pot.tab32};
-  const int var = 2 * (c->pot.wide != 0) + (c->pot.ncls > 0);  // kernel variant
+  const PotOp op = pot_op(c);
+  const Slices& sl = op.sl;
+  const int var = op.var;
   double* part = c->red_part;
   const double* scal = c->scal;
   double zs[NNPP_MAX_SPECIES] = {0};
@@ -503,10 +677,10 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   // r = b - M x in FP64 (START: also z, p, rho, target, converged flag)
   auto resid = [&](bool start) -> int {
     if (start)
-      launch_resid<true>(var, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+      launch_resid<true>(var, op.lds64, g, s, n, sl, op.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
                          c->cg_r, c->cg_z, c->cg_p, part);
     else
-      launch_resid<false>(var, g, s, n, sl, c->pot.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+      launch_resid<false>(var, op.lds64, g, s, n, sl, op.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
                           c->cg_r, nullptr, nullptr, part);
     NNPP_HIP(c, hipGetLastError());
     n64++;
@@ -527,7 +701,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       }
       const int batch = left < CG_BATCH ? left : CG_BATCH;
       for (int it = 0; it < batch; it++) {
-        launch_cg_spmv(var, g, s, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, part, scal);
+        launch_cg_spmv(var, op.lds64, g, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, part, scal);
         if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
         k_cg_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
                                              c->cg_z, part, scal);
@@ -562,7 +736,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         }
         const int batch = left < CG_BATCH ? left : CG_BATCH;
         for (int it = 0; it < batch; it++) {
-          launch_m_spmv(var, g, s, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, part, scal);
+          launch_m_spmv(var, op.lds32, g, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, part, scal);
           if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
           k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
                                               c->f_z, part, scal);
@@ -612,14 +786,14 @@ int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const int32_t n = c->n;
   if (!n) return NNPP_OK;
   const int64_t g = nnpp_red_grid(nnpp_nblocks(n));
-  Slices sl{c->pot.off, c->pot.len, c->pot.col, c->pot.d16, c->pot.first,
-            c->pot.cls, c->pot.tab64, c->pot.tab32};
-  const int var = 2 * (c->pot.wide != 0) + (c->pot.ncls > 0);  // kernel variant
+  const PotOp op = pot_op(c);
+  const Slices& sl = op.sl;
+  const int var = op.var;
   if (precision == NNPP_MIXED)
-    launch_m_spmv(var, g, c->stream, n, sl, c->pot.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
+    launch_m_spmv(var, op.lds32, g, c->stream, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
                   (const double*)c->scal);
   else
-    launch_cg_spmv(var, g, c->stream, n, sl, c->pot.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
+    launch_cg_spmv(var, op.lds64, g, c->stream, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
                    (const double*)c->scal);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;

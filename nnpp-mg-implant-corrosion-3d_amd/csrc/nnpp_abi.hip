@@ -49,6 +49,7 @@ void free_nodes(nnpp_ctx* c) {
   c->nnz = -1;
   free_slices(c->bonds);
   free_slices(c->pot);
+  nnpp_free_stencil(c->pst);
   dfree(c->phase);
   dfree(c->bcmask);
   dfree(c->bcval);
@@ -209,7 +210,10 @@ int nnpp_set_params(nnpp_ctx* c, const nnpp_params* p) {
   c->p = *p;
   c->have_params = true;
   c->w_exponent = 3.0 + 2.0 * s0;
-  if (changed_geom) c->geom_dirty = c->pot_dirty = true;
+  if (changed_geom) {
+    c->geom_dirty = c->pot_dirty = true;
+    c->pst.state = 0;
+  }
   if (changed_bc) c->have_bc = false;
   return NNPP_OK;
 }
@@ -268,6 +272,7 @@ int nnpp_set_nodes(nnpp_ctx* c, int32_t n, const double* x, const double* y, con
   NNPP_HIP(c, hipStreamSynchronize(s));
   c->have_bc = false;
   c->geom_dirty = c->pot_dirty = true;
+  c->pst.state = 0;
   c->step = 1;
   c->voln1rel = 0.0;
   c->vol0rel = 0.0;
@@ -326,6 +331,7 @@ int nnpp_set_neighbors(nnpp_ctx* c, const int64_t* row_ptr, const int32_t* col) 
   c->nnz = nnz;
   c->max_len = longest + 1;
   c->geom_dirty = c->pot_dirty = true;
+  c->pst.state = 0;
   return NNPP_OK;
 }
 
@@ -512,7 +518,9 @@ int nnpp_set_solver(nnpp_ctx* c, double rtol, int32_t maxit, double inner_rtol) 
 }
 
 int nnpp_set_format(nnpp_ctx* c, int32_t flags) {
-  if (!c || (flags & ~(NNPP_FORMAT_WIDE_COLUMNS | NNPP_FORMAT_NO_DICTIONARY))) return NNPP_ERR_ARG;
+  if (!c || (flags & ~(NNPP_FORMAT_WIDE_COLUMNS | NNPP_FORMAT_NO_DICTIONARY |
+                       NNPP_FORMAT_NO_STENCIL)))
+    return NNPP_ERR_ARG;
   if (flags != c->format) c->geom_dirty = c->pot_dirty = true;
   c->format = flags;
   return NNPP_OK;
@@ -587,8 +595,9 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
   loc.flipped += flips;
   loc.nnz_flux = c->bonds.nnz;
   loc.nnz_ll = c->pot.nnz;
-  loc.pot_col_bytes = c->pot.wide ? 4 : 2;
-  loc.pot_weight_bytes = c->pot.ncls > 0 ? 1 : 4;
+  loc.pot_stencil = c->pst.active;
+  loc.pot_col_bytes = c->pst.active ? 0 : (c->pot.wide ? 4 : 2);
+  loc.pot_weight_bytes = c->pst.active ? 0 : (c->pot.ncls > 0 ? 1 : 4);
   return status;
 }
 
@@ -679,6 +688,37 @@ int nnpp_state_device_ptr(nnpp_ctx* c, int32_t slot, int32_t field, void** dptr)
 int nnpp_stream(nnpp_ctx* c, void** stream) {
   if (!c || !stream) return NNPP_ERR_ARG;
   *stream = (void*)c->stream;
+  return NNPP_OK;
+}
+
+int nnpp_potential_operator_apply(nnpp_ctx* c, int32_t precision, const double* x, double* y) {
+  if (!c || !x || !y) return NNPP_ERR_ARG;
+  if (precision != NNPP_FP64 && precision != NNPP_MIXED) return NNPP_ERR_ARG;
+  NNPP_HIP(c, hipSetDevice(c->device));
+  int st = ensure_potential(c);
+  if (st) return st;
+  const int32_t n = c->n;
+  if (!n) return NNPP_OK;
+  hipStream_t s = c->stream;
+  const double zero = 0.0;
+  NNPP_HIP(c, hipMemcpyAsync(c->scal + SC_DONE, &zero, sizeof(double), hipMemcpyHostToDevice, s));
+  std::vector<float> xf;
+  if (precision == NNPP_MIXED) {
+    xf.assign(x, x + n);
+    NNPP_HIP(c, hipMemcpyAsync(c->f_p, xf.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
+  } else {
+    NNPP_HIP(c, hipMemcpyAsync(c->cg_p, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  }
+  st = nnpp_pot_spmv_launch(c, precision);
+  if (st) return st;
+  if (precision == NNPP_MIXED) {
+    NNPP_HIP(c, hipMemcpyAsync(xf.data(), c->f_q, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    NNPP_HIP(c, hipStreamSynchronize(s));
+    for (int32_t i = 0; i < n; i++) y[i] = xf[i];
+  } else {
+    NNPP_HIP(c, hipMemcpyAsync(y, c->cg_q, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    NNPP_HIP(c, hipStreamSynchronize(s));
+  }
   return NNPP_OK;
 }
 

@@ -55,6 +55,33 @@ struct nnpp_slices {
 };
 #define NNPP_DICT_MAX 256
 
+// Stencil form of the potential operator, used when every bond (i, j) of the neighbour lists
+// has j - i in a table of at most NNPP_STENCIL_MAX offsets and a weight that depends only on the
+// offset (a lattice numbered in any fixed order, constant node volume). Row i then needs one
+// 128-bit mask (bit t: entry i + toff[t] present) instead of its column and weight streams.
+// The offsets are sorted ascending (the order of the neighbour lists) and grouped into runs of
+// consecutive offsets; a 256-row tile of the SpMV stages, for each run, the contiguous range of
+// the vector its rows read into LDS, and lane l reads entry t at lds[l + lc[t]].
+#define NNPP_STENCIL_MAX 128
+struct nnpp_stencil {
+  int state = 0;               // 0: not built for this geometry, 1: valid, -1: not a stencil
+  int K = 0, R = 0;            // offsets, runs
+  int lds_elems = 0;           // LDS entries per 256-row tile
+  int32_t* toff = nullptr;     // [K] j - i, ascending
+  int32_t* lc = nullptr;       // [K rounded up to 8] LDS position of entry t relative to the
+                               // lane (padding: 0)
+  int32_t* run_start = nullptr;  // [R] first offset of run r
+  int32_t* run_len = nullptr;    // [R]
+  int32_t* run_base = nullptr;   // [R] LDS base of run r
+  double* tw64 = nullptr;      // [K rounded up to 8] potential weight of offset t (FP64 /
+  float* tw32 = nullptr;       // FP32; padding: 0)
+  uint4* bmask = nullptr;      // [n] bonds of each row
+  uint4* pmask = nullptr;      // [n] potential-operator entries of each row
+  uint8_t* perm = nullptr;     // [256 per tile] row of each lane: rows with every entry first,
+                               // then the other rows with entries, then the rest (stable)
+  int active = 0;              // the current potential operator uses the stencil form
+};
+
 // Differentiation tape (nnpp_tape_begin .. nnpp_tape_end).
 struct nnpp_tape_step {  // host record of one recorded step
   int64_t step;          // step counter before the step
@@ -98,6 +125,7 @@ struct nnpp_ctx {
 
   nnpp_slices bonds;       // every bond, weights with the flux exponent
   nnpp_slices pot;         // free-free liquid bonds of the potential operator
+  nnpp_stencil pst;        // stencil form of the potential operator (lattices)
   bool geom_dirty = true;  // bond slices need a rebuild
   bool pot_dirty = true;   // potential operator needs a rebuild (phase/bc changed)
   int32_t format = 0;      // NNPP_FORMAT_* flags
@@ -349,3 +377,4 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
 int nnpp_adjoint_run(nnpp_ctx* c, const double* lam, double* grad, double* lam0,
                      nnpp_adjoint_stats* stats);
 void nnpp_tape_free(nnpp_ctx* c);
+void nnpp_free_stencil(nnpp_stencil& t);

@@ -139,6 +139,13 @@ class Problem:
         self.set_phase(snap["phase"])
         self.set_time(*snap["time"])
 
+    def potential_operator_apply(self, x, precision=FP64):
+        """One application of the solver's (scaled, free-row) potential operator."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        self._check(self.lib.nnpp_potential_operator_apply(self.ctx, precision, dptr(x), dptr(y)))
+        return y
+
     def set_solver(self, rtol=1e-10, maxit=20000, inner_rtol=1e-4):
         self._check(self.lib.nnpp_set_solver(self.ctx, rtol, maxit, inner_rtol))
 

@@ -237,16 +237,20 @@ def test_mixed_step_close_to_fp64():
 
 @pytest.mark.parametrize("kind", ["regular", "screw"])
 def test_storage_formats_bit_identical(kind):
-    """16-bit column steps vs 32-bit columns, dictionary weight classes vs stored weights:
-    the same sums in the same order, so the trajectories agree bit for bit."""
+    """Stencil form (lattices), 16-bit column steps vs 32-bit columns, dictionary weight classes
+    vs stored weights: the same additions in the same order, so the trajectories agree bit for
+    bit."""
     say("case storage formats", kind)
-    # 16^3 regular: spacing 1/16 makes every bond vector exact, so the dictionary applies
+    # 16^3 regular: spacing 1/16 makes every bond vector exact (dictionary and stencil apply);
+    # the screw case has per-node volumes (column form only)
     case = nn.synth.regular(16, S=11, seed=9) if kind == "regular" else nn.synth.screw(16, S=11, seed=9)
+    case.params.front_threshold = 1.0  # flips: the operator is rebuilt on the way
     rp, col, _ = oracle_nb(case)
+    W, D, T = nn.FORMAT_WIDE_COLUMNS, nn.FORMAT_NO_DICTIONARY, nn.FORMAT_NO_STENCIL
+    formats = [nn.FORMAT_AUTO, W, D, W | D, T, T | W, T | D, T | W | D]
     for prec in (nn.FP64, nn.MIXED):
         out = {}
-        for fmt in (nn.FORMAT_AUTO, nn.FORMAT_WIDE_COLUMNS, nn.FORMAT_NO_DICTIONARY,
-                    nn.FORMAT_WIDE_COLUMNS | nn.FORMAT_NO_DICTIONARY):
+        for fmt in formats:
             with problem(case) as pb:
                 pb.set_format(fmt)
                 pb.set_neighbors(rp, col)
@@ -257,18 +261,70 @@ def test_storage_formats_bit_identical(kind):
                 assert st["status"] == 0
                 out[fmt] = (pb.get_state(), st)
         U0, st0 = out[nn.FORMAT_AUTO]
-        assert st0["pot_col_bytes"] == 2
-        if kind == "regular":
-            assert st0["pot_weight_bytes"] == 1
+        Uc = out[T][0]  # column form
+        assert st0["flipped"] > 0
         for fmt, (U, st) in out.items():
-            assert st["pot_col_bytes"] == (4 if fmt & nn.FORMAT_WIDE_COLUMNS else 2)
-            if fmt & nn.FORMAT_NO_DICTIONARY:
-                assert st["pot_weight_bytes"] == 4
-            assert st["cg_iterations"] == st0["cg_iterations"], fmt
-            assert np.array_equal(U, U0), (prec, fmt)
+            stencil = kind == "regular" and not (fmt & T)
+            assert st["pot_stencil"] == int(stencil), fmt
+            if stencil:
+                assert st["pot_col_bytes"] == 0 and st["pot_weight_bytes"] == 0
+            else:
+                assert st["pot_col_bytes"] == (4 if fmt & W else 2)
+                assert st["pot_weight_bytes"] == (4 if (fmt & D or kind == "screw") else 1)
+            if prec == nn.FP64 or not stencil:
+                # FP64: every form performs the same additions; FP32 column forms too
+                assert np.array_equal(U, U0 if prec == nn.FP64 else Uc), (prec, fmt)
+                if prec == nn.FP64:
+                    assert st["cg_iterations"] == st0["cg_iterations"], fmt
+            else:
+                # FP32 stencil rows sum chunk by chunk (compensated across chunks); the FP64
+                # residual replacements bring the solves to the same rtol
+                for f in range(case.params.S + 1):
+                    assert rel_err(U[f], Uc[f]) < 1e-9, (fmt, f)
     with problem(case) as pb:
         with pytest.raises(nn.NnppError):
-            pb.set_format(4)
+            pb.set_format(8)
+
+
+@pytest.mark.parametrize("kind", ["regular", "screw"])
+def test_potential_operator_kernel(kind):
+    """The solver's SpMV kernel alone: FP64 against the oracle's K11 action on the free rows
+    (1e-12), FP32 + Kahan against FP64 (1e-6), and every storage form bit-identical."""
+    say("case potential operator kernel", kind)
+    case = nn.synth.regular(16, S=11, seed=4) if kind == "regular" else nn.synth.screw(16, S=11, seed=4)
+    rp, col, _ = oracle_nb(case)
+    U, ph, _, _ = O.init_state(case)
+    S = case.params.S
+    liquid = ph == 0
+    rows = np.repeat(np.arange(case.n), np.diff(rp))
+    has_ll = np.zeros(case.n, bool)
+    has_ll[rows[liquid[rows] & liquid[col]]] = True
+    dirphi = (case.bctypes == 0) & (case.bcvalues[S] != -1)
+    free = liquid & has_ll & ~dirphi
+    rng = np.random.default_rng(3)
+    x = np.where(free, rng.standard_normal(case.n), 0.0)
+    yo = O.poisson_apply(case, rp, col, ph, x)
+    sc = case.vol / (case.params.dx * case.params.dy * case.params.dz) if case.vol is not None else 1.0
+    yo = np.where(free, yo * sc, 0.0)
+    T = nn.FORMAT_NO_STENCIL
+    out = {}
+    for fmt in (nn.FORMAT_AUTO, T, T | nn.FORMAT_NO_DICTIONARY, T | nn.FORMAT_WIDE_COLUMNS):
+        with problem(case) as pb:
+            pb.set_format(fmt)
+            pb.set_neighbors(rp, col)
+            pb.set_bc(case.bctypes, case.bcvalues)
+            pb.init_state(case.icvalues)
+            out[fmt] = (pb.potential_operator_apply(x, nn.FP64),
+                        pb.potential_operator_apply(x, nn.MIXED))
+    y64, y32 = out[nn.FORMAT_AUTO]
+    assert rel_err(y64, yo) < 1e-12
+    assert np.abs(y32 - y64).max() <= 1e-6 * np.abs(y64).max()
+    yc32 = out[T][1]
+    for fmt, (a, b) in out.items():
+        assert np.array_equal(a, y64), fmt  # FP64: the same additions in every form
+        assert np.array_equal(b, yc32) or fmt == nn.FORMAT_AUTO, fmt  # FP32 column forms
+    # FP32 stencil rows (chunk sums, compensated across chunks) against FP32 column rows
+    assert np.abs(y32 - yc32).max() <= 4e-7 * np.abs(y64).max()
 
 
 def test_deterministic_and_error_paths():
