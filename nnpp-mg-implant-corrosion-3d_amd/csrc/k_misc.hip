@@ -803,6 +803,8 @@ static int build_stencil(nnpp_ctx* c) {
   }
   T.K = K;
   T.R = R;
+  T.off_min = toff[0];
+  T.off_max = toff[K - 1];
   T.lds_elems = base;
   // the SpMV reads lc and the weights in chunks of 8 consecutive entries: both tables are padded
   // to a multiple of 8 (LDS position 0, weight 0)

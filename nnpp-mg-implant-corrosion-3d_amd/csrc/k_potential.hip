@@ -21,6 +21,7 @@
 //
 // Every kernel with a reduction runs on at most NNPP_RED_GRID blocks that stride over the node
 // blocks (nnpp_blocks_of), so the last-block sum handles at most 2048 partials.
+#include <algorithm>
 #include <cmath>
 
 #include <cstring>
@@ -50,6 +51,7 @@ struct Slices {
   const float* tw32;
   const uint8_t* perm;  // row of each lane within its tile
   int K, R;
+  int off_min, off_max;  // smallest and largest offset
 };
 
 // Load from a table that no kernel writes while it reads it, through the constant address
@@ -262,13 +264,22 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
   const uint32_t any = m.x | m.y | m.z | m.w;
   const T vi = in ? v[i] : T(0);
   if (__syncthreads_or(any != 0u)) {
+    // tiles whose every staged index lies in [0, n) need no bounds checks (all but the first
+    // and last few); the run's source is a wave-uniform pointer
+    const bool inner = r0 + sl.off_min >= 0 && r0 + NNPP_BLOCK - 1 + sl.off_max < n;
     for (int r = 0; r < sl.R; r++) {
-      const int64_t g0 = r0 + ld_table(sl.run_start, r);
-      const int len = NNPP_BLOCK + ld_table(sl.run_len, r) - 1;
+      const int s0 = ld_table(sl.run_start, r);
+      const int tail = ld_table(sl.run_len, r) - 1;  // staged values beyond the first 256
       T* dst = lds + ld_table(sl.run_base, r);
-      for (int u = threadIdx.x; u < len; u += NNPP_BLOCK) {
-        const int64_t g = g0 + u;
-        dst[u] = (g >= 0 && g < n) ? v[g] : T(0);
+      const T* src = v + (r0 + s0);
+      if (inner) {
+        dst[threadIdx.x] = src[threadIdx.x];
+        if ((int)threadIdx.x < tail) dst[NNPP_BLOCK + threadIdx.x] = src[NNPP_BLOCK + threadIdx.x];
+      } else {
+        for (int u = threadIdx.x; u < NNPP_BLOCK + tail; u += NNPP_BLOCK) {
+          const int64_t g = r0 + s0 + u;
+          dst[u] = (g >= 0 && g < n) ? v[g] : T(0);
+        }
       }
     }
     __syncthreads();
@@ -609,7 +620,7 @@ PotOp pot_op(const nnpp_ctx* c) {
   const nnpp_stencil& T = c->pst;
   o.sl = Slices{P.off, P.len, P.col, P.d16, P.first, P.cls, P.tab64, P.tab32,
                 T.pmask, T.lc, T.run_start, T.run_len, T.run_base, T.tw64, T.tw32, T.perm,
-                T.K, T.R};
+                T.K, T.R, T.off_min, T.off_max};
   if (T.active) {
     o.var = 4;
     o.lds32 = sizeof(float) * T.lds_elems;
@@ -634,6 +645,24 @@ static int read_scalars(nnpp_ctx* c) {
 }
 
 static constexpr double RELIABLE_DROP2 = 1e-4;  // replace the residual every 2 decades
+
+// Iterations of the next launch batch: the host sees the residual only between batches, and
+// iterations launched after the stop exit at once but still cost their launches. From the
+// squared-residual reduction per iteration of the batches so far, predict the iterations to the
+// stop threshold (plus one); the first batch of a solve is CG_BATCH long.
+struct BatchPlan {
+  double rate = 0.0;
+  void observe(double rr0, double rr1, int its) {
+    if (its > 0 && rr0 > 0.0 && rr1 > 0.0 && rr1 < rr0) rate = std::pow(rr1 / rr0, 1.0 / its);
+  }
+  int next(double rr, double stop, int left) const {
+    int k = CG_BATCH;
+    if (rate > 0.0 && rate < 1.0 && stop > 0.0 && rr > stop)
+      k = (int)std::ceil(std::log(stop / rr) / std::log(rate)) + 1;
+    k = k < 2 ? 2 : (k > 2 * CG_BATCH ? 2 * CG_BATCH : k);
+    return k < left ? k : left;
+  }
+};
 
 // Uq: state whose species give the charge; phi: warm start in, result out (non-free,
 // non-Dirichlet entries are left as they are).
@@ -693,13 +722,16 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   } else if (precision != NNPP_MIXED) {
     if ((st = resid(true))) return st;
     bool done = c->h_scal[SC_DONE] != 0.0;
+    BatchPlan plan;
+    double rr_b = c->h_scal[SC_RR];
+    int it_b = 0;
     while (!done) {
       const int left = c->maxit - total;
       if (left <= 0) {
         status = NNPP_ERR_NOCONV;
         break;
       }
-      const int batch = left < CG_BATCH ? left : CG_BATCH;
+      const int batch = plan.next(rr_b, c->h_scal[SC_TARGET2], left);
       for (int it = 0; it < batch; it++) {
         launch_cg_spmv(var, op.lds64, g, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, part, scal);
         if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
@@ -714,6 +746,9 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       done = c->h_scal[SC_DONE] != 0.0;
       if (!std::isfinite(c->h_scal[SC_RR]))
         return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
+      plan.observe(rr_b, c->h_scal[SC_RR], total - it_b);
+      rr_b = c->h_scal[SC_RR];
+      it_b = total;
     }
     n64 += total;
     if ((st = resid(false))) return st;  // FP64 residual check of the final iterate
@@ -728,13 +763,16 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                                                  c->f_z, c->f_p, part);
       NNPP_HIP(c, hipGetLastError());
       if ((st = nnpp_finalize(c, FIN_M_REPLACE_START, g, rtol2))) return st;
+      BatchPlan plan;
+      double rr_b = rr / bnorm2, stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
+      int it_b = 0;
       while (true) {
         const int left = c->maxit - total;
         if (left <= 0) {
           status = NNPP_ERR_NOCONV;
           break;
         }
-        const int batch = left < CG_BATCH ? left : CG_BATCH;
+        const int batch = plan.next(rr_b, stop_b, left);
         for (int it = 0; it < batch; it++) {
           launch_m_spmv(var, op.lds32, g, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, part, scal);
           if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
@@ -748,6 +786,10 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         total = (int)c->h_scal[SC_ITERS];
         if (!std::isfinite(c->h_scal[SC_RR]))
           return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
+        plan.observe(rr_b, c->h_scal[SC_RR], total - it_b);
+        rr_b = c->h_scal[SC_RR];
+        stop_b = std::max(c->h_scal[SC_TARGET2], RELIABLE_DROP2 * c->h_scal[SC_RRMAX]);
+        it_b = total;
         if (c->h_scal[SC_DONE] == 0.0) continue;
         // reliable update: fold the correction into x, recompute the residual in FP64
         k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
@@ -762,6 +804,8 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                                                     c->f_z, c->f_p, part);
         NNPP_HIP(c, hipGetLastError());
         if ((st = nnpp_finalize(c, FIN_M_REPLACE, g, rtol2))) return st;
+        rr_b = rr / bnorm2;  // the replaced residual starts the next phase
+        stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
       }
       n32 += total;
       if (status != NNPP_OK) {  // keep the progress made so far

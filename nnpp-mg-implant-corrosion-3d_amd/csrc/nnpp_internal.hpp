@@ -66,6 +66,7 @@ struct nnpp_slices {
 struct nnpp_stencil {
   int state = 0;               // 0: not built for this geometry, 1: valid, -1: not a stencil
   int K = 0, R = 0;            // offsets, runs
+  int off_min = 0, off_max = 0;  // toff[0], toff[K-1]
   int lds_elems = 0;           // LDS entries per 256-row tile
   int32_t* toff = nullptr;     // [K] j - i, ascending
   int32_t* lc = nullptr;       // [K rounded up to 8] LDS position of entry t relative to the
