@@ -806,9 +806,9 @@ static int build_stencil(nnpp_ctx* c) {
   T.off_min = toff[0];
   T.off_max = toff[K - 1];
   T.lds_elems = base;
-  // the SpMV reads lc and the weights in chunks of 8 consecutive entries: both tables are padded
-  // to a multiple of 8 (LDS position 0, weight 0)
-  const int Kp = (K + 7) & ~7;
+  // the SpMV reads lc and the weights in chunks of NNPP_STENCIL_CHUNK consecutive entries: both
+  // tables are padded to a multiple of it (LDS position 0, weight 0)
+  const int Kp = (K + NNPP_STENCIL_CHUNK - 1) / NNPP_STENCIL_CHUNK * NNPP_STENCIL_CHUNK;
   lc.resize(Kp, 0);
   int32_t* reps_d = nullptr;
   NNPP_HIP(c, hipMalloc((void**)&T.toff, sizeof(int32_t) * K));

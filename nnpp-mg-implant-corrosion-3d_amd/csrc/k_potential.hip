@@ -234,12 +234,12 @@ __device__ __forceinline__ T spmv_row(const Slices& sl, const T* __restrict__ w,
 
 // Stencil variant: one 256-row tile. Every run of consecutive offsets needs a contiguous range
 // of v (256 + run length - 1 values), staged into LDS with coalesced loads; lane l then reads
-// entry t at lds[l + lc[t]]. Entries go in chunks of 8: the chunk's LDS positions and weights
-// are wave-uniform (scalar loads), its 8 LDS reads are issued together, and an absent entry
+// entry t at lds[l + lc[t]]. Entries go in chunks of 16: the chunk's LDS positions and weights
+// are wave-uniform (scalar loads), its 16 LDS reads are issued together, and an absent entry
 // takes weight +0 (its bits cleared), which leaves the sum unchanged, so no lane branches.
 // FP64: one fused multiply-add chain in ascending offset order, the additions of the column
-// form (bit for bit). FP32: a fused multiply-add chain per chunk, the chunk sums
-// Kahan-compensated. The leading barrier orders the staging after the previous tile's reads;
+// form (bit for bit). FP32: two fused multiply-add chains per chunk (even and odd entries),
+// the chunk sums Kahan-compensated. The leading barrier orders the staging after the previous tile's reads;
 // tiles without any entry skip the staging. All threads of the block call this.
 template <typename T>
 __device__ __forceinline__ T mask_weight(T w, int keep);
@@ -252,10 +252,13 @@ __device__ __forceinline__ double mask_weight<double>(double w, int keep) {
   return __longlong_as_double(__double_as_longlong(w) & (long long)keep);
 }
 
-template <typename T>
+// ORDERED: the result returned to lane j is row r0 + j's (the column form's lane order, which
+// the FP64 dot products need to match it bit for bit); otherwise lane j gets the result of the
+// row it computed, *row_out.
+template <typename T, bool ORDERED>
 __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict__ dd,
                                           const T* __restrict__ v, int32_t n, int64_t tile,
-                                          T* lds) {
+                                          T* lds, int64_t* row_out) {
   const int64_t r0 = tile * NNPP_BLOCK;
   const int l = sl.perm[r0 + threadIdx.x];  // this lane's row within the tile
   const int64_t i = r0 + l;
@@ -267,24 +270,45 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
     // tiles whose every staged index lies in [0, n) need no bounds checks (all but the first
     // and last few); the run's source is a wave-uniform pointer
     const bool inner = r0 + sl.off_min >= 0 && r0 + NNPP_BLOCK - 1 + sl.off_max < n;
-    for (int r = 0; r < sl.R; r++) {
-      const int s0 = ld_table(sl.run_start, r);
-      const int tail = ld_table(sl.run_len, r) - 1;  // staged values beyond the first 256
-      T* dst = lds + ld_table(sl.run_base, r);
-      const T* src = v + (r0 + s0);
-      if (inner) {
-        dst[threadIdx.x] = src[threadIdx.x];
-        if ((int)threadIdx.x < tail) dst[NNPP_BLOCK + threadIdx.x] = src[NNPP_BLOCK + threadIdx.x];
-      } else {
-        for (int u = threadIdx.x; u < NNPP_BLOCK + tail; u += NNPP_BLOCK) {
-          const int64_t g = r0 + s0 + u;
+    const int R = sl.R;
+    if (inner) {
+      // the loads of 8 runs are issued together, then stored: one memory latency per 8 runs
+      constexpr int RB = 8;
+      for (int rb = 0; rb < R; rb += RB) {
+        T val[RB], tl[RB];
+#pragma unroll
+        for (int k = 0; k < RB; k++) {
+          if (rb + k < R) {
+            const T* src = v + (r0 + ld_table(sl.run_start, rb + k));
+            val[k] = src[threadIdx.x];
+            if ((int)threadIdx.x < ld_table(sl.run_len, rb + k) - 1)  // the run's last values
+              tl[k] = src[NNPP_BLOCK + threadIdx.x];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < RB; k++) {
+          if (rb + k < R) {
+            T* dst = lds + ld_table(sl.run_base, rb + k);
+            dst[threadIdx.x] = val[k];
+            if ((int)threadIdx.x < ld_table(sl.run_len, rb + k) - 1)
+              dst[NNPP_BLOCK + threadIdx.x] = tl[k];
+          }
+        }
+      }
+    } else {
+      for (int r = 0; r < R; r++) {
+        const int64_t g0 = r0 + ld_table(sl.run_start, r);
+        const int len = NNPP_BLOCK + ld_table(sl.run_len, r) - 1;
+        T* dst = lds + ld_table(sl.run_base, r);
+        for (int u = threadIdx.x; u < len; u += NNPP_BLOCK) {
+          const int64_t g = g0 + u;
           dst[u] = (g >= 0 && g < n) ? v[g] : T(0);
         }
       }
     }
     __syncthreads();
   }
-  constexpr int BT = 8;
+  constexpr int BT = NNPP_STENCIL_CHUNK;
   const T* __restrict__ tw = stencil_weights<T>(sl);
   const T* row = lds + l;
   RowAcc<T> acc;
@@ -294,7 +318,8 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
       const int wd = t0 >> 5;
       const uint32_t bits =
           (wd == 0 ? m.x : (wd == 1 ? m.y : (wd == 2 ? m.z : m.w))) >> (t0 & 31);
-      const uint32_t valid = K - t0 >= BT ? 0xffu : (1u << (K - t0)) - 1u;
+      constexpr uint32_t all = (1u << BT) - 1u;
+      const uint32_t valid = K - t0 >= BT ? all : (1u << (K - t0)) - 1u;
       if (!__any(bits & valid)) continue;  // no lane of the wave has an entry in this chunk
       const bool full = __all((bits & valid) == valid);
       int lv[BT];
@@ -306,12 +331,12 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
       }
 #pragma unroll
       for (int k = 0; k < BT; k++) pv[k] = row[lv[k]];
-      T part = T(0);
+      T part[2] = {T(0), T(0)};  // FP32: even and odd entries, two short dependency chains
       if (full) {  // every lane has every entry of the chunk
 #pragma unroll
         for (int k = 0; k < BT; k++) {
           if constexpr (sizeof(T) == 8) acc.add(wv[k], vi - pv[k]);
-          else part = __fmaf_rn(wv[k], vi - pv[k], part);
+          else part[k & 1] = __fmaf_rn(wv[k], vi - pv[k], part[k & 1]);
         }
       } else {
 #pragma unroll
@@ -319,19 +344,25 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
           const int keep = (int)(bits << (31 - k)) >> 31;  // bit k -> all ones, else 0
           const T wk = mask_weight<T>(wv[k], keep);
           if constexpr (sizeof(T) == 8) acc.add(wk, vi - pv[k]);
-          else part = __fmaf_rn(wk, vi - pv[k], part);
+          else part[k & 1] = __fmaf_rn(wk, vi - pv[k], part[k & 1]);
         }
       }
-      if constexpr (sizeof(T) == 4) acc.add(T(1), part);
+      if constexpr (sizeof(T) == 4) acc.add(T(1), part[0] + part[1]);
     }
   }
-  // hand the results back in row order (lane j of the caller owns row r0 + j), through the
-  // start of the staging area once every lane is done reading it
   const T y = in ? acc.result(dd[i], vi) : T(0);
-  __syncthreads();
-  lds[l] = y;
-  __syncthreads();
-  return lds[threadIdx.x];
+  if constexpr (!ORDERED) {
+    *row_out = i;
+    return y;
+  } else {
+    // hand the results back in row order (lane j of the caller owns row r0 + j), through the
+    // start of the staging area once every lane is done reading it
+    __syncthreads();
+    lds[l] = y;
+    __syncthreads();
+    *row_out = r0 + threadIdx.x;
+    return lds[threadIdx.x];
+  }
 }
 
 // b (free rows), warm start x = phi, ||b||^2.
@@ -387,7 +418,8 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     double mx = 0.0;
-    if constexpr (STEN) mx = stencil_tile<double>(sl, dd, x, n, it.cur, st_lds);
+    int64_t ir = i;
+    if constexpr (STEN) mx = stencil_tile<double, true>(sl, dd, x, n, it.cur, st_lds, &ir);
     if (i >= n) continue;
     if constexpr (!STEN)
       if (pinv[i] != 0.0) mx = spmv_row<double, WIDE, CLS>(sl, w, tab, dd, x, i);
@@ -419,7 +451,8 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     double qi = 0.0;
-    if constexpr (STEN) qi = stencil_tile<double>(sl, dd, p, n, it.cur, st_lds);
+    int64_t ir = i;
+    if constexpr (STEN) qi = stencil_tile<double, true>(sl, dd, p, n, it.cur, st_lds, &ir);
     if (i >= n) continue;
     if constexpr (!STEN) qi = spmv_row<double, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
@@ -500,9 +533,10 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
   double v[1] = {0.0};
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
-    const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
+    int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     float qi = 0.0f;
-    if constexpr (STEN) qi = stencil_tile<float>(sl, dd, p, n, it.cur, st_lds32);
+    // FP32: the stencil rows stay in their lane order (the dot is FP64 over FP32 products)
+    if constexpr (STEN) qi = stencil_tile<float, false>(sl, dd, p, n, it.cur, st_lds32, &i);
     if (i >= n) continue;
     if constexpr (!STEN) qi = spmv_row<float, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;

@@ -63,19 +63,20 @@ struct nnpp_slices {
 // consecutive offsets; a 256-row tile of the SpMV stages, for each run, the contiguous range of
 // the vector its rows read into LDS, and lane l reads entry t at lds[l + lc[t]].
 #define NNPP_STENCIL_MAX 128
+#define NNPP_STENCIL_CHUNK 16  // entries per step of the stencil SpMV; tables padded to it
 struct nnpp_stencil {
   int state = 0;               // 0: not built for this geometry, 1: valid, -1: not a stencil
   int K = 0, R = 0;            // offsets, runs
   int off_min = 0, off_max = 0;  // toff[0], toff[K-1]
   int lds_elems = 0;           // LDS entries per 256-row tile
   int32_t* toff = nullptr;     // [K] j - i, ascending
-  int32_t* lc = nullptr;       // [K rounded up to 8] LDS position of entry t relative to the
-                               // lane (padding: 0)
+  int32_t* lc = nullptr;       // [K rounded up to NNPP_STENCIL_CHUNK] LDS position of entry t
+                               // relative to the lane (padding: 0)
   int32_t* run_start = nullptr;  // [R] first offset of run r
   int32_t* run_len = nullptr;    // [R]
   int32_t* run_base = nullptr;   // [R] LDS base of run r
-  double* tw64 = nullptr;      // [K rounded up to 8] potential weight of offset t (FP64 /
-  float* tw32 = nullptr;       // FP32; padding: 0)
+  double* tw64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] potential weight of
+  float* tw32 = nullptr;       // offset t (FP64 / FP32; padding: 0)
   uint4* bmask = nullptr;      // [n] bonds of each row
   uint4* pmask = nullptr;      // [n] potential-operator entries of each row
   uint8_t* perm = nullptr;     // [256 per tile] row of each lane: rows with every entry first,
