@@ -645,7 +645,24 @@ struct PotOp {
   size_t lds32, lds64;
   const double* w64;
   const float* w32;
+  int64_t g32, g64;  // grids of the FP32 / FP64 SpMV-type kernels
 };
+
+// Blocks of the stencil SpMV kernels that fit on the device at once (LDS bounds them): a grid of
+// exactly that many blocks walks the tiles in one round, where a larger grid would leave a
+// partly filled second round. A multiple of 8 (one share per XCD), at most nnpp_red_grid.
+template <typename K>
+int64_t resident_grid(K kernel, size_t lds, int device, int64_t cap) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, NNPP_BLOCK, lds) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      per_cu <= 0 || cus <= 0)
+    return cap;
+  int64_t g = (int64_t)per_cu * cus;
+  g -= g % NNPP_NXCD;
+  return g < NNPP_NXCD ? cap : (g < cap ? g : cap);
+}
 
 PotOp pot_op(const nnpp_ctx* c) {
   PotOp o;
@@ -665,6 +682,12 @@ PotOp pot_op(const nnpp_ctx* c) {
     o.var = 2 * (P.wide != 0) + (P.ncls > 0);
     o.w64 = P.w64;
     o.w32 = P.w32;
+  }
+  const int64_t cap = nnpp_red_grid(nnpp_nblocks(c->n));
+  o.g32 = o.g64 = cap;
+  if (T.active) {
+    o.g32 = resident_grid(k_m_spmv<false, false, true>, o.lds32, c->device, cap);
+    o.g64 = resident_grid(k_cg_spmv<false, false, true>, o.lds64, c->device, cap);
   }
   return o;
 }
@@ -740,14 +763,14 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   // r = b - M x in FP64 (START: also z, p, rho, target, converged flag)
   auto resid = [&](bool start) -> int {
     if (start)
-      launch_resid<true>(var, op.lds64, g, s, n, sl, op.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+      launch_resid<true>(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
                          c->cg_r, c->cg_z, c->cg_p, part);
     else
-      launch_resid<false>(var, op.lds64, g, s, n, sl, op.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
+      launch_resid<false>(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
                           c->cg_r, nullptr, nullptr, part);
     NNPP_HIP(c, hipGetLastError());
     n64++;
-    int e = nnpp_finalize(c, start ? FIN_RESID_START : FIN_RESID, g, rtol2);
+    int e = nnpp_finalize(c, start ? FIN_RESID_START : FIN_RESID, op.g64, rtol2);
     if (e) return e;
     return read_scalars(c);
   };
@@ -767,8 +790,9 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       }
       const int batch = plan.next(rr_b, c->h_scal[SC_TARGET2], left);
       for (int it = 0; it < batch; it++) {
-        launch_cg_spmv(var, op.lds64, g, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, part, scal);
-        if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
+        launch_cg_spmv(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, part,
+                       scal);
+        if ((st = nnpp_finalize(c, FIN_PQ, op.g64, 0.0))) return st;
         k_cg_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
                                              c->cg_z, part, scal);
         if ((st = nnpp_finalize(c, FIN_UPDATE, g, 0.0))) return st;
@@ -808,8 +832,9 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         }
         const int batch = plan.next(rr_b, stop_b, left);
         for (int it = 0; it < batch; it++) {
-          launch_m_spmv(var, op.lds32, g, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, part, scal);
-          if ((st = nnpp_finalize(c, FIN_PQ, g, 0.0))) return st;
+          launch_m_spmv(var, op.lds32, op.g32, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, part,
+                        scal);
+          if ((st = nnpp_finalize(c, FIN_PQ, op.g32, 0.0))) return st;
           k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
                                               c->f_z, part, scal);
           if ((st = nnpp_finalize(c, FIN_M_UPDATE, g, RELIABLE_DROP2))) return st;
@@ -863,15 +888,14 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const int32_t n = c->n;
   if (!n) return NNPP_OK;
-  const int64_t g = nnpp_red_grid(nnpp_nblocks(n));
   const PotOp op = pot_op(c);
   const Slices& sl = op.sl;
   const int var = op.var;
   if (precision == NNPP_MIXED)
-    launch_m_spmv(var, op.lds32, g, c->stream, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
+    launch_m_spmv(var, op.lds32, op.g32, c->stream, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
                   (const double*)c->scal);
   else
-    launch_cg_spmv(var, op.lds64, g, c->stream, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
+    launch_cg_spmv(var, op.lds64, op.g64, c->stream, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, c->red_part,
                    (const double*)c->scal);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
