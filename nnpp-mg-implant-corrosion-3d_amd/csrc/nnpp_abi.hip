@@ -367,7 +367,7 @@ int nnpp_set_bc(nnpp_ctx* c, const int32_t* bctypes, const double* bcvalues) {
 }
 
 int nnpp_set_state(nnpp_ctx* c, int32_t slot, const double* U) {
-  if (!c || !U || slot < 0 || slot > 1) return NNPP_ERR_ARG;
+  if (!c || (!U && c->n) || slot < 0 || slot > 1) return NNPP_ERR_ARG;
   if (int g_ = tape_guard(c)) return g_;
   int st = need_params(c);
   if (st) return st;
@@ -379,7 +379,7 @@ int nnpp_set_state(nnpp_ctx* c, int32_t slot, const double* U) {
 }
 
 int nnpp_get_state(nnpp_ctx* c, int32_t slot, double* U) {
-  if (!c || !U || slot < 0 || slot > 1) return NNPP_ERR_ARG;
+  if (!c || (!U && c->n) || slot < 0 || slot > 1) return NNPP_ERR_ARG;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
   NNPP_HIP(c, hipSetDevice(c->device));
   NNPP_HIP(c, hipStreamSynchronize(c->stream));
@@ -389,7 +389,7 @@ int nnpp_get_state(nnpp_ctx* c, int32_t slot, double* U) {
 }
 
 int nnpp_set_phase(nnpp_ctx* c, const int32_t* phase) {
-  if (!c || !phase) return NNPP_ERR_ARG;
+  if (!c || (!phase && c->n)) return NNPP_ERR_ARG;
   if (int g_ = tape_guard(c)) return g_;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
   std::vector<uint8_t> h(c->n);
@@ -406,7 +406,7 @@ int nnpp_set_phase(nnpp_ctx* c, const int32_t* phase) {
 }
 
 int nnpp_get_phase(nnpp_ctx* c, int32_t* phase) {
-  if (!c || !phase) return NNPP_ERR_ARG;
+  if (!c || (!phase && c->n)) return NNPP_ERR_ARG;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
   std::vector<uint8_t> h(c->n);
   NNPP_HIP(c, hipSetDevice(c->device));
@@ -417,7 +417,7 @@ int nnpp_get_phase(nnpp_ctx* c, int32_t* phase) {
 }
 
 int nnpp_init_state(nnpp_ctx* c, const double* ic) {
-  if (!c || !ic) return NNPP_ERR_ARG;
+  if (!c || (!ic && c->n)) return NNPP_ERR_ARG;
   if (int g_ = tape_guard(c)) return g_;
   int st = need_params(c);
   if (st) return st;
@@ -477,7 +477,7 @@ int nnpp_get_time(nnpp_ctx* c, int64_t* step, double* voln1rel, double* vol0rel)
 
 int nnpp_residuals(nnpp_ctx* c, double voln1rel, int32_t apply_bc, int32_t precision,
                    double* F_out) {
-  if (!c || !F_out) return NNPP_ERR_ARG;
+  if (!c || (!F_out && c->n)) return NNPP_ERR_ARG;
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = ensure_bonds(c);
   if (st) return st;
@@ -495,7 +495,7 @@ int nnpp_residuals(nnpp_ctx* c, double voln1rel, int32_t apply_bc, int32_t preci
 }
 
 int nnpp_poisson_apply(nnpp_ctx* c, const double* x, double* y) {
-  if (!c || !x || !y) return NNPP_ERR_ARG;
+  if (!c || ((!x || !y) && c->n)) return NNPP_ERR_ARG;
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = need_graph(c);
   if (st) return st;
@@ -659,7 +659,7 @@ int nnpp_tape_begin(nnpp_ctx* c, int32_t every) {
 
 int nnpp_tape_adjoint(nnpp_ctx* c, const double* lam, double* grad, double* lam0,
                       nnpp_adjoint_stats* stats) {
-  if (!c || !lam || !grad) return NNPP_ERR_ARG;
+  if (!c || (!lam && c->n) || !grad) return NNPP_ERR_ARG;
   if (!c->tape.active) return nnpp_set_error(c, NNPP_ERR_STATE, "no tape (nnpp_tape_begin)");
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = ensure_bonds(c);
@@ -692,7 +692,7 @@ int nnpp_stream(nnpp_ctx* c, void** stream) {
 }
 
 int nnpp_potential_operator_apply(nnpp_ctx* c, int32_t precision, const double* x, double* y) {
-  if (!c || !x || !y) return NNPP_ERR_ARG;
+  if (!c || ((!x || !y) && c->n)) return NNPP_ERR_ARG;
   if (precision != NNPP_FP64 && precision != NNPP_MIXED) return NNPP_ERR_ARG;
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = ensure_potential(c);

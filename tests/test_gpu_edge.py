@@ -1,0 +1,110 @@
+"""Edge cases through the C ABI on the GPU: empty node sets, isolated nodes and ragged rows,
+all-solid and all-Dirichlet potentials, a single-tile lattice, against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import nn, orc, rel_err
+
+pytestmark = pytest.mark.gpu
+
+O = orc.Oracle(nn.Params)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_both(case, nsteps, prec=nn.FP64, rtol=1e-13):
+    p = case.params
+    st, rp, col, _ = O.build_neighbors(case.x, case.y, case.z, p.delta * (1 + 1e-5), p.dx * 10.,
+                                       p.dy * 10., p.dz * 10.)
+    assert st == 0
+    ref = O.run(case, rp, col, nsteps, rtol=rtol)
+    with nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0) as pb:
+        nnz, _ = pb.build_clouds_hash()
+        assert nnz == rp[-1]
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=rtol, maxit=100000)
+        st = pb.step(nsteps, prec)
+        return ref, pb.get_state(), pb.get_phase(), st
+
+
+def test_empty_node_set():
+    say("case empty")
+    p = nn.synth.synthetic_params(8, S=3)
+    p.n_nodes = 0
+    e = np.zeros(0)
+    with nn.Problem(p, e, e, e, None, device=0) as pb:
+        nnz, ml = pb.build_clouds_hash()
+        assert nnz == 0
+        pb.set_bc(np.zeros(0, np.int32), np.zeros((4, 0)))
+        pb.init_state(np.zeros((4, 0)))
+        st = pb.step(2, nn.FP64)
+        # no solid node: the run stops after one step, as main.cpp:924-932 does
+        assert st["status"] == 0 and st["steps"] == 1
+        st = pb.step(1, nn.MIXED)
+        assert st["status"] == 0
+        assert pb.get_state().shape == (4, 0)
+
+
+def test_isolated_nodes_and_ragged_rows():
+    """A regular block plus far-away isolated nodes (rows of length 0) and a sparse cluster."""
+    say("case isolated / ragged")
+    case = nn.synth.regular(8, S=3, seed=5)
+    rng = np.random.default_rng(0)
+    extra = 40
+    ex = rng.uniform(3.0, 9.0, size=(3, extra))  # far from the unit cube, mostly isolated
+    ex[:, :8] = 5.0 + rng.uniform(0, 0.2, size=(3, 8))  # a small sparse cluster
+    case.x = np.concatenate([case.x, ex[0]])
+    case.y = np.concatenate([case.y, ex[1]])
+    case.z = np.concatenate([case.z, ex[2]])
+    F = case.params.S + 1
+    case.bctypes = np.concatenate([case.bctypes, np.ones(extra, np.int32)])
+    case.bcvalues = np.concatenate([case.bcvalues, -np.ones((F, extra))], axis=1)
+    ic = rng.uniform(0.9, 1.1, size=(F, extra))
+    ic[case.params.S] = 0.0
+    case.icvalues = np.concatenate([case.icvalues, ic], axis=1)
+    case.params.n_nodes = case.x.size
+    for prec in (nn.FP64, nn.MIXED):
+        ref, U, ph, st = run_both(case, 5, prec)
+        assert st["status"] == 0
+        assert np.array_equal(ph, ref["phase"])
+        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        for f in range(F):
+            assert rel_err(U[f], ref["U"][f]) < tol, (prec, f)
+
+
+def test_all_solid_and_all_dirichlet_potential():
+    say("case all-solid / all-Dirichlet")
+    case = nn.synth.regular(6, S=3, seed=2)
+    case.icvalues[0] = 1.0  # every node solid: no liquid bond, no potential unknown
+    ref, U, ph, st = run_both(case, 3)
+    assert st["status"] == 0 and np.all(ph == ref["phase"])
+    for f in range(4):
+        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+    case = nn.synth.regular(6, S=3, seed=2)
+    case.bctypes[:] = 0  # every node Dirichlet
+    case.bcvalues[:, :] = np.where(case.bcvalues == -1, 1.0, case.bcvalues)
+    ref, U, ph, st = run_both(case, 2)
+    assert st["status"] == 0
+    for f in range(4):
+        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+
+
+@pytest.mark.parametrize("ng", [5, 7, 9, 16])
+def test_small_lattices_tile_edges(ng):
+    """Lattices smaller than one 256-row tile or ending inside one (stencil staging bounds)."""
+    say("case lattice", ng)
+    case = nn.synth.regular(ng, S=3, seed=ng)
+    case.params.front_threshold = 1.0
+    for prec in (nn.FP64, nn.MIXED):
+        ref, U, ph, st = run_both(case, 4, prec)
+        assert st["status"] == 0
+        assert np.array_equal(ph, ref["phase"])
+        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        for f in range(4):
+            assert rel_err(U[f], ref["U"][f]) < tol, (ng, prec, f)
