@@ -728,8 +728,8 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
 }
 
 void nnpp_free_stencil(nnpp_stencil& t) {
-  void* ps[] = {t.toff, t.lc, t.run_start, t.run_len, t.run_base, t.tw64, t.tw32, t.bmask,
-                t.pmask, t.perm};
+  void* ps[] = {t.toff, t.lc, t.seg_start, t.seg_len, t.seg_base, t.piece, t.tw64, t.tw32,
+                t.bmask, t.pmask, t.perm};
   for (void* q : ps)
     if (q) hipFree(q);
   t = nnpp_stencil();
@@ -774,23 +774,35 @@ static int build_stencil(nnpp_ctx* c) {
     if (ht[h]) offs.push_back({(int32_t)(uint32_t)ht[h], hr[h]});
   std::sort(offs.begin(), offs.end());
   const int K = (int)offs.size();
-  std::vector<int32_t> toff(K), reps(K), lc(K), rs, rl, rb;
+  std::vector<int32_t> toff(K), reps(K), lc(K), rs, rl, rb, pc;
   for (int t = 0; t < K; t++) {
     toff[t] = offs[t].first;
     reps[t] = offs[t].second;
   }
   int base = 0;
-  for (int t = 0; t < K;) {  // runs of consecutive offsets
+  for (int t = 0; t < K;) {  // segments: ranges [toff + r0, toff + r0 + 255] that overlap or touch
     int u = t + 1;
-    while (u < K && toff[u] == toff[u - 1] + 1) u++;
+    while (u < K && (int64_t)toff[u] <= (int64_t)toff[u - 1] + NNPP_BLOCK) u++;
+    const int span = toff[u - 1] - toff[t] + 1;
+    const int64_t staged = (int64_t)NNPP_BLOCK + span - 1;
+    if ((int64_t)base + staged > (int64_t)1 << 20) {  // far beyond any LDS: not a stencil
+      hipFree(flags);
+      return NNPP_OK;
+    }
     rs.push_back(toff[t]);
-    rl.push_back(u - t);
+    rl.push_back(span);
     rb.push_back(base);
     for (int q = t; q < u; q++) lc[q] = base + (toff[q] - toff[t]);
-    base += NNPP_BLOCK + (u - t) - 1;
+    for (int64_t k = 0; k < staged; k += NNPP_BLOCK) {
+      pc.push_back(toff[t] + (int32_t)k);
+      pc.push_back(base + (int32_t)k);
+      pc.push_back((int32_t)(staged - k < NNPP_BLOCK ? staged - k : NNPP_BLOCK));
+    }
+    base += (int)staged;
     t = u;
   }
   const int R = (int)rs.size();
+  const int P = (int)pc.size() / 3;
   // the FP64 tile (plus the kernels' few static LDS words) must fit one workgroup's LDS; it
   // also holds the tile's 256 results at the end
   int max_lds = 65536;
@@ -803,6 +815,7 @@ static int build_stencil(nnpp_ctx* c) {
   }
   T.K = K;
   T.R = R;
+  T.P = P;
   T.off_min = toff[0];
   T.off_max = toff[K - 1];
   T.lds_elems = base;
@@ -814,9 +827,10 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMalloc((void**)&T.toff, sizeof(int32_t) * K));
   NNPP_HIP(c, hipMalloc((void**)&T.lc, sizeof(int32_t) * Kp));
   NNPP_HIP(c, hipMalloc((void**)&reps_d, sizeof(int32_t) * K));
-  NNPP_HIP(c, hipMalloc((void**)&T.run_start, sizeof(int32_t) * R));
-  NNPP_HIP(c, hipMalloc((void**)&T.run_len, sizeof(int32_t) * R));
-  NNPP_HIP(c, hipMalloc((void**)&T.run_base, sizeof(int32_t) * R));
+  NNPP_HIP(c, hipMalloc((void**)&T.seg_start, sizeof(int32_t) * R));
+  NNPP_HIP(c, hipMalloc((void**)&T.seg_len, sizeof(int32_t) * R));
+  NNPP_HIP(c, hipMalloc((void**)&T.seg_base, sizeof(int32_t) * R));
+  NNPP_HIP(c, hipMalloc((void**)&T.piece, sizeof(int32_t) * 3 * P));
   NNPP_HIP(c, hipMalloc((void**)&T.tw64, sizeof(double) * Kp));
   NNPP_HIP(c, hipMalloc((void**)&T.tw32, sizeof(float) * Kp));
   NNPP_HIP(c, hipMemsetAsync(T.tw64, 0, sizeof(double) * Kp, s));
@@ -827,9 +841,11 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMemcpyAsync(T.toff, toff.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(T.lc, lc.data(), sizeof(int32_t) * Kp, hipMemcpyHostToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(reps_d, reps.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, s));
-  NNPP_HIP(c, hipMemcpyAsync(T.run_start, rs.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
-  NNPP_HIP(c, hipMemcpyAsync(T.run_len, rl.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
-  NNPP_HIP(c, hipMemcpyAsync(T.run_base, rb.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.seg_start, rs.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.seg_len, rl.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.seg_base, rb.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
+  NNPP_HIP(c, hipMemcpyAsync(T.piece, pc.data(), sizeof(int32_t) * 3 * P, hipMemcpyHostToDevice,
+                             s));
   const WeightArgs wa = weight_args(c, 3.0 + 2.0 * c->p.s_phi);
   k_st_weights<<<1, NNPP_STENCIL_MAX, 0, s>>>(K, T.toff, reps_d, c->x, c->y, c->z, wa, T.tw64,
                                               T.tw32);

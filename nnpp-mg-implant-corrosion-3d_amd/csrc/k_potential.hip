@@ -44,13 +44,14 @@ struct Slices {
   // stencil variant
   const uint4* pmask;  // bit t: entry t of the row present
   const int32_t* lc;   // LDS position of entry t relative to the lane (padded to 8 entries)
-  const int32_t* run_start;
-  const int32_t* run_len;
-  const int32_t* run_base;
+  const int32_t* seg_start;  // staging segments: first offset, span, LDS base
+  const int32_t* seg_len;
+  const int32_t* seg_base;
+  const int32_t* piece;  // staging pieces: vector position - r0, LDS position, count
   const double* tw64;  // weight of entry t (padded with 0 to 8 entries)
   const float* tw32;
   const uint8_t* perm;  // row of each lane within its tile
-  int K, R;
+  int K, R, P;
   int off_min, off_max;  // smallest and largest offset
 };
 
@@ -232,9 +233,9 @@ __device__ __forceinline__ T spmv_row(const Slices& sl, const T* __restrict__ w,
   return acc.result(dd[i], vi);
 }
 
-// Stencil variant: one 256-row tile. Every run of consecutive offsets needs a contiguous range
-// of v (256 + run length - 1 values), staged into LDS with coalesced loads; lane l then reads
-// entry t at lds[l + lc[t]]. Entries go in chunks of 16: the chunk's LDS positions and weights
+// Stencil variant: one 256-row tile. Every segment of offsets a..b (offsets whose ranges overlap)
+// needs the contiguous range v[r0 + a .. r0 + b + 255], staged into LDS with coalesced loads in
+// pieces of up to 256 values; lane l then reads entry t at lds[l + lc[t]]. Entries go in chunks of 16: the chunk's LDS positions and weights
 // are wave-uniform (scalar loads), its 16 LDS reads are issued together, and an absent entry
 // takes weight +0 (its bits cleared), which leaves the sum unchanged, so no lane branches.
 // FP64: one fused multiply-add chain in ascending offset order, the additions of the column
@@ -268,38 +269,31 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
   const T vi = in ? v[i] : T(0);
   if (__syncthreads_or(any != 0u)) {
     // tiles whose every staged index lies in [0, n) need no bounds checks (all but the first
-    // and last few); the run's source is a wave-uniform pointer
+    // and last few); a piece's source and destination are wave-uniform
     const bool inner = r0 + sl.off_min >= 0 && r0 + NNPP_BLOCK - 1 + sl.off_max < n;
-    const int R = sl.R;
     if (inner) {
-      // the loads of 8 runs are issued together, then stored: one memory latency per 8 runs
-      constexpr int RB = 8;
-      for (int rb = 0; rb < R; rb += RB) {
-        T val[RB], tl[RB];
+      // the loads of 8 pieces are issued together, then stored: one memory latency per 8 pieces
+      constexpr int PB = 8;
+      const int P = sl.P;
+      for (int pb = 0; pb < P; pb += PB) {
+        T val[PB];
 #pragma unroll
-        for (int k = 0; k < RB; k++) {
-          if (rb + k < R) {
-            const T* src = v + (r0 + ld_table(sl.run_start, rb + k));
-            val[k] = src[threadIdx.x];
-            if ((int)threadIdx.x < ld_table(sl.run_len, rb + k) - 1)  // the run's last values
-              tl[k] = src[NNPP_BLOCK + threadIdx.x];
-          }
+        for (int k = 0; k < PB; k++) {
+          if (pb + k < P && (int)threadIdx.x < ld_table(sl.piece, 3 * (pb + k) + 2))
+            val[k] = v[r0 + ld_table(sl.piece, 3 * (pb + k)) + threadIdx.x];
         }
 #pragma unroll
-        for (int k = 0; k < RB; k++) {
-          if (rb + k < R) {
-            T* dst = lds + ld_table(sl.run_base, rb + k);
-            dst[threadIdx.x] = val[k];
-            if ((int)threadIdx.x < ld_table(sl.run_len, rb + k) - 1)
-              dst[NNPP_BLOCK + threadIdx.x] = tl[k];
-          }
+        for (int k = 0; k < PB; k++) {
+          if (pb + k < P && (int)threadIdx.x < ld_table(sl.piece, 3 * (pb + k) + 2))
+            lds[ld_table(sl.piece, 3 * (pb + k) + 1) + threadIdx.x] = val[k];
         }
       }
     } else {
+      const int R = sl.R;
       for (int r = 0; r < R; r++) {
-        const int64_t g0 = r0 + ld_table(sl.run_start, r);
-        const int len = NNPP_BLOCK + ld_table(sl.run_len, r) - 1;
-        T* dst = lds + ld_table(sl.run_base, r);
+        const int64_t g0 = r0 + ld_table(sl.seg_start, r);
+        const int len = NNPP_BLOCK + ld_table(sl.seg_len, r) - 1;
+        T* dst = lds + ld_table(sl.seg_base, r);
         for (int u = threadIdx.x; u < len; u += NNPP_BLOCK) {
           const int64_t g = g0 + u;
           dst[u] = (g >= 0 && g < n) ? v[g] : T(0);
@@ -670,8 +664,8 @@ PotOp pot_op(const nnpp_ctx* c) {
   const nnpp_slices& P = c->pot;
   const nnpp_stencil& T = c->pst;
   o.sl = Slices{P.off, P.len, P.col, P.d16, P.first, P.cls, P.tab64, P.tab32,
-                T.pmask, T.lc, T.run_start, T.run_len, T.run_base, T.tw64, T.tw32, T.perm,
-                T.K, T.R, T.off_min, T.off_max};
+                T.pmask, T.lc, T.seg_start, T.seg_len, T.seg_base, T.piece, T.tw64, T.tw32,
+                T.perm, T.K, T.R, T.P, T.off_min, T.off_max};
   if (T.active) {
     o.var = 4;
     o.lds32 = sizeof(float) * T.lds_elems;

@@ -59,22 +59,27 @@ struct nnpp_slices {
 // has j - i in a table of at most NNPP_STENCIL_MAX offsets and a weight that depends only on the
 // offset (a lattice numbered in any fixed order, constant node volume). Row i then needs one
 // 128-bit mask (bit t: entry i + toff[t] present) instead of its column and weight streams.
-// The offsets are sorted ascending (the order of the neighbour lists) and grouped into runs of
-// consecutive offsets; a 256-row tile of the SpMV stages, for each run, the contiguous range of
-// the vector its rows read into LDS, and lane l reads entry t at lds[l + lc[t]].
+// The offsets are sorted ascending (the order of the neighbour lists). A 256-row tile of the
+// SpMV reads, for offsets a..b, the vector range [r0 + a, r0 + b + 255]; offsets whose ranges
+// overlap or touch form one segment, whose range the tile stages into LDS once (on a lattice
+// numbered x-fastest, one segment per z-plane of the stencil), and lane l reads entry t at
+// lds[l + lc[t]]. The staging walks the segments in pieces of at most 256 values, one value per
+// thread and piece.
 #define NNPP_STENCIL_MAX 128
 #define NNPP_STENCIL_CHUNK 16  // entries per step of the stencil SpMV; tables padded to it
 struct nnpp_stencil {
   int state = 0;               // 0: not built for this geometry, 1: valid, -1: not a stencil
-  int K = 0, R = 0;            // offsets, runs
+  int K = 0, R = 0, P = 0;     // offsets, segments, staging pieces
   int off_min = 0, off_max = 0;  // toff[0], toff[K-1]
   int lds_elems = 0;           // LDS entries per 256-row tile
   int32_t* toff = nullptr;     // [K] j - i, ascending
   int32_t* lc = nullptr;       // [K rounded up to NNPP_STENCIL_CHUNK] LDS position of entry t
                                // relative to the lane (padding: 0)
-  int32_t* run_start = nullptr;  // [R] first offset of run r
-  int32_t* run_len = nullptr;    // [R]
-  int32_t* run_base = nullptr;   // [R] LDS base of run r
+  int32_t* seg_start = nullptr;  // [R] first offset of segment r
+  int32_t* seg_len = nullptr;    // [R] last offset - first offset + 1
+  int32_t* seg_base = nullptr;   // [R] LDS base of segment r
+  int32_t* piece = nullptr;      // [3P] per piece: vector position relative to the tile's first
+                                 // row, LDS position, value count (1..256)
   double* tw64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] potential weight of
   float* tw32 = nullptr;       // offset t (FP64 / FP32; padding: 0)
   uint4* bmask = nullptr;      // [n] bonds of each row

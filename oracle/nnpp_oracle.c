@@ -863,3 +863,318 @@ int or_adjoint(int32_t n, const nnpp_params* p, const double* x, const double* y
   free(dir);
   return status;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Implicit (reference) step: utils.h:844-1849 (NewtonMatrices) and main.cpp:547-1025.
+ * ------------------------------------------------------------------------------------------ */
+
+/* R'_k of the Jacobian diagonal (reference chemistry: utils.h:1001-1006, 1089-1100, 1184-1190,
+ * 1274-1279, 1364-1369, 1454-1459, 1545-1550, 1635-1640; none for Na+ and Cl-). un = U^n at the
+ * node. Generic chemistry: the exact derivative of its sink, -r_diss at interface solid nodes. */
+static double jac_reaction(int k, const nnpp_params* p, int phase, int intf, const double* un) {
+  const int liquid = (phase == 0);
+  if (p->chemistry != NNPP_CHEM_REFERENCE)
+    return (k == 0 && phase == 1 && intf == 1) ? -p->r_diss : 0.0;
+  switch (k) {
+    case 1: /* H+: interface or liquid */
+      if (intf == 1 || liquid) return p->k7b * (-un[2]) + p->k8b * (-un[4]) + p->k9b * (-un[6]);
+      return 0.0;
+    case 2: /* OH-: liquid only (interface 0) */
+      return (intf != 1 && liquid) ? p->k7b * (-un[1]) : 0.0;
+    case 3: /* HCO3- */
+    case 4: /* CO3 2- */
+      return (!(phase == 1 && intf == 1) && liquid) ? p->k8b * (-un[1]) : 0.0;
+    case 6: /* PO4 3- */
+      return (!(phase == 1 && intf == 1) && liquid) ? p->k9b * (-un[1]) : 0.0;
+    default: /* Mg2+ (0 although R1 depends on U1n1), HPO4 2-, Ca2+, Na+, Cl- */
+      return 0.0;
+  }
+}
+
+static int is_dirichlet(const int32_t* bctypes, const double* bcvalues, int32_t n, int f,
+                        int32_t i) {
+  return bctypes && bcvalues && bctypes[i] == 0 && bcvalues[(size_t)f * n + i] != -1;
+}
+
+/* Block f of the frozen Jacobian applied to in (one n-vector); diag (optional) gets K_ii. */
+static void jac_field(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                      const double* z, const double* vol, const int64_t* row_ptr,
+                      const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                      const double* bcvalues, const double* Un, double voln1rel, int f,
+                      const double* in, double* out, double* diag) {
+  const int S = p->n_species;
+  const double BV = pow(10., -p->vol_red * (1. - voln1rel));
+  const double* phin = Un + (size_t)S * n;
+  if (f == S) { /* utils.h:1794-1848 */
+    const double ks = p->K_phi * p->sigma1, ex = 3.0 + 2.0 * p->s_phi;
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; i++) {
+      double sum = 0.0, off = 0.0;
+      int flag = 0;
+      if (!is_dirichlet(bctypes, bcvalues, n, S, i))
+        for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+          const int32_t nn = col[b];
+          if (!(phase[i] == 0 && phase[nn] == 0)) continue;
+          const double xi = bond_xi(x, y, z, i, nn), be = or_beta(xi, p->delta, p->dx);
+          sum += bond_vol(vol, p, -(ks / pow(xi, ex)) * be, nn);
+          off += bond_vol(vol, p, -(1. / ks) * (ks / pow(xi, ex)) * be, nn) * in[nn];
+          flag = 1;
+        }
+      const double d = flag ? -(1. / ks) * sum : 1.0;
+      out[i] = d * in[i] + off;
+      if (diag) diag[i] = d;
+    }
+    return;
+  }
+  const double K1 = p->K1[f], K2 = p->K2[f], Ds = p->D_solid[f], Dl = p->D_liquid[f];
+  const double zk = p->z[f], ex1 = 3.0 + 2.0 * p->s1[f], ex2 = 3.0 + 2.0 * p->s2[f];
+  const int sb = p->solid_bonds[f];
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < n; i++) {
+    if (is_dirichlet(bctypes, bcvalues, n, f, i)) { /* identity row */
+      out[i] = in[i];
+      if (diag) diag[i] = 1.0;
+      continue;
+    }
+    int intf = 0;
+    if (phase[i] == 1)
+      for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++)
+        if (phase[col[b]] == 0) {
+          intf = 1;
+          break;
+        }
+    double sum = 0.0, off = 0.0;
+    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+      const int32_t nn = col[b];
+      const int pi = phase[i], pn = phase[nn];
+      double D, bv;
+      if (pi == 1 && pn == 1) { /* only species with solid bonds, BV = 1 */
+        if (!sb) continue;
+        D = Ds;
+        bv = 1.0;
+      } else if (pi != pn) {
+        D = Ds;
+        bv = sb ? BV : 1.0;
+      } else {
+        D = Dl;
+        bv = 1.0;
+      }
+      const double xi = bond_xi(x, y, z, i, nn), be = or_beta(xi, p->delta, p->dx);
+      sum += bond_vol(vol, p, -(K1 * D * bv / pow(xi, ex1)) * be, nn);
+      off += bond_vol(vol, p,
+                      -((K1 * D * bv / pow(xi, ex1)) +
+                        (K2 * D * bv * zk * p->F / (p->R * p->T) * (phin[nn] - phin[i]) /
+                         pow(xi, ex2))) * be, nn) * in[nn];
+    }
+    double un[NNPP_MAX_SPECIES];
+    for (int q = 0; q < S; q++) un[q] = Un[(size_t)q * n + i];
+    const double d = (1. / p->dt) - sum - jac_reaction(f, p, phase[i], intf, un);
+    out[i] = d * in[i] + off;
+    if (diag) diag[i] = d;
+  }
+}
+
+/*
+ * utils.h:844-1849 as an operator: out_f = K_f in_f for every field f (the reference's 11
+ * matrices are block-diagonal in the fields), diag (optional) = the diagonals. The Jacobian is
+ * frozen at U^n: its off-diagonal entries use the lagged potential U^n_S, its diagonal
+ * 1/dt - sum_K - R'_k the reaction derivatives at U^n. Dirichlet rows are identity rows.
+ */
+int or_jacobian_apply(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                      const double* z, const double* vol, const int64_t* row_ptr,
+                      const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                      const double* bcvalues, const double* Un, double voln1rel,
+                      const double* in, double* out, double* diag) {
+  const int S = p->n_species;
+  if (S < 1 || S > NNPP_MAX_SPECIES) return NNPP_ERR_ARG;
+  if (p->chemistry == NNPP_CHEM_REFERENCE && S != 10) return NNPP_ERR_UNSUPPORTED;
+  for (int f = 0; f <= S; f++)
+    jac_field(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, Un, voln1rel, f,
+              in + (size_t)f * n, out + (size_t)f * n, diag ? diag + (size_t)f * n : NULL);
+  return NNPP_OK;
+}
+
+/* Jacobi-preconditioned BiCGSTAB on block f from x = 0 (the reference's GMRES also starts from
+ * zero), stopping at ||b - K x||_2 <= rtol ||b||_2 or after maxit iterations. */
+static int jac_solve(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                     const double* z, const double* vol, const int64_t* row_ptr,
+                     const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                     const double* bcvalues, const double* Un, double voln1rel, int f,
+                     const double* b, double* xs, double rtol, int32_t maxit, int32_t* iters) {
+  double* dg = (double*)malloc(sizeof(double) * (size_t)n);
+  double* w = (double*)malloc(sizeof(double) * (size_t)n * 8);
+  double *r = w, *rh = w + n, *pp = w + 2 * (size_t)n, *v = w + 3 * (size_t)n,
+         *s = w + 4 * (size_t)n, *t = w + 5 * (size_t)n, *ph = w + 6 * (size_t)n,
+         *sh = w + 7 * (size_t)n;
+  jac_field(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, Un, voln1rel, f, b, v,
+            dg);
+  for (int32_t i = 0; i < n; i++) {
+    xs[i] = 0.0;
+    r[i] = rh[i] = b[i];
+    pp[i] = v[i] = 0.0;
+  }
+  const double bn = sqrt(dot_(n, b, b));
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  int it = 0, status = NNPP_OK;
+  if (bn > 0.0)
+    while (1) {
+      if (sqrt(dot_(n, r, r)) <= rtol * bn) break;
+      if (it >= maxit) {
+        status = NNPP_ERR_NOCONV;
+        break;
+      }
+      const double rho1 = dot_(n, rh, r);
+      if (rho1 == 0.0) break;
+      const double beta = (rho1 / rho) * (alpha / omega);
+      rho = rho1;
+      for (int32_t i = 0; i < n; i++) {
+        pp[i] = r[i] + beta * (pp[i] - omega * v[i]);
+        ph[i] = pp[i] / dg[i];
+      }
+      jac_field(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, Un, voln1rel, f, ph,
+                v, NULL);
+      const double rv = dot_(n, rh, v);
+      if (rv == 0.0) break;
+      alpha = rho / rv;
+      for (int32_t i = 0; i < n; i++) {
+        s[i] = r[i] - alpha * v[i];
+        sh[i] = s[i] / dg[i];
+      }
+      it++;
+      if (sqrt(dot_(n, s, s)) <= rtol * bn) {
+        for (int32_t i = 0; i < n; i++) {
+          xs[i] += alpha * ph[i];
+          r[i] = s[i];
+        }
+        break;
+      }
+      jac_field(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, Un, voln1rel, f, sh,
+                t, NULL);
+      const double tt = dot_(n, t, t);
+      omega = tt > 0.0 ? dot_(n, t, s) / tt : 0.0;
+      for (int32_t i = 0; i < n; i++) {
+        xs[i] += alpha * ph[i] + omega * sh[i];
+        r[i] = s[i] - omega * t[i];
+      }
+      if (omega == 0.0) break;
+    }
+  if (iters) *iters += it;
+  free(w);
+  free(dg);
+  return status;
+}
+
+/*
+ * One implicit step of main.cpp:547-1025 (modified Newton with the Jacobian of utils.h:844-1849
+ * frozen at U^n):
+ *   iterate: first != 0 -> U^n with the Dirichlet values (main.cpp:556-640), else Uit as left
+ *            by the previous step's rotation;
+ *   per Newton iteration (at most newton_maxit, main.cpp:665): F = Residuals(Uit, U^n)
+ *            (utils.h:27-842) with the Dirichlet rows F = U - bc (main.cpp:676-711); per field
+ *            K_f dU_f = -F_f (jac_solve); Uit += dU; res_f = ||dU_f||_2; clamp
+ *            (main.cpp:805-837); stop when max_f |res_f - res_f,prev| < newton_tol
+ *            (main.cpp:863-911, res_prev = 0 at the first iteration);
+ *   front update on the iterate (main.cpp:915-922), voln1rel every sub_steps steps
+ *            (main.cpp:944-949);
+ *   rotation (main.cpp:1002-1024): Uprev <- U^n, U <- U^{n+1}, Uit <- old Uprev.
+ * newton_iters: Newton iterations of the step; lin_iters: BiCGSTAB iterations summed over the
+ * fields; converged: 1 when the Newton test passed.
+ */
+int or_step_implicit(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                     const double* z, const double* vol, const int64_t* row_ptr,
+                     const int32_t* col, const int32_t* bctypes, const double* bcvalues,
+                     double* U, double* Uprev, double* Uit, int32_t first, int32_t* phase,
+                     int64_t* step, double* voln1rel, double vol0rel, int64_t* solid_count,
+                     double newton_tol, int32_t newton_maxit, double lin_rtol, int32_t lin_maxit,
+                     int32_t* newton_iters, int32_t* lin_iters, int32_t* converged,
+                     double* resmax_out, int32_t* flips) {
+  const int S = p->n_species;
+  if (S < 1 || S > NNPP_MAX_SPECIES) return NNPP_ERR_ARG;
+  if (p->chemistry == NNPP_CHEM_REFERENCE && S != 10) return NNPP_ERR_UNSUPPORTED;
+  const size_t nF = (size_t)(S + 1) * n;
+  if (first) {
+    memcpy(Uit, U, sizeof(double) * nF);
+    if (bctypes && bcvalues)
+      for (int32_t i = 0; i < n; i++) {
+        if (bctypes[i] != 0) continue;
+        for (int f = 0; f <= S; f++) {
+          const double v = bcvalues[(size_t)f * n + i];
+          if (v != -1) Uit[(size_t)f * n + i] = (f < S) ? v / p->CSolid : v;
+        }
+      }
+  }
+  double* Fv = (double*)malloc(sizeof(double) * nF);
+  double* bvec = (double*)malloc(sizeof(double) * (size_t)n);
+  double* dU = (double*)malloc(sizeof(double) * (size_t)n);
+  double res_old[NNPP_MAX_SPECIES + 1], res[NNPP_MAX_SPECIES + 1];
+  for (int f = 0; f <= S; f++) res_old[f] = 0.0;
+  int32_t nit = 0, lit = 0, conv = 0;
+  double resmax = 0.0;
+  int status = NNPP_OK;
+  for (int it = 0; it < newton_maxit; it++) {
+    int st = or_residuals(n, p, x, y, z, vol, row_ptr, col, phase, Uit, U, *voln1rel, Fv);
+    if (st) {
+      status = st;
+      break;
+    }
+    if (bctypes && bcvalues) or_bc_override(n, p, bctypes, bcvalues, Uit, Fv);
+    for (int f = 0; f <= S; f++) {
+      for (int32_t i = 0; i < n; i++) bvec[i] = -Fv[(size_t)f * n + i];
+      const int ls = jac_solve(n, p, x, y, z, vol, row_ptr, col, phase, bctypes, bcvalues, U,
+                               *voln1rel, f, bvec, dU, lin_rtol, lin_maxit, &lit);
+      if (ls == NNPP_ERR_NOCONV) status = NNPP_ERR_NOCONV;
+      double* u = Uit + (size_t)f * n;
+      for (int32_t i = 0; i < n; i++) u[i] += dU[i];
+      res[f] = sqrt(dot_(n, dU, dU));
+    }
+    for (int k = 0; k < S; k++) {
+      double* u = Uit + (size_t)k * n;
+      for (int32_t i = 0; i < n; i++)
+        if ((p->clamp[k] == NNPP_CLAMP_ALWAYS ||
+             (p->clamp[k] == NNPP_CLAMP_LIQUID && phase[i] == 0)) &&
+            u[i] < p->clamp_floor)
+          u[i] = p->clamp_floor;
+    }
+    nit++;
+    resmax = 0.0;
+    for (int f = 0; f <= S; f++) {
+      const double d = fabs(res[f] - res_old[f]);
+      if (d > resmax) resmax = d;
+    }
+    if (resmax < newton_tol) {
+      conv = 1;
+      break;
+    }
+    for (int f = 0; f <= S; f++) res_old[f] = res[f];
+  }
+  int32_t fl = 0;
+  for (int32_t i = 0; i < n; i++)
+    if (phase[i] == 1 && Uit[i] <= p->front_threshold) {
+      phase[i] = 0;
+      fl++;
+    }
+  if (solid_count) *solid_count -= fl;
+  if (p->sub_steps > 0 && (*step % p->sub_steps) == 0) {
+    double v = 0.0;
+    for (int32_t i = 0; i < n; i++)
+      if (phase[i] == 1) v += (Uit[i] - p->CSat / p->CSolid);
+    *voln1rel = v / vol0rel;
+  }
+  (*step)++;
+  /* rotation: (Uprev, U, Uit) <- (U, Uit, Uprev) */
+  for (size_t q = 0; q < nF; q++) {
+    const double a = Uprev[q];
+    Uprev[q] = U[q];
+    U[q] = Uit[q];
+    Uit[q] = a;
+  }
+  if (newton_iters) *newton_iters = nit;
+  if (lin_iters) *lin_iters = lit;
+  if (converged) *converged = conv;
+  if (resmax_out) *resmax_out = resmax;
+  if (flips) *flips = fl;
+  free(Fv);
+  free(bvec);
+  free(dU);
+  return status;
+}

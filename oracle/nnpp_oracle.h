@@ -60,6 +60,28 @@ int or_step(int32_t n, const nnpp_params* p, const double* x, const double* y, c
             int64_t* step, double* voln1rel, double vol0rel, int64_t* solid_count, double rtol,
             int32_t maxit, int32_t* cg_iters, double* relres, int32_t* flips);
 
+int or_adjoint(int32_t n, const nnpp_params* p, const double* x, const double* y, const double* z,
+               const double* vol, const int64_t* row_ptr, const int32_t* col,
+               const int32_t* bctypes, const double* bcvalues, const double* U0,
+               const int32_t* phase0, int64_t step0, double voln1rel0, double vol0rel,
+               int32_t nsteps, double rtol, int32_t maxit, const double* lamN, double* grad,
+               double* lam0, int32_t* steps_done);
+
+int or_jacobian_apply(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                      const double* z, const double* vol, const int64_t* row_ptr,
+                      const int32_t* col, const int32_t* phase, const int32_t* bctypes,
+                      const double* bcvalues, const double* Un, double voln1rel,
+                      const double* in, double* out, double* diag);
+
+int or_step_implicit(int32_t n, const nnpp_params* p, const double* x, const double* y,
+                     const double* z, const double* vol, const int64_t* row_ptr,
+                     const int32_t* col, const int32_t* bctypes, const double* bcvalues,
+                     double* U, double* Uprev, double* Uit, int32_t first, int32_t* phase,
+                     int64_t* step, double* voln1rel, double vol0rel, int64_t* solid_count,
+                     double newton_tol, int32_t newton_maxit, double lin_rtol, int32_t lin_maxit,
+                     int32_t* newton_iters, int32_t* lin_iters, int32_t* converged,
+                     double* resmax_out, int32_t* flips);
+
 #ifdef __cplusplus
 }
 #endif

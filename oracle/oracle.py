@@ -55,6 +55,11 @@ class Oracle:
                                            _dp, _dp, _d, _i32, _ip, _dp]
         L.or_adjoint.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _dp, _dp, _ip,
                                  C.c_int64, _d, _d, _i32, _d, _i32, _dp, _dp, _dp, _ip]
+        L.or_jacobian_apply.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _ip, _dp,
+                                        _dp, _d, _dp, _dp, _dp]
+        L.or_step_implicit.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _dp, _dp, _dp,
+                                       _dp, _i32, _ip, _lp, _dp, _d, _lp, _d, _i32, _d, _i32,
+                                       _ip, _ip, _ip, _dp, _ip]
 
     def beta(self, xi, delta, dx):
         return self.lib.or_beta(xi, delta, dx)
@@ -137,6 +142,59 @@ class Oracle:
                                  maxit, _d_(np.ascontiguousarray(lamN, dtype=np.float64)),
                                  _d_(grad), _d_(lam0), C.byref(done))
         return grad, lam0, int(done.value), st
+
+    def jacobian_apply(self, case, rp, col, phase, Un, v, voln1rel=0.0, want_diag=False):
+        """Frozen Jacobian of utils.h:844-1849 at Un applied field by field to v (F, n));
+        returns (K v, diagonals or None)."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros_like(v)
+        dg = np.zeros_like(v) if want_diag else None
+        st = self.lib.or_jacobian_apply(case.n, C.byref(case.params), _d_(case.x), _d_(case.y),
+                                        _d_(case.z), _d_(case.vol), _l_(rp), _i_(col),
+                                        _i_(np.ascontiguousarray(phase, dtype=np.int32)),
+                                        _i_(case.bctypes), _d_(case.bcvalues),
+                                        _d_(np.ascontiguousarray(Un)), voln1rel, _d_(v),
+                                        _d_(out), _d_(dg))
+        assert st == 0, st
+        return out, dg
+
+    def run_implicit(self, case, rp, col, nsteps, state=None, newton_tol=1e-5, newton_maxit=200,
+                     lin_rtol=1e-5, lin_maxit=200):
+        """nsteps implicit (reference Newton) steps from the initial state (or `state`, which
+        then also carries the rotation buffers "Uprev", "Uit" and the flag "first")."""
+        if state is None:
+            U, ph, sc, v0 = self.init_state(case)
+            st_ = {"U": U, "phase": ph, "step": 1, "voln1rel": 0.0, "vol0rel": v0, "solid": sc}
+        else:
+            st_ = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in state.items()}
+        st_.setdefault("Uprev", np.zeros_like(st_["U"]))
+        st_.setdefault("Uit", np.zeros_like(st_["U"]))
+        st_.setdefault("first", 1)
+        step = C.c_int64(st_["step"])
+        vr = C.c_double(st_["voln1rel"])
+        sc = C.c_int64(st_["solid"])
+        nit, lit, conv, fl = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        rm = C.c_double()
+        log = []
+        for _ in range(nsteps):
+            s = self.lib.or_step_implicit(
+                case.n, C.byref(case.params), _d_(case.x), _d_(case.y), _d_(case.z),
+                _d_(case.vol), _l_(rp), _i_(col), _i_(case.bctypes), _d_(case.bcvalues),
+                _d_(st_["U"]), _d_(st_["Uprev"]), _d_(st_["Uit"]), int(st_["first"]),
+                _i_(st_["phase"]), C.byref(step), C.byref(vr), st_["vol0rel"], C.byref(sc),
+                newton_tol, newton_maxit, lin_rtol, lin_maxit, C.byref(nit), C.byref(lit),
+                C.byref(conv), C.byref(rm), C.byref(fl))
+            assert s in (0, 4), s
+            st_["first"] = 0
+            log.append({"newton": nit.value, "lin": lit.value, "converged": conv.value,
+                        "resmax": rm.value, "flips": fl.value})
+            if sc.value <= 0:
+                break
+        st_["step"] = step.value
+        st_["voln1rel"] = vr.value
+        st_["solid"] = sc.value
+        st_["log"] = log
+        return st_
 
     def run(self, case, rp, col, nsteps, rtol=1e-10, maxit=100000, state=None):
         """nsteps explicit steps from the initial state (or from `state`)."""

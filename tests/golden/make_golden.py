@@ -29,11 +29,24 @@ def main():
         kr, kc, kv = R.newton_matrix(case, ell, ln, phase, U1, Un, 11, voln1rel=voln1rel)
         beta_xi = np.linspace(0.0, 4.0 * p.dx, 41)
         beta = np.array([R.lib.ref_beta(v, p.delta, p.dx) for v in beta_xi])
+        # every block K_1..K_11 of NewtonMatrices applied to two seeded random vectors, and its
+        # diagonal (the full matrices stay out of the fixture)
+        rng = np.random.default_rng(100 + seed)
+        jac_v = rng.uniform(-1.0, 1.0, size=(2, 11, case.n))
+        jac_kv = np.zeros((11, 2, case.n))
+        jac_diag = np.zeros((11, case.n))
+        for f in range(11):
+            r_, c_, v_ = R.newton_matrix(case, ell, ln, phase, U1, Un, f + 1, voln1rel=voln1rel)
+            for m in range(2):
+                np.add.at(jac_kv[f, m], r_, v_ * jac_v[m, f, c_])
+            d = r_ == c_
+            np.add.at(jac_diag[f], r_[d], v_[d])
         np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), x=case.x, y=case.y, z=case.z,
                             bctypes=case.bctypes, bcvalues=case.bcvalues, phase=phase, U1=U1,
                             Un=Un, voln1rel=voln1rel, row_ptr=rp, col=col, max_len=ml,
                             lengths=ln, F=F, K11_rows=kr, K11_cols=kc, K11_vals=kv,
-                            beta_xi=beta_xi, beta=beta)
+                            beta_xi=beta_xi, beta=beta, jac_v=jac_v, jac_kv=jac_kv,
+                            jac_diag=jac_diag)
         print(name, case.n, int(rp[-1]), ml)
 
 
