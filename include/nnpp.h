@@ -299,6 +299,73 @@ int nnpp_poisson_solve(nnpp_ctx* ctx, int32_t precision, int32_t* iters, double*
 int nnpp_step(nnpp_ctx* ctx, int32_t nsteps, int32_t precision, int32_t check_finite,
               nnpp_step_stats* stats);
 
+/* -------------------------------------------------------- implicit (reference) step ----
+ * The reference's own time step, main.cpp:547-1025: modified Newton with the Jacobian of
+ * utils::NewtonMatrices (utils.h:844-1849) frozen at U^n for the whole step, every field solved
+ * on its own block K_f dU_f = -F_f. Per step:
+ *   - iterate: on the first implicit step after the state was set, U^n with the Dirichlet
+ *     values (main.cpp:556-640); afterwards, with start = NNPP_NEWTON_START_ROTATE, the buffer
+ *     the reference's rotation (main.cpp:1002-1024) leaves there: U^{n-1}, which is all zeros on
+ *     the second step (U1n0.setZero(), main.cpp:388); with NNPP_NEWTON_START_STATE, U^n with
+ *     the Dirichlet values every step;
+ *   - at most newton_maxit iterations (MAX_ITER = 200, main.cpp:36) of: F = utils::Residuals
+ *     (iterate, U^n) with the Dirichlet rows F = U - bc (main.cpp:667-711); species blocks
+ *     solved together by Jacobi-preconditioned BiCGSTAB, the potential block by the
+ *     potential CG (both from zero, to ||b - K x|| <= lin_rtol ||b||; the reference runs
+ *     GMRES(30) with a diagonal preconditioner to EPS = 1e-5, main.cpp:714-779); U += dU;
+ *     res_f = ||dU_f||_2; the clamp (main.cpp:805-837); stop once
+ *     max_f |res_f - res_f of the previous iteration| < newton_tol (main.cpp:863-911; the
+ *     previous value is 0 at the first iteration);
+ *   - the front update on the iterate (main.cpp:915-922), voln1rel every sub_steps steps
+ *     (main.cpp:944-949), the step counter, and the rotation (U^{n-1}, U^n, iterate) <-
+ *     (U^n, iterate, U^{n-1}); slot N then holds U^{n+1} and slot N1 the next start.
+ * Species solves are FP64; precision selects the potential solve (NNPP_FP64 or NNPP_MIXED). */
+#define NNPP_NEWTON_START_ROTATE 0
+#define NNPP_NEWTON_START_STATE 1
+typedef struct nnpp_newton_opts {
+  double newton_tol;     /* EPS = 1e-5 (main.cpp:35) */
+  double lin_rtol;       /* relative residual of every linear solve (default 1e-5 = EPS) */
+  int32_t newton_maxit;  /* MAX_ITER = 200 */
+  int32_t lin_maxit;     /* MAX_ITER = 200 */
+  int32_t start;         /* NNPP_NEWTON_START_ROTATE (default) or NNPP_NEWTON_START_STATE */
+  int32_t reserved;
+} nnpp_newton_opts;
+
+typedef struct nnpp_newton_stats {
+  int32_t steps;
+  int32_t newton_iterations;  /* summed over the steps */
+  int32_t unconverged_steps;  /* steps that ran out of Newton iterations */
+  int32_t lin_iterations;     /* BiCGSTAB iterations (the species advance together) */
+  int32_t cg_iterations;      /* potential CG iterations */
+  int32_t flipped;            /* solid -> liquid flips */
+  int64_t solid_count;
+  double voln1rel;
+  double last_resmax;         /* Newton test value of the last iteration */
+  double lin_relres_max;      /* largest final relative residual of a species solve */
+} nnpp_newton_stats;
+
+/* Fill opts with the reference's values. */
+int nnpp_newton_defaults(nnpp_newton_opts* opts);
+
+/* Advance nsteps implicit steps. opts NULL: the reference's values; stats may be NULL.
+ * Returns NNPP_ERR_NOCONV when a linear solve stops at its iteration limit (the step is still
+ * taken, as the reference takes it), NNPP_ERR_NONFINITE if check_finite != 0 and the state
+ * holds NaN/Inf, NNPP_ERR_UNSUPPORTED while a differentiation tape is recording. Replaces the
+ * body of the time loop main.cpp:547-1025 (Jacobian, Newton loop, front, volume). */
+int nnpp_step_implicit(nnpp_ctx* ctx, int32_t nsteps, int32_t precision, int32_t check_finite,
+                       const nnpp_newton_opts* opts, nnpp_newton_stats* stats);
+
+/* The frozen Jacobian at slot N (utils::NewtonMatrices, utils.h:844-1849) applied block by
+ * block: out_f = K_f in_f for every field (F*n doubles each, field-major); diag (F*n, may be
+ * NULL) receives the diagonals. Species blocks: diagonal 1/dt + sum of the diffusion weights
+ * - R'_k (R' at U^n), off-diagonals -(diffusion + migration with the lagged phi^n) by phase
+ * pair; the potential block is K11 including its identity rows; Dirichlet rows are identity
+ * rows. The reference assembles eleven Eigen sparse matrices; here they are never stored. */
+int nnpp_jacobian_apply(nnpp_ctx* ctx, double voln1rel, const double* in, double* out,
+                        double* diag);
+
+int64_t nnpp_abi_newton_stats_size(void);
+
 /* Front update alone (main.cpp:915-922) on slot N. */
 int nnpp_front_update(nnpp_ctx* ctx, double threshold, int32_t* n_flipped);
 

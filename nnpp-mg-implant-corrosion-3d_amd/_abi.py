@@ -91,6 +91,29 @@ class AdjointStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+NEWTON_START_ROTATE = 0
+NEWTON_START_STATE = 1
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [
+        ("newton_tol", C.c_double), ("lin_rtol", C.c_double), ("newton_maxit", C.c_int32),
+        ("lin_maxit", C.c_int32), ("start", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32), ("newton_iterations", C.c_int32),
+        ("unconverged_steps", C.c_int32), ("lin_iterations", C.c_int32),
+        ("cg_iterations", C.c_int32), ("flipped", C.c_int32), ("solid_count", C.c_int64),
+        ("voln1rel", C.c_double), ("last_resmax", C.c_double), ("lin_relres_max", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class NnppError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"nnpp error {code}: {msg}")
@@ -146,6 +169,11 @@ SIGNATURES = {
     "nnpp_tape_adjoint": (C.c_int, [_p, _dp, _dp, _dp, C.POINTER(AdjointStats)]),
     "nnpp_tape_end": (C.c_int, [_p]),
     "nnpp_abi_adjoint_stats_size": (C.c_int64, []),
+    "nnpp_newton_defaults": (C.c_int, [C.POINTER(NewtonOpts)]),
+    "nnpp_step_implicit": (C.c_int, [_p, _i32, _i32, _i32, C.POINTER(NewtonOpts),
+                                     C.POINTER(NewtonStats)]),
+    "nnpp_jacobian_apply": (C.c_int, [_p, _d, _dp, _dp, _dp]),
+    "nnpp_abi_newton_stats_size": (C.c_int64, []),
 }
 
 _lib = None

@@ -20,6 +20,12 @@
 //         clamp of main.cpp:805-837.
 // MIXED: FP32 weights, FP32 differences taken from the FP64 state, Kahan-compensated FP32 sums
 // per node and species; reactions, the update and the output stay FP64.
+// JAC: the species blocks of the frozen Jacobian of utils.h:844-1849 applied to v (the gathered
+//      record holds v instead of the iterate): out_k = d0_k v_k + sum_KR_k1(v) + sum_KR_k2(v)
+//      with d0_k = 1/dt - R'_k (R' at U^n), identity rows where field k is Dirichlet. The
+//      bond sums are linear in v with the lagged phi^n, so they are the Jacobian's off-diagonal
+//      and diffusion-diagonal terms exactly. With d0_out set the kernel also computes d0 from
+//      U^n and the diagonal d0 + sum of the diffusion weights (the Jacobi preconditioner).
 #include "nnpp_internal.hpp"
 
 namespace {
@@ -54,6 +60,9 @@ struct FluxArgs {
   double* out;
   uint16_t* passmask;  // explicit update: bit k set where U_k^{n+1} is not a Dirichlet or clamped
                        // value (the derivative passes); nullptr: not written
+  const double* d0;    // JAC: 1/dt - R'_k per species and node (S*n)
+  double* d0_out;      // JAC setup: d0 written here
+  double* diag_out;    // JAC setup: diagonal of each species block (1 on Dirichlet rows)
   double A_LL[NNPP_MAX_SPECIES], A_SL[NNPP_MAX_SPECIES], A_SS[NNPP_MAX_SPECIES];
   double M_LL[NNPP_MAX_SPECIES], M_SL[NNPP_MAX_SPECIES], M_SS[NNPP_MAX_SPECIES];
   double z[NNPP_MAX_SPECIES];
@@ -138,6 +147,29 @@ __device__ double ref_reaction(int k, const RxConst& c, int phase, int intf, con
   return R;
 }
 
+// R'_k of the frozen Jacobian's diagonal at U^n (reference chemistry: utils.h:1001-1006,
+// 1089-1100, 1184-1190, 1274-1279, 1364-1369, 1454-1459, 1545-1550, 1635-1640; none for Na+
+// and Cl-). Generic chemistry: the derivative of its sink, -r_diss at interface solid nodes.
+__device__ double jac_reaction(int k, int chemistry, const RxConst& c, double r_diss, int phase,
+                               int intf, const double* un) {
+  const int liquid = (phase == 0);
+  if (chemistry != NNPP_CHEM_REFERENCE) return (k == 0 && intf == 1) ? -r_diss : 0.0;
+  switch (k) {
+    case 1:
+      return (intf == 1 || liquid) ? c.k7b * (-un[2]) + c.k8b * (-un[4]) + c.k9b * (-un[6])
+                                   : 0.0;
+    case 2:
+      return (intf != 1 && liquid) ? c.k7b * (-un[1]) : 0.0;
+    case 3:
+    case 4:
+      return liquid ? c.k8b * (-un[1]) : 0.0;
+    case 6:
+      return liquid ? c.k9b * (-un[1]) : 0.0;
+    default:
+      return 0.0;
+  }
+}
+
 // Gather record of node j: [c_0 .. c_{S-1} (iterate), phi^n (lagged), phi^{n+1} (RESID)],
 // padded to an even number of doubles so every record is 16-byte aligned.
 template <int S, bool RESID>
@@ -165,8 +197,9 @@ __global__ void k_pack(int32_t n, const double* __restrict__ U1, const double* _
   for (int k = 0; k < R / 2; k++) out[k] = make_double2(v[2 * k], v[2 * k + 1]);
 }
 
-template <int S, bool RESID, bool MIXED, bool WIDE>
+template <int S, bool RESID, bool MIXED, bool WIDE, bool JAC = false>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
+  static_assert(!JAC || (!RESID && !MIXED), "the Jacobian apply is FP64 and not a residual");
   const int64_t i = nnpp_xcd_block(blockIdx.x, gridDim.x) * NNPP_BLOCK + threadIdx.x;
   if (i >= a.n) return;
   constexpr int R = Rec<S, RESID>::len;
@@ -197,6 +230,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
   int mixed = 0;
   double sphi = 0.0;
   int flagLL = 0;
+  double sw[2] = {0.0, 0.0};  // JAC: weight sums of the two passes (diffusion diagonal)
 
   // pass 0: same-phase bonds (branch-free: other bonds get weight 0); pass 1 (interface nodes
   // only): solid-liquid bonds
@@ -264,6 +298,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
           v[2 * k + 1] = t.y;
         }
         const double dphi = v[S] - phii;
+        if (JAC) sw[pass] += w;
         if (MIXED) {
           const float wf = (float)w, dpf = (float)dphi;
 #pragma unroll
@@ -306,10 +341,29 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
   const double* __restrict__ Un = a.Un;
   double un[S];
 #pragma unroll
-  for (int k = 0; k < S; k++) un[k] = RESID ? Un[(int64_t)k * n + i] : ci[k];
+  for (int k = 0; k < S; k++)
+    un[k] = (RESID || (JAC && a.d0_out)) ? Un[(int64_t)k * n + i] : ci[k];
   const unsigned bm = a.bcmask[i];
   unsigned pass_bits = 0;
   (void)U1;
+  if constexpr (JAC) {
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      const bool dir = (bm >> k) & 1u;
+      double d0;
+      if (a.d0_out) {
+        d0 = (1. / a.dt) - jac_reaction(k, a.chemistry, a.rx, a.r_diss, pi, intf, un);
+        a.d0_out[(int64_t)k * n + i] = d0;
+        const double A0 = pi ? a.A_SS[k] : a.A_LL[k];
+        const double dg = d0 + (A0 * sw[0] + a.A_SL[k] * sw[1]);
+        a.diag_out[(int64_t)k * n + i] = dir ? 1.0 : dg;
+      } else {
+        d0 = a.d0[(int64_t)k * n + i];
+      }
+      a.out[(int64_t)k * n + i] = dir ? ci[k] : d0 * ci[k] - acc[k];
+    }
+    return;
+  }
 
 #pragma unroll
   for (int k = 0; k < S; k++) {
@@ -365,13 +419,23 @@ hipError_t launch_s(const FluxArgs& a, int resid, int mixed, int wide, int64_t n
   return hipGetLastError();
 }
 
+template <int S>
+hipError_t launch_jac(const FluxArgs& a, int wide, int64_t nb, hipStream_t s) {
+  if (wide) k_flux<S, false, false, true, true><<<nb, NNPP_BLOCK, 0, s>>>(a);
+  else k_flux<S, false, false, false, true><<<nb, NNPP_BLOCK, 0, s>>>(a);
+  return hipGetLastError();
+}
+
 }  // namespace
 
-int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
-                     const double* Un, double* out, uint16_t* passmask) {
+static FluxArgs flux_args(nnpp_ctx* c, double BV, const double* U1, const double* Un,
+                          double* out, uint16_t* passmask) {
   const nnpp_params& p = c->p;
   const int S = p.n_species;
   FluxArgs a;
+  a.d0 = nullptr;
+  a.d0_out = nullptr;
+  a.diag_out = nullptr;
   a.n = c->n;
   a.chemistry = p.chemistry;
   a.off = c->bonds.off;
@@ -438,7 +502,13 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   r.F = p.F;
   r.R = p.R;
   r.T = p.T;
+  return a;
+}
 
+int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
+                     const double* Un, double* out, uint16_t* passmask) {
+  const int S = c->p.n_species;
+  const FluxArgs a = flux_args(c, BV, U1, Un, out, passmask);
   if (!c->n) return NNPP_OK;
   const int64_t nb = nnpp_nblocks(c->n);
   const int mixed = (precision == NNPP_MIXED);
@@ -446,6 +516,30 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   switch (S) {
 #define NNPP_CASE(SS) \
   case SS: e = launch_s<SS>(a, resid, mixed, c->bonds.wide, nb, c->stream); break;
+    NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
+    NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
+    NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)
+#undef NNPP_CASE
+    default:
+      return nnpp_set_error(c, NNPP_ERR_ARG, "n_species out of range");
+  }
+  if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
+  return NNPP_OK;
+}
+
+int nnpp_launch_jac(nnpp_ctx* c, double BV, const double* Un, const double* d0, double* out,
+                    double* d0_out, double* diag_out) {
+  const int S = c->p.n_species;
+  FluxArgs a = flux_args(c, BV, nullptr, Un, out, nullptr);
+  a.d0 = d0;
+  a.d0_out = d0_out;
+  a.diag_out = diag_out;
+  if (!c->n) return NNPP_OK;
+  const int64_t nb = nnpp_nblocks(c->n);
+  hipError_t e = hipSuccess;
+  switch (S) {
+#define NNPP_CASE(SS) \
+  case SS: e = launch_jac<SS>(a, c->bonds.wide, nb, c->stream); break;
     NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
     NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
     NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)

@@ -366,6 +366,31 @@ __global__ void k_pot_apply_csr(int32_t n, const int64_t* __restrict__ row_ptr,
   out[i] = acc;
 }
 
+// K11 of utils.h:1794-1848 with its identity rows (Dirichlet rows, rows without a liquid-liquid
+// bond): out = K11 in, diag = its diagonal. w = Beta(xi) V_j / xi^(3+2 s_phi).
+__global__ void k_jac_phi(int32_t n, int S, const int64_t* __restrict__ row_ptr,
+                          const int32_t* __restrict__ col, const uint8_t* __restrict__ phase,
+                          const uint16_t* __restrict__ bcmask, const double* __restrict__ x,
+                          const double* __restrict__ y, const double* __restrict__ z,
+                          WeightArgs wa, const double* __restrict__ in, double* __restrict__ out,
+                          double* __restrict__ diag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0, d = 0.0;
+  int flag = 0;
+  if (phase[i] == 0 && !((bcmask[i] >> S) & 1u))
+    for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
+      const int32_t j = col[b];
+      if (phase[j] != 0) continue;
+      const double w = bond_weight(wa, x, y, z, i, j);
+      acc += w * (in[i] - in[j]);
+      d += w;
+      flag = 1;
+    }
+  out[i] = flag ? acc : in[i];
+  if (diag) diag[i] = flag ? d : 1.0;
+}
+
 constexpr int DICT_HASH = 4096;
 
 // Collect the distinct nonzero FP64 weights in a small open-addressing hash set; gives up
@@ -1009,6 +1034,15 @@ int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad) {
                              c->stream));
   NNPP_HIP(c, hipStreamSynchronize(c->stream));
   *bad = (int32_t)c->h_scal[SC_R3];
+  return NNPP_OK;
+}
+
+int nnpp_jac_phi_dev(nnpp_ctx* c, const double* in, double* out, double* diag) {
+  if (!c->n) return NNPP_OK;
+  k_jac_phi<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(
+      c->n, c->p.n_species, c->row_ptr, c->col, c->phase, c->bcmask, c->x, c->y, c->z,
+      weight_args(c, 3.0 + 2.0 * c->p.s_phi), in, out, diag);
+  NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }
 

@@ -56,6 +56,7 @@ void free_nodes(nnpp_ctx* c) {
   dfree(c->U[0]);
   dfree(c->U[1]);
   dfree(c->Unew);
+  nnpp_implicit_free(c);
   dfree(c->Fres);
   dfree(c->aos);
   dfree(c->pfree);
@@ -106,6 +107,16 @@ int ensure_potential(nnpp_ctx* c) {
 }
 
 int64_t fields(const nnpp_ctx* c) { return (int64_t)c->p.n_species + 1; }
+
+// The residual kernel evaluates the phi row (utils.h:798-841) with the species bond weights,
+// so it needs the potential exponent s111 to equal the species exponent (every shipped
+// parameter set uses 0 for both).
+int phi_row_guard(nnpp_ctx* c) {
+  if (c->p.s_phi != c->p.s1[0])
+    return nnpp_set_error(c, NNPP_ERR_UNSUPPORTED,
+                          "residuals need the potential kernel exponent equal to the species'");
+  return NNPP_OK;
+}
 
 // Calls that would change what a recorded step depends on are refused while a tape exists.
 int tape_guard(nnpp_ctx* c) {
@@ -373,8 +384,10 @@ int nnpp_set_state(nnpp_ctx* c, int32_t slot, const double* U) {
   if (st) return st;
   if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
   NNPP_HIP(c, hipSetDevice(c->device));
+  NNPP_HIP(c, hipStreamSynchronize(c->stream));
   if (c->n)
     NNPP_HIP(c, hipMemcpy(c->U[slot], U, sizeof(double) * c->n * fields(c), hipMemcpyHostToDevice));
+  c->imp_first = true;  // an implicit step starts again from U^n with the Dirichlet values
   return NNPP_OK;
 }
 
@@ -481,6 +494,7 @@ int nnpp_residuals(nnpp_ctx* c, double voln1rel, int32_t apply_bc, int32_t preci
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = ensure_bonds(c);
   if (st) return st;
+  if ((st = phi_row_guard(c))) return st;
   const double BV = std::pow(10., -c->p.vol_red * (1. - voln1rel));  // utils.h:100-116
   st = nnpp_launch_flux(c, 1, precision, BV, c->U[NNPP_SLOT_N1], c->U[NNPP_SLOT_N], c->Fres);
   if (st) return st;
@@ -612,6 +626,7 @@ int nnpp_step(nnpp_ctx* c, int32_t nsteps, int32_t precision, int32_t check_fini
   NNPP_HIP(c, hipSetDevice(c->device));
   int st = ensure_bonds(c);
   if (st) return st;
+  c->imp_first = true;  // the implicit rotation buffers no longer follow the state
   nnpp_step_stats loc;
   std::memset(&loc, 0, sizeof(loc));
   int status = NNPP_OK;
@@ -642,6 +657,89 @@ int nnpp_step(nnpp_ctx* c, int32_t nsteps, int32_t precision, int32_t check_fini
   if (stats) *stats = loc;
   return status;
 }
+
+int nnpp_newton_defaults(nnpp_newton_opts* o) {
+  if (!o) return NNPP_ERR_ARG;
+  o->newton_tol = 10E-6;  // EPS, main.cpp:35
+  o->lin_rtol = 10E-6;
+  o->newton_maxit = 200;  // MAX_ITER, main.cpp:36
+  o->lin_maxit = 200;
+  o->start = NNPP_NEWTON_START_ROTATE;
+  o->reserved = 0;
+  return NNPP_OK;
+}
+
+int nnpp_step_implicit(nnpp_ctx* c, int32_t nsteps, int32_t precision, int32_t check_finite,
+                       const nnpp_newton_opts* opts, nnpp_newton_stats* stats) {
+  if (!c || nsteps < 0) return NNPP_ERR_ARG;
+  if (precision != NNPP_FP64 && precision != NNPP_MIXED) return NNPP_ERR_ARG;
+  nnpp_newton_opts o;
+  nnpp_newton_defaults(&o);
+  if (opts) o = *opts;
+  if (!(o.newton_tol >= 0) || !(o.lin_rtol >= 0) || o.newton_maxit < 0 || o.lin_maxit < 0 ||
+      (o.start != NNPP_NEWTON_START_ROTATE && o.start != NNPP_NEWTON_START_STATE))
+    return nnpp_set_error(c, NNPP_ERR_ARG, "invalid Newton options");
+  if (c->tape.active)
+    return nnpp_set_error(c, NNPP_ERR_UNSUPPORTED, "a differentiation tape records explicit steps only");
+  NNPP_HIP(c, hipSetDevice(c->device));
+  int st = ensure_bonds(c);
+  if (st) return st;
+  if ((st = phi_row_guard(c))) return st;
+  nnpp_newton_stats loc;
+  std::memset(&loc, 0, sizeof(loc));
+  int status = NNPP_OK;
+  for (int32_t k = 0; k < nsteps; k++) {
+    if ((st = ensure_potential(c))) return st;
+    st = nnpp_implicit_once(c, precision, o, loc);
+    if (st == NNPP_ERR_NOCONV) status = NNPP_ERR_NOCONV;
+    else if (st) {
+      if (stats) *stats = loc;
+      return st;
+    }
+    if (c->solid_count <= 0) break;  // main.cpp:924-932
+  }
+  loc.solid_count = c->solid_count;
+  loc.voln1rel = c->voln1rel;
+  if (check_finite) {
+    int32_t bad = 0;
+    st = nnpp_check_finite_dev(c, c->U[NNPP_SLOT_N], &bad);
+    if (st) return st;
+    if (bad) status = nnpp_set_error(c, NNPP_ERR_NONFINITE, "non-finite values in the state");
+  }
+  NNPP_HIP(c, hipStreamSynchronize(c->stream));
+  if (stats) *stats = loc;
+  return status;
+}
+
+int nnpp_jacobian_apply(nnpp_ctx* c, double voln1rel, const double* in, double* out,
+                        double* diag) {
+  if (!c || ((!in || !out) && c->n)) return NNPP_ERR_ARG;
+  NNPP_HIP(c, hipSetDevice(c->device));
+  int st = ensure_bonds(c);
+  if (st) return st;
+  if (!c->n) return NNPP_OK;
+  const size_t bytes = sizeof(double) * c->n * fields(c);
+  double* d = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&d, 3 * bytes));
+  double* din = d;
+  double* dout = (double*)((char*)d + bytes);
+  double* ddg = (double*)((char*)d + 2 * bytes);
+  auto done = [&](int e) {
+    hipFree(d);
+    return e;
+  };
+  if (hipMemcpy(din, in, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return done(nnpp_set_error(c, NNPP_ERR_HIP, "copy"));
+  st = nnpp_jacobian_apply_dev(c, voln1rel, din, dout, diag ? ddg : nullptr);
+  if (st) return done(st);
+  if (hipStreamSynchronize(c->stream) != hipSuccess ||
+      hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+      (diag && hipMemcpy(diag, ddg, bytes, hipMemcpyDeviceToHost) != hipSuccess))
+    return done(nnpp_set_error(c, NNPP_ERR_HIP, "copy"));
+  return done(NNPP_OK);
+}
+
+int64_t nnpp_abi_newton_stats_size(void) { return (int64_t)sizeof(nnpp_newton_stats); }
 
 int nnpp_tape_begin(nnpp_ctx* c, int32_t every) {
   if (!c || every < 1) return NNPP_ERR_ARG;

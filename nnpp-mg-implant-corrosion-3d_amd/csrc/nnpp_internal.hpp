@@ -178,6 +178,17 @@ struct nnpp_ctx {
   int32_t maxit = 20000;
   double inner_rtol = 1e-4;
 
+  // implicit (reference Newton) steps, allocated on first use
+  double* imp = nullptr;       // species vectors, S*n each: x r rh p v s t d0 diag
+  double* imp_prev = nullptr;  // F*n: U^{n-1} of the iterate rotation
+  double* imp_dphi = nullptr;  // 3*n: potential update, Dirichlet part, coupling
+  double* imp_part = nullptr;  // [NNPP_RED_GRID * 32] block partials
+  double* imp_red = nullptr;   // [32] totals
+  double* h_imp = nullptr;     // pinned host copy of the totals
+  int64_t imp_n = -1;          // n and S the buffers were allocated for
+  int imp_S = 0;
+  bool imp_first = true;       // next implicit step starts from U^n with the Dirichlet values
+
   int64_t step = 1;
   double voln1rel = 0.0;
   double vol0rel = 0.0;
@@ -372,6 +383,20 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                              int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32,
                              const double* rhs = nullptr);
 int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out);
+// Species blocks of the frozen Jacobian (utils.h:844-1849) applied to the vectors staged in the
+// gather records c->aos ([v_0 .. v_{S-1}, phi^n] per node): out = d0 v + bond sums. With d0_out
+// set, d0 = 1/dt - R' at Un and the block diagonals are written first (d0 is then not read).
+int nnpp_launch_jac(nnpp_ctx* c, double BV, const double* Un, const double* d0, double* out,
+                    double* d0_out = nullptr, double* diag_out = nullptr);
+// phi block K11 (utils.h:1794-1848) with its identity rows: out = K11 in, diag = its diagonal.
+int nnpp_jac_phi_dev(nnpp_ctx* c, const double* in, double* out, double* diag);
+// Implicit (reference Newton) time step; the potential operator must be current.
+int nnpp_implicit_once(nnpp_ctx* c, int precision, const nnpp_newton_opts& o,
+                       nnpp_newton_stats& st);
+// Frozen-Jacobian apply of every field block on device vectors (F*n).
+int nnpp_jacobian_apply_dev(nnpp_ctx* c, double voln1rel, const double* in, double* out,
+                            double* diag);
+void nnpp_implicit_free(nnpp_ctx* c);
 int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flips,
                    double* volsum);
 int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo);

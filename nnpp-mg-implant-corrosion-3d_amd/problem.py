@@ -183,6 +183,37 @@ class Problem:
             self._check(st)
         return d
 
+    def newton_opts(self, **kw):
+        """The reference's Newton settings (main.cpp:35-36), with keyword overrides."""
+        o = _abi.NewtonOpts()
+        self._check(self.lib.nnpp_newton_defaults(C.byref(o)))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def step_implicit(self, nsteps=1, precision=FP64, check_finite=True, **opts):
+        """The reference's implicit step (main.cpp:547-1025): modified Newton with the frozen
+        Jacobian of utils.h:844-1849; keyword options as nnpp_newton_opts."""
+        o = self.newton_opts(**opts)
+        s = _abi.NewtonStats()
+        st = self.lib.nnpp_step_implicit(self.ctx, nsteps, precision, int(check_finite),
+                                          C.byref(o), C.byref(s))
+        d = s.as_dict()
+        d["status"] = st
+        if st not in (_abi.OK, _abi.ERR_NOCONV):
+            self._check(st)
+        return d
+
+    def jacobian_apply(self, v, voln1rel=0.0, want_diag=False):
+        """Frozen Jacobian at slot N (utils.h:844-1849) applied block by block to v (F, n);
+        returns (K v, diagonals or None)."""
+        a = np.ascontiguousarray(v, dtype=np.float64).reshape(self.F, self.n)
+        out = np.empty((self.F, self.n))
+        dg = np.empty((self.F, self.n)) if want_diag else None
+        self._check(self.lib.nnpp_jacobian_apply(self.ctx, voln1rel, dptr(a), dptr(out),
+                                                 dptr(dg) if want_diag else None))
+        return out, dg
+
     def front_update(self, threshold=None):
         thr = self.params.front_threshold if threshold is None else threshold
         fl = C.c_int32()

@@ -32,6 +32,17 @@ def test_struct_layouts_match():
     assert nn.lib.nnpp_abi_params_size() == C.sizeof(nn.Params)
     assert nn.lib.nnpp_abi_stats_size() == C.sizeof(nn.StepStats)
     assert nn.lib.nnpp_abi_adjoint_stats_size() == C.sizeof(nn._abi.AdjointStats)
+    assert nn.lib.nnpp_abi_newton_stats_size() == C.sizeof(nn._abi.NewtonStats)
+
+
+def test_newton_defaults_are_the_references():
+    o = nn._abi.NewtonOpts()
+    assert nn.lib.nnpp_newton_defaults(C.byref(o)) == 0
+    # main.cpp:35-36: EPS 10E-6, MAX_ITER 200 (Newton and linear solves)
+    assert o.newton_tol == 10E-6 and o.lin_rtol == 10E-6
+    assert o.newton_maxit == 200 and o.lin_maxit == 200
+    assert o.start == nn._abi.NEWTON_START_ROTATE
+    assert nn.lib.nnpp_newton_defaults(None) == nn._abi.ERR_ARG
 
 
 def test_params_defaults_and_errors():
