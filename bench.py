@@ -332,6 +332,34 @@ def measured_traffic(kernel_prefix):
     return None
 
 
+def implicit_workload(nn, kind, ng, steps, warmup, lin_rtol, newton_tol):
+    """The reference's implicit step (modified Newton with the frozen Jacobian, main.cpp's
+    tolerances unless overridden) on the same synthetic case: per-step time and the iteration
+    counts behind it. Not the headline metric (that is the explicit step)."""
+    import torch
+    case = (nn.synth.regular if kind == "regular" else nn.synth.screw)(ng, S=11, seed=0)
+    pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
+    nnz, _ = pb.build_clouds_hash()
+    pb.set_bc(case.bctypes, case.bcvalues)
+    pb.init_state(case.icvalues)
+    opts = dict(lin_rtol=lin_rtol, newton_tol=newton_tol)
+    if warmup:
+        pb.step_implicit(warmup, nn.FP64, **opts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = pb.step_implicit(steps, nn.FP64, **opts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"workload": f"implicit (reference Newton) step, {kind} {ng}^3, 11 species + phi",
+           "nodes": case.n, "bonds": nnz, "steps": st["steps"], "warmup": warmup,
+           "ms_per_step": dt / max(st["steps"], 1) * 1e3, "newton_tol": newton_tol,
+           "lin_rtol": lin_rtol, "stats": st}
+    nj = 2 * st["lin_iterations"] + st["newton_iterations"] * 1  # Jacobian applies + residuals
+    out["bond_passes_per_step"] = nj / max(st["steps"], 1)
+    pb.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +387,10 @@ def main():
                          "checkpoints every 20; loss against a +5%% diffusivity target)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-grid", type=int, default=64)
+    ap.add_argument("--implicit", action="store_true",
+                    help="only time the reference's implicit (Newton) step on the case")
+    ap.add_argument("--lin-rtol", type=float, default=1e-5)
+    ap.add_argument("--newton-tol", type=float, default=1e-5)
     a = ap.parse_args()
     if a.summarize_rocprof:
         d = a.summarize_rocprof
@@ -384,6 +416,9 @@ def main():
         return
     if a.adjoint:
         adjoint_workload(nn, a.adjoint_grid, a.adjoint_steps)
+        return
+    if a.implicit:
+        implicit_workload(nn, a.case, ng, a.steps, a.warmup, a.lin_rtol, a.newton_tol)
         return
     log(f"[bench] building {a.case} {ng}^3 case")
     case = (nn.synth.regular if a.case == "regular" else nn.synth.screw)(ng, S=11, seed=0)

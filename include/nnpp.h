@@ -230,7 +230,8 @@ int nnpp_get_phase(nnpp_ctx* ctx, int32_t* phase);
 /* Initial state from icvalues (F*n, field-major), main.cpp:400-421: U^n = icvalues, phase =
  * (U_Mg > phase_eps), then Dirichlet values are written into every field (the reference
  * writes them into the first Newton iterate, main.cpp:556-640). Sets solid_count, vol0rel =
- * sum over solid nodes of (U_Mg - CSat/CSolid) (main.cpp:474-479) and voln1rel = 0, and
+ * sum over solid nodes of (icvalue_Mg - CSat/CSolid), before any Dirichlet value
+ * (main.cpp:474-479), and voln1rel = 0, and
  * resets the step counter to s0 = 1. */
 int nnpp_init_state(nnpp_ctx* ctx, const double* icvalues);
 

@@ -441,9 +441,11 @@ int nnpp_init_state(nnpp_ctx* c, const double* ic) {
   std::vector<double> U(ic, ic + (size_t)n * F);
   std::vector<int32_t> ph(n);
   int64_t sc = 0;
+  double v0 = 0.0;  // main.cpp:476-479, from the initial values before any Dirichlet value
   for (int32_t i = 0; i < n; i++) {  // main.cpp:413-421
     ph[i] = (U[i] > 0.0 + c->p.phase_eps) ? 1 : 0;
     sc += ph[i];
+    if (ph[i] == 1) v0 += (U[i] - c->p.CSat / c->p.CSolid);
   }
   if (c->have_bc && n) {  // Dirichlet values (main.cpp:569-625)
     std::vector<uint16_t> mask(n);
@@ -454,9 +456,6 @@ int nnpp_init_state(nnpp_ctx* c, const double* ic) {
       for (int f = 0; f < F; f++)
         if ((mask[i] >> f) & 1u) U[(int64_t)f * n + i] = val[(int64_t)f * n + i];
   }
-  double v0 = 0.0;  // main.cpp:476-479
-  for (int32_t i = 0; i < n; i++)
-    if (ph[i] == 1) v0 += (U[i] - c->p.CSat / c->p.CSolid);
   (void)S;
   st = nnpp_set_state(c, NNPP_SLOT_N, U.data());
   if (st) return st;

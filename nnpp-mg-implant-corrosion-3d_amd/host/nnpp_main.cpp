@@ -3,13 +3,17 @@
 //
 //   nnpp_main <prefix> <jobname> <startfileNo> [--species S] [--chem reference|generic]
 //             [--precision fp64|mixed] [--steps N] [--rtol R] [--device D] [--volumes]
+//             [--implicit [--newton-tol T] [--lin-rtol L]]
 //
 // Reads <job>.data1/.data2/.datac*/.nodes/.bctypes/.bcvalues*/.icvalues* (main.cpp:210-289),
 // builds the horizons with r = delta (1 + 1e-5) and 10 dx cells (main.cpp:304-305), sets the
 // icvalues initial state (main.cpp:400-421), optionally restarts from
 // <job><prefix>-<startfileNo>.output.csv (main.cpp:506-542), then marches to stepsNo writing
 // <job><prefix>-<s>.output.csv, <job><prefix>.volume and <job><prefix>.energy every sub_steps
-// steps (main.cpp:935-999). The time step is the explicit update of nnpp_step.
+// steps (main.cpp:935-999). The time step is the explicit update of nnpp_step, or with
+// --implicit the reference's own modified-Newton step (nnpp_step_implicit, main.cpp:547-1025)
+// with its tolerances (EPS = 1e-5 unless overridden); the state then starts from the raw
+// initial values and the first Newton iterate gets the Dirichlet values, as main.cpp:556-640.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -38,15 +42,16 @@ double now() {
 int main(int argc, char** argv) {
   if (argc < 4) {  // main.cpp:172-177 (the reference also needs the third argument, :181)
     std::printf("Usage: %s <prefix> <jobname> <startfileNo> [--species S] [--chem reference|generic]"
-                " [--precision fp64|mixed] [--steps N] [--rtol R] [--device D] [--volumes]\n",
+                " [--precision fp64|mixed] [--steps N] [--rtol R] [--device D] [--volumes]"
+                " [--implicit [--newton-tol T] [--lin-rtol L]]\n",
                 argv[0]);
     return 0;
   }
   const std::string prefix = argv[1], job = argv[2];
   const long start_no = std::atol(argv[3]);
-  int S = 10, chem = -1, prec = NNPP_FP64, device = 0, use_vol = 0;
+  int S = 10, chem = -1, prec = NNPP_FP64, device = 0, use_vol = 0, implicit = 0;
   long steps_override = -1;
-  double rtol = 1e-10;
+  double rtol = 1e-10, newton_tol = -1.0, lin_rtol = -1.0;
   for (int a = 4; a < argc; a++) {
     const std::string k = argv[a];
     auto next = [&](void) -> const char* {
@@ -63,6 +68,9 @@ int main(int argc, char** argv) {
     else if (k == "--rtol") rtol = std::atof(next());
     else if (k == "--device") device = std::atoi(next());
     else if (k == "--volumes") use_vol = 1;
+    else if (k == "--implicit") implicit = 1;
+    else if (k == "--newton-tol") newton_tol = std::atof(next());
+    else if (k == "--lin-rtol") lin_rtol = std::atof(next());
     else {
       std::fprintf(stderr, "unknown option %s\n", k.c_str());
       return 2;
@@ -115,15 +123,21 @@ int main(int argc, char** argv) {
   if ((st = nnpp_set_bc(c, bct.data(), bcv.data()))) die("nnpp_set_bc", st, c);
   if ((st = nnpp_init_state(c, ic.data()))) die("nnpp_init_state", st, c);
   if ((st = nnpp_set_solver(c, rtol, 20000, 1e-4))) die("nnpp_set_solver", st, c);
+  if (implicit && (st = nnpp_set_state(c, NNPP_SLOT_N, ic.data())))  // raw U^n, main.cpp:400-411
+    die("nnpp_set_state", st, c);
   if ((st = nnpp_get_state(c, NNPP_SLOT_N, U.data()))) die("nnpp_get_state", st, c);
   if ((st = nnpp_get_phase(c, phase.data()))) die("nnpp_get_phase", st, c);
+  nnpp_newton_opts nopt;
+  nnpp_newton_defaults(&nopt);
+  if (newton_tol >= 0) nopt.newton_tol = newton_tol;
+  if (lin_rtol >= 0) nopt.lin_rtol = lin_rtol;
   const double V0 = p.dx * p.dy * p.dz;
   int64_t solid = 0;
   double vol0rel = 0.0;
-  for (int32_t i = 0; i < n; i++)
+  for (int32_t i = 0; i < n; i++)  // main.cpp:472-479, from the initial values
     if (phase[i] == 1) {
       solid++;
-      vol0rel += (U[i] - p.CSat / p.CSolid);
+      vol0rel += (ic[i] - p.CSat / p.CSolid);
     }
   std::printf("Initial volume in mm^3: %g\n", solid * V0 * 1e9);
 
@@ -157,14 +171,29 @@ int main(int argc, char** argv) {
     if (next_out > p.n_steps) next_out = p.n_steps;
     const int32_t k = (int32_t)(next_out - s + 1);
     nnpp_step_stats ss;
+    std::memset(&ss, 0, sizeof(ss));
     const double ts = now();
-    st = nnpp_step(c, k, prec, 1, &ss);
-    if (st && st != NNPP_ERR_NOCONV) die("nnpp_step", st, c);
-    if (st == NNPP_ERR_NOCONV) std::printf("warning: potential solve hit its iteration limit\n");
+    if (implicit) {
+      nnpp_newton_stats ns;
+      st = nnpp_step_implicit(c, k, prec, 1, &nopt, &ns);
+      if (st && st != NNPP_ERR_NOCONV) die("nnpp_step_implicit", st, c);
+      if (st == NNPP_ERR_NOCONV) std::printf("warning: a linear solve hit its iteration limit\n");
+      ss.steps = ns.steps;
+      ss.solid_count = ns.solid_count;
+      ss.voln1rel = ns.voln1rel;
+      std::printf("Time for a step: %g seconds. (%d steps, %d Newton iterations, %d unconverged,"
+                  " %d BiCGSTAB + %d CG iterations, norm of residual %.3e)\n",
+                  (now() - ts) / (ns.steps ? ns.steps : 1), ns.steps, ns.newton_iterations,
+                  ns.unconverged_steps, ns.lin_iterations, ns.cg_iterations, ns.last_resmax);
+    } else {
+      st = nnpp_step(c, k, prec, 1, &ss);
+      if (st && st != NNPP_ERR_NOCONV) die("nnpp_step", st, c);
+      if (st == NNPP_ERR_NOCONV) std::printf("warning: potential solve hit its iteration limit\n");
+      std::printf("Time for a step: %g seconds. (%lld steps, %lld CG iterations, relres %.2e)\n",
+                  (now() - ts) / (ss.steps ? ss.steps : 1), (long long)ss.steps,
+                  (long long)ss.cg_iterations, ss.last_relres);
+    }
     s += ss.steps;
-    std::printf("Time for a step: %g seconds. (%lld steps, %lld CG iterations, relres %.2e)\n",
-                (now() - ts) / (ss.steps ? ss.steps : 1), (long long)ss.steps,
-                (long long)ss.cg_iterations, ss.last_relres);
     const long done = s - 1;
     const double t = done * p.dt;
     if (ss.solid_count <= 0) {  // main.cpp:924-932

@@ -579,9 +579,11 @@ int or_init_state(int32_t n, const nnpp_params* p, const int32_t* bctypes,
   const int S = p->n_species;
   memcpy(U, icvalues, sizeof(double) * (size_t)(S + 1) * n);
   int64_t sc = 0;
+  double v0 = 0.0; /* main.cpp:476-479: from the initial values, before the Dirichlet values */
   for (int32_t i = 0; i < n; i++) {
     phase[i] = (U[i] > 0.0 + p->phase_eps) ? 1 : 0;
     sc += phase[i];
+    if (phase[i] == 1) v0 += (U[i] - p->CSat / p->CSolid);
   }
   if (bctypes && bcvalues)
     for (int32_t i = 0; i < n; i++) {
@@ -591,9 +593,6 @@ int or_init_state(int32_t n, const nnpp_params* p, const int32_t* bctypes,
         if (v != -1) U[(size_t)f * n + i] = (f < S) ? v / p->CSolid : v;
       }
     }
-  double v0 = 0.0;
-  for (int32_t i = 0; i < n; i++)
-    if (phase[i] == 1) v0 += (U[i] - p->CSat / p->CSolid);
   if (solid_count) *solid_count = sc;
   if (vol0rel) *vol0rel = v0;
   return NNPP_OK;

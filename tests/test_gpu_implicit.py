@@ -165,3 +165,35 @@ def test_implicit_reference_defaults_and_guards():
             pb.step_implicit(1)
         assert e.value.code == nn._abi.ERR_UNSUPPORTED
         pb.tape_end()
+
+
+def test_cpp_driver_implicit(tmp_path):
+    """host/nnpp_main --implicit on reference-format job files: the raw initial values as U^n,
+    the Dirichlet values on the first iterate (main.cpp:556-640), against or_step_implicit."""
+    import os
+    import subprocess
+    say("case nnpp_main --implicit")
+    case = _flip_case(ng=10)
+    case.params.n_steps = 8
+    job = str(tmp_path / "NNPP")
+    nn.formats.write_job(job, case)
+    exe = os.path.join(os.path.dirname(nn._abi.LIB_PATH), "bin", "nnpp_main")
+    r = subprocess.run([exe, "-01", job, "0", "--species", "3", "--chem", "generic", "--rtol",
+                        "1e-13", "--implicit", "--newton-tol", "1e-11", "--lin-rtol", "1e-13"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rp, col, _ = oracle_nb(case)
+    _, ph0, sc0, v0 = O.init_state(case)
+    state = {"U": case.icvalues.copy(), "phase": ph0, "step": 1, "voln1rel": 0.0,
+             "vol0rel": v0, "solid": sc0}
+    ref = O.run_implicit(case, rp, col, 8, state=state, **TIGHT)
+    U = np.zeros((4, case.n))
+    ph = np.zeros(case.n, dtype=np.int32)
+    assert nn.lib.nnpp_read_start_csv((job + "-01-8.output.csv").encode(), case.n, 3,
+                                      nn._abi.dptr(U), nn._abi.iptr(ph)) == 0
+    assert np.array_equal(ph, ref["phase"])
+    for f in range(4):
+        assert rel_err(U[f], ref["U"][f]) < 1e-9, f
+    vol = open(job + "-01.volume").read().split()
+    assert len(vol) == 4  # sub_steps = 2: outputs after steps 2, 4, 6, 8
+    assert abs(float(vol[-1].rstrip(",")) - ref["voln1rel"]) <= 1e-10 * abs(ref["voln1rel"])

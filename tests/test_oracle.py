@@ -263,7 +263,15 @@ def test_init_state_rules():
     assert np.array_equal(ph, (c.icvalues[0] > p.phase_eps).astype(np.int32))
     m = c.bctypes == 0
     assert np.all(U[:p.S, m] == 1.0) and np.all(U[p.S, m] == 0.0)
-    assert abs(v0 - (U[0][ph == 1] - p.CSat / p.CSolid).sum()) < 1e-12 * abs(v0)
+    # main.cpp:476-479 sums the initial values, before any Dirichlet value is written
+    assert abs(v0 - (c.icvalues[0][ph == 1] - p.CSat / p.CSolid).sum()) < 1e-12 * abs(v0)
+    c2 = nn.synth.regular(6, S=2)
+    c2.icvalues[0][:] = 3.0  # every node solid, the Dirichlet faces included
+    c2.bcvalues[0][c2.bctypes == 0] = 0.5 * c2.params.CSolid
+    U2, ph2, sc2, v02 = O.init_state(c2)
+    p2 = c2.params
+    assert sc2 == c2.n and abs(v02 - c2.n * (3.0 - p2.CSat / p2.CSolid)) < 1e-12 * abs(v02)
+    assert np.all(U2[0][c2.bctypes == 0] == 0.5)
 
 
 def _fd_case(ng, S, seed, thr=None):
