@@ -203,6 +203,21 @@ int nnpp_build_neighbors(nnpp_ctx* ctx, double r, double cell_dx, double cell_dy
                          double cell_dz, int32_t max_len, int64_t* nnz_out,
                          int32_t* max_len_out);
 
+/* Implant classification on the GPU. Replaces build_clouds(0, max_node_no, r, 0, nodes,
+ * bunnyNodes, horizonsBunny, horizonsBunnyLengths, ...) (kd_tree_interface.h:156-252) as
+ * main.cpp:316-320 calls it, r = 3 * 0.4 * dx * (1 + 1e-5): counts[i] (n ints) = the number of
+ * the nb implant points p with sqrt(x*x + y*y + z*z) <= r, (x, y, z) = p - node i (the
+ * reference's Node::distance_euclidean), over a uniform grid of the points instead of a
+ * kd-tree. NNPP_ERR_OVERFLOW when a count exceeds max_count (the reference throws,
+ * kd_tree_interface.h:214-217); counts are still written. ic_out (F*n doubles, may be NULL;
+ * the ten reference species only) receives the initial values main.cpp:422-457 selects: count
+ * 0 -> liquid (H+ and OH- 1e-4, HCO3- C0HCO3, HPO4 2- C0HPO4, Ca2+ C0Ca, Na+ C0Na, Cl- C0Cl,
+ * all / CSolid, the rest 0), else solid (Mg2+ CSolid/CSolid = 1, the rest 0); pass them to
+ * nnpp_init_state. Uses the nodes of nnpp_set_nodes. */
+int nnpp_implant_classify(nnpp_ctx* ctx, int64_t nb, const double* bx, const double* by,
+                          const double* bz, double r, int32_t max_count, int32_t* counts,
+                          double* ic_out);
+
 /* Copy the device CSR out (row_ptr: n+1, col: nnz). */
 int nnpp_get_neighbors(nnpp_ctx* ctx, int64_t* row_ptr, int32_t* col);
 

@@ -10,6 +10,7 @@
 #pragma clang fp contract(off)
 
 #include <climits>
+#include <cmath>
 #include <vector>
 
 #include "nnpp_internal.hpp"
@@ -70,6 +71,80 @@ __global__ void k_cell_scatter(int32_t n, const double* __restrict__ x,
   cy[pos] = y[i];
   cz[pos] = z[i];
   cid[pos] = (int32_t)i;
+}
+
+// ---- implant classification (kd_tree_interface.h:156-252 as called at main.cpp:316-320) ----
+// A uniform grid of cells at least 1.001 r wide over the implant points' bounding box; the
+// points are counting-sorted by cell, and a node tests the points of the 3x3x3 cells around
+// its own (every point within r of it lies there). The predicate is the reference's
+// Node::distance_euclidean: sqrt(x*x + y*y + z*z) <= r with (x, y, z) = point - node.
+struct IGrid {
+  double x0, y0, z0, h;
+  int64_t mx, my, mz;
+};
+
+__device__ __forceinline__ int64_t icell_axis(double v, double v0, double h, int64_t m) {
+  const double f = floor((v - v0) / h);
+  return f < 0.0 ? 0 : (f > (double)(m - 1) ? m - 1 : (int64_t)f);
+}
+
+__global__ void k_imp_cell(int64_t nb, const double* __restrict__ bx,
+                           const double* __restrict__ by, const double* __restrict__ bz,
+                           IGrid g, int32_t* __restrict__ cell, int32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int64_t c = icell_axis(bx[i], g.x0, g.h, g.mx) +
+                    g.mx * (icell_axis(by[i], g.y0, g.h, g.my) +
+                            g.my * icell_axis(bz[i], g.z0, g.h, g.mz));
+  cell[i] = (int32_t)c;
+  atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_imp_scatter(int64_t nb, const double* __restrict__ bx,
+                              const double* __restrict__ by, const double* __restrict__ bz,
+                              const int32_t* __restrict__ cell, const int64_t* __restrict__ start,
+                              int32_t* __restrict__ cursor, double* __restrict__ sx,
+                              double* __restrict__ sy, double* __restrict__ sz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int32_t c = cell[i];
+  const int64_t pos = start[c] + atomicAdd(&cursor[c], 1);
+  sx[pos] = bx[i];
+  sy[pos] = by[i];
+  sz[pos] = bz[i];
+}
+
+__global__ void k_imp_count(int32_t n, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, IGrid g,
+                            const int64_t* __restrict__ start, const double* __restrict__ sx,
+                            const double* __restrict__ sy, const double* __restrict__ sz,
+                            double r, int32_t max_count, int32_t* __restrict__ counts,
+                            int* __restrict__ over) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = x[i], yi = y[i], zi = z[i];
+  // index range of the neighbouring cells, clipped to the grid (empty when the node is far)
+  const double fx = floor((xi - g.x0) / g.h), fy = floor((yi - g.y0) / g.h),
+               fz = floor((zi - g.z0) / g.h);
+  int32_t cnt = 0;
+  if (fx >= -1.0 && fx <= (double)g.mx && fy >= -1.0 && fy <= (double)g.my && fz >= -1.0 &&
+      fz <= (double)g.mz) {
+    const int64_t ax = (int64_t)fx, ay = (int64_t)fy, az = (int64_t)fz;
+    const int64_t x0 = ax - 1 < 0 ? 0 : ax - 1, x1 = ax + 1 > g.mx - 1 ? g.mx - 1 : ax + 1;
+    const int64_t y0 = ay - 1 < 0 ? 0 : ay - 1, y1 = ay + 1 > g.my - 1 ? g.my - 1 : ay + 1;
+    const int64_t z0 = az - 1 < 0 ? 0 : az - 1, z1 = az + 1 > g.mz - 1 ? g.mz - 1 : az + 1;
+    for (int64_t cz = z0; cz <= z1; cz++)
+      for (int64_t cy = y0; cy <= y1; cy++)
+        for (int64_t cx = x0; cx <= x1; cx++) {
+          const int64_t c = cx + g.mx * (cy + g.my * cz);
+          for (int64_t k = start[c]; k < start[c + 1]; k++) {
+            const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+            if (__dsqrt_rn(dx * dx + dy * dy + dz * dz) <= r) cnt++;
+          }
+        }
+  }
+  counts[i] = cnt;
+  if (cnt > max_count) atomicOr(over, 1);
 }
 
 // Candidate scan of one node; FILL == false counts, FILL == true writes the hits.
@@ -340,4 +415,88 @@ cleanup:
 #undef NB_TRY
 #undef NB_HIP
   return st;
+}
+
+// Implant points within r of every node (device arrays bx/by/bz of nb points, bounding box
+// given); counts on the device. NNPP_ERR_OVERFLOW when a count exceeds max_count.
+int nnpp_implant_count_dev(nnpp_ctx* c, int64_t nb, const double* bx, const double* by,
+                           const double* bz, const double bbox[6], double r, int32_t max_count,
+                           int32_t* counts) {
+  const int32_t n = c->n;
+  hipStream_t s = c->stream;
+  if (n == 0) return NNPP_OK;
+  if (nb == 0) {
+    NNPP_HIP(c, hipMemsetAsync(counts, 0, sizeof(int32_t) * n, s));
+    NNPP_HIP(c, hipStreamSynchronize(s));
+    return NNPP_OK;
+  }
+  IGrid g;
+  g.x0 = bbox[0];
+  g.y0 = bbox[2];
+  g.z0 = bbox[4];
+  g.h = r > 0 ? r * 1.001 : 1.0;
+  const double ext[3] = {bbox[1] - bbox[0], bbox[3] - bbox[2], bbox[5] - bbox[4]};
+  if (g.h < 1e-300) g.h = 1e-300;
+  for (;;) {  // at most 2^25 cells: widen the cells (never below 1.001 r)
+    g.mx = (int64_t)std::floor(ext[0] / g.h) + 1;
+    g.my = (int64_t)std::floor(ext[1] / g.h) + 1;
+    g.mz = (int64_t)std::floor(ext[2] / g.h) + 1;
+    if ((double)g.mx * (double)g.my * (double)g.mz <= (double)(1 << 25)) break;
+    g.h *= 2.0;
+  }
+  const int64_t ncell = g.mx * g.my * g.mz;
+#define NB_TRY(expr)           \
+  do {                         \
+    st = (expr);               \
+    if (st) goto cleanup;      \
+  } while (0)
+#define NB_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      st = nnpp_set_error(c, NNPP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+      goto cleanup;                                                                    \
+    }                                                                                  \
+  } while (0)
+  int32_t *cell = nullptr, *cnt = nullptr, *cursor = nullptr;
+  int64_t* start = nullptr;
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  int* over = nullptr;
+  int st = NNPP_OK;
+  {
+    NB_TRY(dev_alloc(c, &cell, nb));
+    NB_TRY(dev_alloc(c, &cnt, ncell));
+    NB_TRY(dev_alloc(c, &cursor, ncell));
+    NB_TRY(dev_alloc(c, &start, ncell + 1));
+    NB_TRY(dev_alloc(c, &sx, nb));
+    NB_TRY(dev_alloc(c, &sy, nb));
+    NB_TRY(dev_alloc(c, &sz, nb));
+    NB_TRY(dev_alloc(c, &over, 1));
+    NB_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * ncell, s));
+    NB_HIP(hipMemsetAsync(cursor, 0, sizeof(int32_t) * ncell, s));
+    NB_HIP(hipMemsetAsync(over, 0, sizeof(int), s));
+    const int64_t gb = (nb + NNPP_BLOCK - 1) / NNPP_BLOCK;
+    k_imp_cell<<<gb, NNPP_BLOCK, 0, s>>>(nb, bx, by, bz, g, cell, cnt);
+    NB_HIP(hipGetLastError());
+    NB_TRY(nnpp_device_exclusive_scan_i32(c, cnt, ncell, start));
+    k_imp_scatter<<<gb, NNPP_BLOCK, 0, s>>>(nb, bx, by, bz, cell, start, cursor, sx, sy, sz);
+    NB_HIP(hipGetLastError());
+    k_imp_count<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, g, start, sx, sy, sz,
+                                                       r, max_count, counts, over);
+    NB_HIP(hipGetLastError());
+    int h_over = 0;
+    NB_HIP(hipMemcpyAsync(&h_over, over, sizeof(int), hipMemcpyDeviceToHost, s));
+    NB_HIP(hipStreamSynchronize(s));
+    if (h_over)
+      st = nnpp_set_error(c, NNPP_ERR_OVERFLOW,
+                          "an implant count exceeds max_count (kd_tree_interface.h:214-217)");
+  }
+cleanup:
+  hipStreamSynchronize(s);
+  void* ps[] = {cell, cnt, cursor, start, sx, sy, sz, over};
+  for (void* q : ps)
+    if (q) hipFree(q);
+  return st;
+#undef NB_TRY
+#undef NB_HIP
 }

@@ -657,6 +657,73 @@ int nnpp_step(nnpp_ctx* c, int32_t nsteps, int32_t precision, int32_t check_fini
   return status;
 }
 
+int nnpp_implant_classify(nnpp_ctx* c, int64_t nb, const double* bx, const double* by,
+                          const double* bz, double r, int32_t max_count, int32_t* counts,
+                          double* ic_out) {
+  if (!c || nb < 0 || (nb && (!bx || !by || !bz)) || (!counts && c->n) || !(r >= 0) ||
+      max_count < 0)
+    return NNPP_ERR_ARG;
+  int st = need_params(c);
+  if (st) return st;
+  if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
+  if (ic_out && c->p.n_species != 10)
+    return nnpp_set_error(c, NNPP_ERR_UNSUPPORTED, "implant initial values need the 10 reference species");
+  NNPP_HIP(c, hipSetDevice(c->device));
+  const int32_t n = c->n;
+  if (n == 0) return NNPP_OK;
+  double bbox[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t k = 0; k < nb; k++) {
+    const double v[3] = {bx[k], by[k], bz[k]};
+    for (int a = 0; a < 3; a++) {
+      if (k == 0 || v[a] < bbox[2 * a]) bbox[2 * a] = v[a];
+      if (k == 0 || v[a] > bbox[2 * a + 1]) bbox[2 * a + 1] = v[a];
+    }
+  }
+  if (!(std::isfinite(bbox[0]) && std::isfinite(bbox[1]) && std::isfinite(bbox[2]) &&
+        std::isfinite(bbox[3]) && std::isfinite(bbox[4]) && std::isfinite(bbox[5])))
+    return nnpp_set_error(c, NNPP_ERR_ARG, "non-finite implant coordinates");
+  double* d = nullptr;
+  int32_t* dc = nullptr;
+  const size_t m = nb ? (size_t)nb : 1;
+  NNPP_HIP(c, hipMalloc((void**)&d, sizeof(double) * 3 * m));
+  if (hipMalloc((void**)&dc, sizeof(int32_t) * n) != hipSuccess) {
+    hipFree(d);
+    return nnpp_set_error(c, NNPP_ERR_HIP, "implant classification allocation");
+  }
+  auto done = [&](int e) {
+    hipFree(d);
+    hipFree(dc);
+    return e;
+  };
+  if (nb && (hipMemcpy(d, bx, sizeof(double) * nb, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d + m, by, sizeof(double) * nb, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d + 2 * m, bz, sizeof(double) * nb, hipMemcpyHostToDevice) != hipSuccess))
+    return done(nnpp_set_error(c, NNPP_ERR_HIP, "implant point upload"));
+  st = nnpp_implant_count_dev(c, nb, d, d + m, d + 2 * m, bbox, r, max_count, dc);
+  if (st && st != NNPP_ERR_OVERFLOW) return done(st);
+  if (hipMemcpy(counts, dc, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return done(nnpp_set_error(c, NNPP_ERR_HIP, "count download"));
+  if (ic_out) {  // main.cpp:422-457
+    const double CS = c->p.CSolid;
+    for (int32_t i = 0; i < n; i++) {
+      double v[11] = {0.0};
+      if (counts[i] == 0) {
+        v[1] = 1e-4 / CS;
+        v[2] = 1e-4 / CS;
+        v[3] = c->p.C0HCO3 / CS;
+        v[5] = c->p.C0HPO4 / CS;
+        v[7] = c->p.C0Ca / CS;
+        v[8] = c->p.C0Na / CS;
+        v[9] = c->p.C0Cl / CS;
+      } else {
+        v[0] = CS / CS;
+      }
+      for (int f = 0; f < 11; f++) ic_out[(int64_t)f * n + i] = v[f];
+    }
+  }
+  return done(st);
+}
+
 int nnpp_newton_defaults(nnpp_newton_opts* o) {
   if (!o) return NNPP_ERR_ARG;
   o->newton_tol = 10E-6;  // EPS, main.cpp:35

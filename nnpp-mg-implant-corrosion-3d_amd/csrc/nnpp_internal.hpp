@@ -374,6 +374,9 @@ __host__ __device__ static inline int64_t nnpp_nblocks(int64_t n) { return (n + 
 int nnpp_build_neighbors_dev(nnpp_ctx* c, double r, double cdx, double cdy, double cdz,
                              int32_t max_len, int64_t* nnz_out, int32_t* max_len_out);
 int nnpp_build_bond_slices(nnpp_ctx* c);
+int nnpp_implant_count_dev(nnpp_ctx* c, int64_t nb, const double* bx, const double* by,
+                           const double* bz, const double bbox[6], double r, int32_t max_count,
+                           int32_t* counts);
 int nnpp_build_potential(nnpp_ctx* c);
 int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
                      const double* Un, double* out, uint16_t* passmask = nullptr);

@@ -3,7 +3,7 @@
 //
 //   nnpp_main <prefix> <jobname> <startfileNo> [--species S] [--chem reference|generic]
 //             [--precision fp64|mixed] [--steps N] [--rtol R] [--device D] [--volumes]
-//             [--implicit [--newton-tol T] [--lin-rtol L]]
+//             [--implicit [--newton-tol T] [--lin-rtol L]] [--bunny]
 //
 // Reads <job>.data1/.data2/.datac*/.nodes/.bctypes/.bcvalues*/.icvalues* (main.cpp:210-289),
 // builds the horizons with r = delta (1 + 1e-5) and 10 dx cells (main.cpp:304-305), sets the
@@ -14,6 +14,9 @@
 // --implicit the reference's own modified-Newton step (nnpp_step_implicit, main.cpp:547-1025)
 // with its tolerances (EPS = 1e-5 unless overridden); the state then starts from the raw
 // initial values and the first Newton iterate gets the Dirichlet values, as main.cpp:556-640.
+// --bunny is the reference's BunnyConditions = 1 (main.cpp:40): the initial values come from
+// the implant points <job>.bunnyNodes (bunnyNodesNo of .data1, main.cpp:236-240), classified on
+// the GPU (nnpp_implant_classify, main.cpp:316-320, 422-457) instead of the icvalues files.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -43,13 +46,13 @@ int main(int argc, char** argv) {
   if (argc < 4) {  // main.cpp:172-177 (the reference also needs the third argument, :181)
     std::printf("Usage: %s <prefix> <jobname> <startfileNo> [--species S] [--chem reference|generic]"
                 " [--precision fp64|mixed] [--steps N] [--rtol R] [--device D] [--volumes]"
-                " [--implicit [--newton-tol T] [--lin-rtol L]]\n",
+                " [--implicit [--newton-tol T] [--lin-rtol L]] [--bunny]\n",
                 argv[0]);
     return 0;
   }
   const std::string prefix = argv[1], job = argv[2];
   const long start_no = std::atol(argv[3]);
-  int S = 10, chem = -1, prec = NNPP_FP64, device = 0, use_vol = 0, implicit = 0;
+  int S = 10, chem = -1, prec = NNPP_FP64, device = 0, use_vol = 0, implicit = 0, bunny = 0;
   long steps_override = -1;
   double rtol = 1e-10, newton_tol = -1.0, lin_rtol = -1.0;
   for (int a = 4; a < argc; a++) {
@@ -69,6 +72,7 @@ int main(int argc, char** argv) {
     else if (k == "--device") device = std::atoi(next());
     else if (k == "--volumes") use_vol = 1;
     else if (k == "--implicit") implicit = 1;
+    else if (k == "--bunny") bunny = 1;
     else if (k == "--newton-tol") newton_tol = std::atof(next());
     else if (k == "--lin-rtol") lin_rtol = std::atof(next());
     else {
@@ -120,6 +124,27 @@ int main(int argc, char** argv) {
     die("nnpp_build_neighbors", st, c);
   std::printf("Building horizons took %g seconds. Max. horizon length: %d\n", now() - t0, max_len);
   std::printf("%d nodes; horizon radius: %g\n", n, p.delta);
+  if (bunny) {  // main.cpp:236-240, 316-325, 422-457
+    double d1[4];
+    if ((st = nnpp_read_vector((job + ".data1").c_str(), 4, d1))) die("reading .data1", st);
+    const int64_t nb = (int64_t)d1[3];
+    std::vector<double> bx(nb), by(nb), bz(nb);
+    if ((st = nnpp_read_nodes((job + ".bunnyNodes").c_str(), nb, bx.data(), by.data(), bz.data())))
+      die("reading .bunnyNodes", st);
+    std::vector<int32_t> counts(n);
+    t0 = now();
+    st = nnpp_implant_classify(c, nb, bx.data(), by.data(), bz.data(),
+                               3. * 0.4 * p.dx * (1.00 + 1e-5), 1024, counts.data(), ic.data());
+    if (st) die("nnpp_implant_classify", st, c);
+    int32_t mx = 0;
+    double mean = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+      mx = counts[i] > mx ? counts[i] : mx;
+      mean += counts[i];
+    }
+    std::printf("Building bunnyNodes took %g seconds. Max. horizonsBunny length: %d \n"
+                "Mean horizonBunny length: %g\n", now() - t0, mx, n ? mean / n : 0.0);
+  }
   if ((st = nnpp_set_bc(c, bct.data(), bcv.data()))) die("nnpp_set_bc", st, c);
   if ((st = nnpp_init_state(c, ic.data()))) die("nnpp_init_state", st, c);
   if ((st = nnpp_set_solver(c, rtol, 20000, 1e-4))) die("nnpp_set_solver", st, c);

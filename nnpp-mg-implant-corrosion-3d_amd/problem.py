@@ -77,6 +77,20 @@ class Problem:
         self.nnz = int(nnz.value)
         return self.nnz, int(ml.value)
 
+    def implant_classify(self, bx, by, bz, r=None, max_count=1024, want_ic=False):
+        """kd_tree_interface.h:156-252 as main.cpp:316-320 calls it: implant points within
+        r = 3*0.4*dx*(1+1e-5) of every node; returns (counts, initial values of main.cpp:422-457
+        or None). Raises NnppError(ERR_OVERFLOW) where the reference throws."""
+        p = self.params
+        r = 3. * 0.4 * p.dx * (1.00 + 1e-5) if r is None else r
+        b = [np.ascontiguousarray(a, dtype=np.float64) for a in (bx, by, bz)]
+        counts = np.empty(self.n, dtype=np.int32)
+        ic = np.empty((self.F, self.n)) if want_ic else None
+        self._check(self.lib.nnpp_implant_classify(self.ctx, b[0].size, dptr(b[0]), dptr(b[1]),
+                                                   dptr(b[2]), r, max_count, iptr(counts),
+                                                   dptr(ic) if want_ic else None))
+        return counts, ic
+
     def neighbors(self):
         rp = np.empty(self.n + 1, dtype=np.int64)
         col = np.empty(max(self.nnz or 0, 1), dtype=np.int32)

@@ -1177,3 +1177,57 @@ int or_step_implicit(int32_t n, const nnpp_params* p, const double* x, const dou
   free(dU);
   return status;
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Implant classification: build_clouds (kd_tree_interface.h:156-252) as main.cpp:316-320 calls
+ * it (min_node_no = 0, dr = 0) and the initial values it selects (main.cpp:422-457).
+ * ------------------------------------------------------------------------------------------ */
+
+/* counts[i] = number of implant points p with sqrt(x*x + y*y + z*z) <= r, (x, y, z) = p - node
+ * (Node::distance_euclidean, kd_tree_interface.h:44-50), by brute force over all pairs.
+ * NNPP_ERR_OVERFLOW if a count exceeds max_count (the reference throws,
+ * kd_tree_interface.h:214-217). */
+int or_implant_classify(int32_t n, const double* x, const double* y, const double* z,
+                        int64_t nb, const double* bx, const double* by, const double* bz,
+                        double r, int32_t max_count, int32_t* counts) {
+  int status = NNPP_OK;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int32_t i = 0; i < n; i++) {
+    int32_t c = 0;
+    for (int64_t k = 0; k < nb; k++) {
+      const double dx = bx[k] - x[i], dy = by[k] - y[i], dz = bz[k] - z[i];
+      if (sqrt(dx * dx + dy * dy + dz * dz) <= r) c++;
+    }
+    counts[i] = c;
+    if (c > max_count) {
+#pragma omp atomic write
+      status = NNPP_ERR_OVERFLOW;
+    }
+  }
+  return status;
+}
+
+/* main.cpp:422-457: no implant point within r -> liquid (Mg2+ 0, H+ and OH- 1e-4/CSolid,
+ * HCO3- C0HCO3/CSolid, CO3 2- 0, HPO4 2- C0HPO4/CSolid, PO4 3- 0, Ca2+ C0Ca/CSolid,
+ * Na+ C0Na/CSolid, Cl- C0Cl/CSolid, phi 0); otherwise solid (Mg2+ CSolid/CSolid, the rest 0).
+ * The ten reference species only. ic: 11*n, field-major. */
+int or_implant_icvalues(int32_t n, const nnpp_params* p, const int32_t* counts, double* ic) {
+  if (p->n_species != 10) return NNPP_ERR_UNSUPPORTED;
+  const double CS = p->CSolid;
+  for (int32_t i = 0; i < n; i++) {
+    double v[11] = {0.0};
+    if (counts[i] == 0) {
+      v[1] = 1e-4 / CS;
+      v[2] = 1e-4 / CS;
+      v[3] = p->C0HCO3 / CS;
+      v[5] = p->C0HPO4 / CS;
+      v[7] = p->C0Ca / CS;
+      v[8] = p->C0Na / CS;
+      v[9] = p->C0Cl / CS;
+    } else {
+      v[0] = CS / CS;
+    }
+    for (int f = 0; f < 11; f++) ic[(size_t)f * n + i] = v[f];
+  }
+  return NNPP_OK;
+}

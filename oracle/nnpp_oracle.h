@@ -82,6 +82,11 @@ int or_step_implicit(int32_t n, const nnpp_params* p, const double* x, const dou
                      int32_t* newton_iters, int32_t* lin_iters, int32_t* converged,
                      double* resmax_out, int32_t* flips);
 
+int or_implant_classify(int32_t n, const double* x, const double* y, const double* z,
+                        int64_t nb, const double* bx, const double* by, const double* bz,
+                        double r, int32_t max_count, int32_t* counts);
+int or_implant_icvalues(int32_t n, const nnpp_params* p, const int32_t* counts, double* ic);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,9 @@ class Oracle:
                                            _dp, _dp, _d, _i32, _ip, _dp]
         L.or_adjoint.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _dp, _dp, _ip,
                                  C.c_int64, _d, _d, _i32, _d, _i32, _dp, _dp, _dp, _ip]
+        L.or_implant_classify.argtypes = [_i32, _dp, _dp, _dp, C.c_int64, _dp, _dp, _dp, _d,
+                                          _i32, _ip]
+        L.or_implant_icvalues.argtypes = [_i32, P, _ip, _dp]
         L.or_jacobian_apply.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _ip, _dp,
                                         _dp, _d, _dp, _dp, _dp]
         L.or_step_implicit.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _dp, _dp, _dp,
@@ -142,6 +145,21 @@ class Oracle:
                                  maxit, _d_(np.ascontiguousarray(lamN, dtype=np.float64)),
                                  _d_(grad), _d_(lam0), C.byref(done))
         return grad, lam0, int(done.value), st
+
+    def implant_classify(self, x, y, z, bx, by, bz, r, max_count=1024):
+        """Implant points within r of every node (brute force); (status, counts)."""
+        counts = np.zeros(x.size, dtype=np.int32)
+        st = self.lib.or_implant_classify(x.size, _d_(x), _d_(y), _d_(z), bx.size, _d_(bx),
+                                          _d_(by), _d_(bz), r, max_count, _i_(counts))
+        return st, counts
+
+    def implant_icvalues(self, params, counts):
+        ic = np.zeros((11, counts.size))
+        st = self.lib.or_implant_icvalues(counts.size, C.byref(params),
+                                          _i_(np.ascontiguousarray(counts, dtype=np.int32)),
+                                          _d_(ic))
+        assert st == 0, st
+        return ic
 
     def jacobian_apply(self, case, rp, col, phase, Un, v, voln1rel=0.0, want_diag=False):
         """Frozen Jacobian of utils.h:844-1849 at Un applied field by field to v (F, n));
@@ -244,6 +262,8 @@ class RefHarness:
                                             _ip, _ip]
         L.ref_residuals.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _ip,
                                     _dp, _dp, _dp, _dp, _ip, _ip, _d, _dp]
+        L.ref_build_clouds_implant.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
+                                               _d, C.c_int, _ip]
         L.ref_newton_matrix.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip,
                                         _ip, _dp, _dp, _dp, _ip, _ip, _d, C.c_int, C.c_int, _ip,
                                         _ip, _dp, _ip]
@@ -279,6 +299,12 @@ class RefHarness:
                                _i_(np.ascontiguousarray(phase, dtype=np.int32)),
                                _i_(case.bctypes), voln1rel, _d_(F))
         return F
+
+    def build_clouds_implant(self, x, y, z, bx, by, bz, r, max_len=1024):
+        lengths = np.zeros(x.size, dtype=np.int32)
+        st = self.lib.ref_build_clouds_implant(x.size, _d_(x), _d_(y), _d_(z), bx.size, _d_(bx),
+                                               _d_(by), _d_(bz), r, max_len, _i_(lengths))
+        return st, lengths
 
     def newton_matrix(self, case, ell, lengths, phase, U1, Un, which, voln1rel=0.0):
         p = case.params

@@ -1,6 +1,7 @@
 // ref_harness.cpp — extern "C" entry points around the reference's own headers, so the tests
 // can run the reference's build_clouds_hash (src/helpers/grid.h:43-204), utils::Residuals
-// (src/helpers/utils.h:27-842) and utils::NewtonMatrices (utils.h:844-1849) on small inputs.
+// (src/helpers/utils.h:27-842), utils::NewtonMatrices (utils.h:844-1849) and the implant
+// classification build_clouds (src/helpers/kd_tree_interface.h:156-252) on small inputs.
 //
 // TEST INFRASTRUCTURE ONLY. Compiled by oracle/Makefile (target `ref`) from the reference
 // sources where they lie (-I <reference>/src/helpers -I <reference>/eigen); the output goes to
@@ -14,6 +15,7 @@
 #include <vector>
 
 #include "grid.h"
+#include "kd_tree_interface.h"
 #include "utils.h"
 
 typedef Eigen::SparseMatrix<double> SpMat;
@@ -174,6 +176,29 @@ int ref_newton_matrix(int n, const double* data1, const double* data2, const dou
       vals[c] = it.value();
       c++;
     }
+  return 0;
+}
+
+// build_clouds(min_node_no, max_node_no, r, dr, nodes, nodes2, ...) (kd_tree_interface.h:156-252)
+// with nodes2 = the implant points, as main.cpp:316-320 calls it with min_node_no = 0, dr = 0:
+// lengths[i] = number of implant points within r of node i. Returns 0, or 2 on the reference's
+// overflow exception.
+int ref_build_clouds_implant(int n, const double* x, const double* y, const double* z, int nb,
+                             const double* bx, const double* by, const double* bz, double r,
+                             int max_len, int* lengths) {
+  Eigen::MatrixXd nodes, bunny;
+  make_nodes(n, x, y, z, nodes);
+  make_nodes(nb, bx, by, bz, bunny);
+  Eigen::MatrixXi clouds;
+  Eigen::VectorXi cl;
+  int mx = 0;
+  double mean = 0.0;
+  try {
+    build_clouds(0, max_len, r, 0.0, nodes, bunny, clouds, cl, mx, mean);
+  } catch (const std::runtime_error&) {
+    return 2;
+  }
+  for (int i = 0; i < n; i++) lengths[i] = cl[i];
   return 0;
 }
 

@@ -11,7 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from helpers import ell_to_sorted_csr, orc, ref_chem_case  # noqa: E402
+from helpers import ell_to_sorted_csr, orc, ref_chem_case, reference_params  # noqa: E402
 
 
 def main():
@@ -50,5 +50,46 @@ def main():
         print(name, case.n, int(rp[-1]), ml)
 
 
+def implant_case(seed=11, ng=12):
+    """Lattice nodes with the production spacing and an implant point cloud: points inside a
+    sphere plus points at distance r (1 + e) of nodes for e within a few ulps of 0, some exactly
+    on an axis, so the boundary of the predicate is exercised."""
+    p = reference_params()
+    rng = np.random.default_rng(seed)
+    dx = p.dx
+    g = (np.arange(ng) + 0.5) * dx
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    x, y, z = (a.ravel().copy() for a in (xx, yy, zz))
+    L = ng * dx
+    r = 3. * 0.4 * dx * (1.00 + 1e-5)  # main.cpp:317
+    u = rng.standard_normal((3, 3000))
+    u /= np.linalg.norm(u, axis=0)
+    rad = 0.3 * L * rng.uniform(0, 1, 3000) ** (1 / 3)
+    pts = [L / 2 + u * rad]
+    pick = rng.choice(x.size, 400, replace=False)
+    v = rng.standard_normal((3, 400))
+    v /= np.linalg.norm(v, axis=0)
+    e = rng.choice([-4e-16, -2e-16, 0.0, 2e-16, 4e-16, -1e-12, 1e-12], 400)
+    node = np.stack([x[pick], y[pick], z[pick]])
+    pts.append(node + v * (r * (1 + e)))
+    ax = np.zeros((3, 60))
+    for j in range(60):
+        ax[j % 3, j] = r if j % 2 == 0 else -r
+    pts.append(np.stack([x[pick[:60]], y[pick[:60]], z[pick[:60]]]) + ax)
+    P = np.concatenate(pts, axis=1)
+    return p, x, y, z, P[0].copy(), P[1].copy(), P[2].copy(), r
+
+
+def main_implant():
+    R = orc.RefHarness()
+    p, x, y, z, bx, by, bz, r = implant_case()
+    st, counts = R.build_clouds_implant(x, y, z, bx, by, bz, r)
+    assert st == 0
+    np.savez_compressed(os.path.join(HERE, "ref_implant.npz"), x=x, y=y, z=z, bx=bx, by=by,
+                        bz=bz, r=r, counts=counts)
+    print("implant", x.size, bx.size, int((counts > 0).sum()), int(counts.max()))
+
+
 if __name__ == "__main__":
     main()
+    main_implant()
