@@ -177,6 +177,14 @@ int nnpp_read_start_csv(const char* path, int32_t n, int32_t n_species, double* 
 int nnpp_neighbors_to_ell(int32_t n, const int64_t* row_ptr, const int32_t* col,
                           int32_t max_len, int32_t* ell, int32_t* lengths);
 
+/* The inverse adapter: the reference's horizons and horizonsLengths (n x max_len column-major,
+ * self at column 0, lengths counting self; grid.h:145-149,170-195) to CSR with rows sorted by
+ * id and self dropped, e.g. for nnpp_set_neighbors. row_ptr (n+1) is always written; col
+ * (sum of lengths - n entries) when not NULL, so a first call with col = NULL sizes it.
+ * NNPP_ERR_ARG on a malformed row (self not first, an id out of range, a repeated id). */
+int nnpp_neighbors_from_ell(int32_t n, int32_t max_len, const int32_t* ell,
+                            const int32_t* lengths, int64_t* row_ptr, int32_t* col);
+
 /* ------------------------------------------------------------------- context ------------- */
 
 /* Create a context on HIP device `device`. Replaces nothing (the reference has no device). */
@@ -217,6 +225,17 @@ int nnpp_build_neighbors(nnpp_ctx* ctx, double r, double cell_dx, double cell_dy
 int nnpp_implant_classify(nnpp_ctx* ctx, int64_t nb, const double* bx, const double* by,
                           const double* bz, double r, int32_t max_count, int32_t* counts,
                           double* ic_out);
+
+/* Voronoi-approximate node volumes for irregular clouds (the reference multiplies every bond
+ * by dx*dy*dz, utils.h:122-124; pass the result to nnpp_set_nodes as per-node volumes).
+ * Probes on a lattice of spacing h fill the nodes' bounding box widened by margin on every
+ * side (round((extent + 2 margin) / h) probes per axis, centres at lo + (k + 1/2) h); every
+ * probe goes to its nearest node within radius (ties to the lowest index), and vol_out[i]
+ * (n doubles) = probes of node i * h^3. Probes with no node within radius count for none.
+ * On a lattice of spacing dx with margin dx/2, radius >= dx and h = dx/k every node gets dx^3
+ * exactly. No reference counterpart. */
+int nnpp_voronoi_volumes(nnpp_ctx* ctx, double h, double margin, double radius,
+                         double* vol_out);
 
 /* Copy the device CSR out (row_ptr: n+1, col: nnz). */
 int nnpp_get_neighbors(nnpp_ctx* ctx, int64_t* row_ptr, int32_t* col);

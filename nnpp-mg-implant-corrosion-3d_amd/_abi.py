@@ -136,6 +136,7 @@ SIGNATURES = {
     "nnpp_read_ivector": (C.c_int, [C.c_char_p, _i64, _ip]),
     "nnpp_read_nodes": (C.c_int, [C.c_char_p, _i64, _dp, _dp, _dp]),
     "nnpp_neighbors_to_ell": (C.c_int, [_i32, _lp, _ip, _i32, _ip, _ip]),
+    "nnpp_neighbors_from_ell": (C.c_int, [_i32, _i32, _ip, _ip, _lp, _ip]),
     "nnpp_write_output_csv": (C.c_int, [C.c_char_p, _i32, _i32, _dp, _dp, _dp, _dp, _ip]),
     "nnpp_read_start_csv": (C.c_int, [C.c_char_p, _i32, _i32, _dp, _ip]),
     "nnpp_create": (C.c_int, [C.POINTER(_p), _i32]),
@@ -170,6 +171,7 @@ SIGNATURES = {
     "nnpp_tape_end": (C.c_int, [_p]),
     "nnpp_abi_adjoint_stats_size": (C.c_int64, []),
     "nnpp_implant_classify": (C.c_int, [_p, _i64, _dp, _dp, _dp, _d, _i32, _ip, _dp]),
+    "nnpp_voronoi_volumes": (C.c_int, [_p, _d, _d, _d, _dp]),
     "nnpp_newton_defaults": (C.c_int, [C.POINTER(NewtonOpts)]),
     "nnpp_step_implicit": (C.c_int, [_p, _i32, _i32, _i32, C.POINTER(NewtonOpts),
                                      C.POINTER(NewtonStats)]),

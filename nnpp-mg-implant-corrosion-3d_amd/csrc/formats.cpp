@@ -5,6 +5,7 @@
 // doubles, main.cpp:83,210), <job>.data2 = 51 values, <job>.datac<k> = 7 values per species,
 // <job>.datac<S+1> = 4 values for phi, <job>.nodes = n lines "x y z", <job>.bctypes = n ints,
 // <job>.bcvalues<f> / <job>.icvalues<f> = n doubles per field.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -213,6 +214,37 @@ int nnpp_neighbors_to_ell(int32_t n, const int64_t* row_ptr, const int32_t* col,
     ell[i] = i;  // clouds(i, 0) = i, grid.h:171
     for (int64_t t = 0; t < len; t++) ell[(int64_t)(t + 1) * n + i] = col[b0 + t];
     lengths[i] = (int32_t)(len + 1);  // grid.h:195
+  }
+  return NNPP_OK;
+}
+
+int nnpp_neighbors_from_ell(int32_t n, int32_t max_len, const int32_t* ell,
+                            const int32_t* lengths, int64_t* row_ptr, int32_t* col) {
+  if (n < 0 || max_len < 1 || (n && (!ell || !lengths || !row_ptr))) return NNPP_ERR_ARG;
+  if (!row_ptr) return NNPP_ERR_ARG;
+  row_ptr[0] = 0;
+  for (int32_t i = 0; i < n; i++) {
+    if (lengths[i] < 1 || lengths[i] > max_len) return NNPP_ERR_ARG;
+    row_ptr[i + 1] = row_ptr[i] + (lengths[i] - 1);
+  }
+  if (!col) return NNPP_OK;
+  std::vector<int32_t> row;
+  for (int32_t i = 0; i < n; i++) {
+    row.clear();
+    for (int32_t t = 0; t < lengths[i]; t++) {  // column-major, self at column 0 (grid.h:171)
+      const int32_t j = ell[(int64_t)t * n + i];
+      if (j < 0 || j >= n) return NNPP_ERR_ARG;
+      if (t == 0) {
+        if (j != i) return NNPP_ERR_ARG;
+        continue;
+      }
+      row.push_back(j);
+    }
+    std::sort(row.begin(), row.end());
+    for (size_t t = 0; t < row.size(); t++) {
+      if (row[t] == i || (t && row[t] == row[t - 1])) return NNPP_ERR_ARG;
+      col[row_ptr[i] + (int64_t)t] = row[t];
+    }
   }
   return NNPP_OK;
 }

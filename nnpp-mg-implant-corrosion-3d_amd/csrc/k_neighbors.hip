@@ -104,7 +104,8 @@ __global__ void k_imp_scatter(int64_t nb, const double* __restrict__ bx,
                               const double* __restrict__ by, const double* __restrict__ bz,
                               const int32_t* __restrict__ cell, const int64_t* __restrict__ start,
                               int32_t* __restrict__ cursor, double* __restrict__ sx,
-                              double* __restrict__ sy, double* __restrict__ sz) {
+                              double* __restrict__ sy, double* __restrict__ sz,
+                              int32_t* __restrict__ sid) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   const int32_t c = cell[i];
@@ -112,6 +113,47 @@ __global__ void k_imp_scatter(int64_t nb, const double* __restrict__ bx,
   sx[pos] = bx[i];
   sy[pos] = by[i];
   sz[pos] = bz[i];
+  if (sid) sid[pos] = (int32_t)i;
+}
+
+// Voronoi-approximate volumes: probe p of the lattice (centres lo + (k + 1/2) h) goes to the
+// nearest node within radius (squared distances; ties to the lowest node index).
+__global__ void k_vor_probe(int64_t np, int64_t px, int64_t py, double lx, double ly, double lz,
+                            double hp, IGrid g, const int64_t* __restrict__ start,
+                            const double* __restrict__ sx, const double* __restrict__ sy,
+                            const double* __restrict__ sz, const int32_t* __restrict__ sid,
+                            double r2, unsigned long long* __restrict__ count) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int64_t kx = p % px, ky = (p / px) % py, kz = p / (px * py);
+  const double qx = lx + ((double)kx + 0.5) * hp, qy = ly + ((double)ky + 0.5) * hp,
+               qz = lz + ((double)kz + 0.5) * hp;
+  const double fx = floor((qx - g.x0) / g.h), fy = floor((qy - g.y0) / g.h),
+               fz = floor((qz - g.z0) / g.h);
+  if (!(fx >= -1.0 && fx <= (double)g.mx && fy >= -1.0 && fy <= (double)g.my && fz >= -1.0 &&
+        fz <= (double)g.mz))
+    return;
+  const int64_t ax = (int64_t)fx, ay = (int64_t)fy, az = (int64_t)fz;
+  const int64_t x0 = ax - 1 < 0 ? 0 : ax - 1, x1 = ax + 1 > g.mx - 1 ? g.mx - 1 : ax + 1;
+  const int64_t y0 = ay - 1 < 0 ? 0 : ay - 1, y1 = ay + 1 > g.my - 1 ? g.my - 1 : ay + 1;
+  const int64_t z0 = az - 1 < 0 ? 0 : az - 1, z1 = az + 1 > g.mz - 1 ? g.mz - 1 : az + 1;
+  double best = r2;
+  int32_t who = -1;
+  for (int64_t cz = z0; cz <= z1; cz++)
+    for (int64_t cy = y0; cy <= y1; cy++)
+      for (int64_t cx = x0; cx <= x1; cx++) {
+        const int64_t c = cx + g.mx * (cy + g.my * cz);
+        for (int64_t k = start[c]; k < start[c + 1]; k++) {
+          const double dx = sx[k] - qx, dy = sy[k] - qy, dz = sz[k] - qz;
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          const int32_t j = sid[k];
+          if (d2 < best || (d2 == best && who >= 0 && j < who) || (d2 == best && who < 0)) {
+            best = d2;
+            who = j;
+          }
+        }
+      }
+  if (who >= 0) atomicAdd(&count[who], 1ull);
 }
 
 __global__ void k_imp_count(int32_t n, const double* __restrict__ x, const double* __restrict__ y,
@@ -417,6 +459,88 @@ cleanup:
   return st;
 }
 
+// A uniform grid (cells at least cell_min wide, at most 2^25 cells) over np points with the
+// given bounding box: the points counting-sorted by cell (sx, sy, sz, source index sid).
+struct PointGrid {
+  IGrid g;
+  int64_t* start = nullptr;
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  int32_t* sid = nullptr;
+  void release() {
+    void* ps[] = {start, sx, sy, sz, sid};
+    for (void* q : ps)
+      if (q) hipFree(q);
+    start = nullptr;
+    sx = sy = sz = nullptr;
+    sid = nullptr;
+  }
+};
+
+static int build_point_grid(nnpp_ctx* c, int64_t np, const double* px, const double* py,
+                            const double* pz, const double bbox[6], double cell_min,
+                            bool want_id, PointGrid& G) {
+  hipStream_t s = c->stream;
+  IGrid& g = G.g;
+  g.x0 = bbox[0];
+  g.y0 = bbox[2];
+  g.z0 = bbox[4];
+  g.h = cell_min > 0 ? cell_min : 1.0;
+  if (g.h < 1e-300) g.h = 1e-300;
+  const double ext[3] = {bbox[1] - bbox[0], bbox[3] - bbox[2], bbox[5] - bbox[4]};
+  for (;;) {  // widen the cells (never below cell_min) until the grid has at most 2^25 cells
+    g.mx = (int64_t)std::floor(ext[0] / g.h) + 1;
+    g.my = (int64_t)std::floor(ext[1] / g.h) + 1;
+    g.mz = (int64_t)std::floor(ext[2] / g.h) + 1;
+    if ((double)g.mx * (double)g.my * (double)g.mz <= (double)(1 << 25)) break;
+    g.h *= 2.0;
+  }
+  const int64_t ncell = g.mx * g.my * g.mz;
+  int32_t *cell = nullptr, *cnt = nullptr, *cursor = nullptr;
+  int st = NNPP_OK;
+  auto fail = [&](int e) {
+    if (cell) hipFree(cell);
+    if (cnt) hipFree(cnt);
+    if (cursor) hipFree(cursor);
+    G.release();
+    return e;
+  };
+#define PG_TRY(expr)          \
+  do {                        \
+    st = (expr);              \
+    if (st) return fail(st);  \
+  } while (0)
+#define PG_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(nnpp_set_error(c, NNPP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+  PG_TRY(dev_alloc(c, &cell, np));
+  PG_TRY(dev_alloc(c, &cnt, ncell));
+  PG_TRY(dev_alloc(c, &cursor, ncell));
+  PG_TRY(dev_alloc(c, &G.start, ncell + 1));
+  PG_TRY(dev_alloc(c, &G.sx, np));
+  PG_TRY(dev_alloc(c, &G.sy, np));
+  PG_TRY(dev_alloc(c, &G.sz, np));
+  if (want_id) PG_TRY(dev_alloc(c, &G.sid, np));
+  PG_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * ncell, s));
+  PG_HIP(hipMemsetAsync(cursor, 0, sizeof(int32_t) * ncell, s));
+  const int64_t gb = (np + NNPP_BLOCK - 1) / NNPP_BLOCK;
+  k_imp_cell<<<gb, NNPP_BLOCK, 0, s>>>(np, px, py, pz, g, cell, cnt);
+  PG_HIP(hipGetLastError());
+  PG_TRY(nnpp_device_exclusive_scan_i32(c, cnt, ncell, G.start));
+  k_imp_scatter<<<gb, NNPP_BLOCK, 0, s>>>(np, px, py, pz, cell, G.start, cursor, G.sx, G.sy,
+                                          G.sz, G.sid);
+  PG_HIP(hipGetLastError());
+  PG_HIP(hipStreamSynchronize(s));
+  hipFree(cell);
+  hipFree(cnt);
+  hipFree(cursor);
+  return NNPP_OK;
+#undef PG_TRY
+#undef PG_HIP
+}
+
 // Implant points within r of every node (device arrays bx/by/bz of nb points, bounding box
 // given); counts on the device. NNPP_ERR_OVERFLOW when a count exceeds max_count.
 int nnpp_implant_count_dev(nnpp_ctx* c, int64_t nb, const double* bx, const double* by,
@@ -430,73 +554,73 @@ int nnpp_implant_count_dev(nnpp_ctx* c, int64_t nb, const double* bx, const doub
     NNPP_HIP(c, hipStreamSynchronize(s));
     return NNPP_OK;
   }
-  IGrid g;
-  g.x0 = bbox[0];
-  g.y0 = bbox[2];
-  g.z0 = bbox[4];
-  g.h = r > 0 ? r * 1.001 : 1.0;
-  const double ext[3] = {bbox[1] - bbox[0], bbox[3] - bbox[2], bbox[5] - bbox[4]};
-  if (g.h < 1e-300) g.h = 1e-300;
-  for (;;) {  // at most 2^25 cells: widen the cells (never below 1.001 r)
-    g.mx = (int64_t)std::floor(ext[0] / g.h) + 1;
-    g.my = (int64_t)std::floor(ext[1] / g.h) + 1;
-    g.mz = (int64_t)std::floor(ext[2] / g.h) + 1;
-    if ((double)g.mx * (double)g.my * (double)g.mz <= (double)(1 << 25)) break;
-    g.h *= 2.0;
-  }
-  const int64_t ncell = g.mx * g.my * g.mz;
-#define NB_TRY(expr)           \
-  do {                         \
-    st = (expr);               \
-    if (st) goto cleanup;      \
-  } while (0)
-#define NB_HIP(call)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess) {                                                            \
-      st = nnpp_set_error(c, NNPP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-      goto cleanup;                                                                    \
-    }                                                                                  \
-  } while (0)
-  int32_t *cell = nullptr, *cnt = nullptr, *cursor = nullptr;
-  int64_t* start = nullptr;
-  double *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  PointGrid G;
+  int st = build_point_grid(c, nb, bx, by, bz, bbox, r * 1.001, false, G);
+  if (st) return st;
   int* over = nullptr;
-  int st = NNPP_OK;
-  {
-    NB_TRY(dev_alloc(c, &cell, nb));
-    NB_TRY(dev_alloc(c, &cnt, ncell));
-    NB_TRY(dev_alloc(c, &cursor, ncell));
-    NB_TRY(dev_alloc(c, &start, ncell + 1));
-    NB_TRY(dev_alloc(c, &sx, nb));
-    NB_TRY(dev_alloc(c, &sy, nb));
-    NB_TRY(dev_alloc(c, &sz, nb));
-    NB_TRY(dev_alloc(c, &over, 1));
-    NB_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * ncell, s));
-    NB_HIP(hipMemsetAsync(cursor, 0, sizeof(int32_t) * ncell, s));
-    NB_HIP(hipMemsetAsync(over, 0, sizeof(int), s));
-    const int64_t gb = (nb + NNPP_BLOCK - 1) / NNPP_BLOCK;
-    k_imp_cell<<<gb, NNPP_BLOCK, 0, s>>>(nb, bx, by, bz, g, cell, cnt);
-    NB_HIP(hipGetLastError());
-    NB_TRY(nnpp_device_exclusive_scan_i32(c, cnt, ncell, start));
-    k_imp_scatter<<<gb, NNPP_BLOCK, 0, s>>>(nb, bx, by, bz, cell, start, cursor, sx, sy, sz);
-    NB_HIP(hipGetLastError());
-    k_imp_count<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, g, start, sx, sy, sz,
-                                                       r, max_count, counts, over);
-    NB_HIP(hipGetLastError());
-    int h_over = 0;
-    NB_HIP(hipMemcpyAsync(&h_over, over, sizeof(int), hipMemcpyDeviceToHost, s));
-    NB_HIP(hipStreamSynchronize(s));
-    if (h_over)
-      st = nnpp_set_error(c, NNPP_ERR_OVERFLOW,
-                          "an implant count exceeds max_count (kd_tree_interface.h:214-217)");
+  if (hipMalloc((void**)&over, sizeof(int)) != hipSuccess ||
+      hipMemsetAsync(over, 0, sizeof(int), s) != hipSuccess) {
+    if (over) hipFree(over);
+    G.release();
+    return nnpp_set_error(c, NNPP_ERR_HIP, "implant flag allocation");
   }
-cleanup:
-  hipStreamSynchronize(s);
-  void* ps[] = {cell, cnt, cursor, start, sx, sy, sz, over};
-  for (void* q : ps)
-    if (q) hipFree(q);
-  return st;
-#undef NB_TRY
-#undef NB_HIP
+  k_imp_count<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, G.g, G.start, G.sx,
+                                                     G.sy, G.sz, r, max_count, counts, over);
+  hipError_t e = hipGetLastError();
+  int h_over = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h_over, over, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipFree(over);
+  G.release();
+  if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
+  if (h_over)
+    return nnpp_set_error(c, NNPP_ERR_OVERFLOW,
+                          "an implant count exceeds max_count (kd_tree_interface.h:214-217)");
+  return NNPP_OK;
+}
+
+// Voronoi-approximate node volumes (probe lattice of spacing hp over the nodes' bounding box
+// widened by margin, nearest node within radius); counts on the device (n values).
+int nnpp_voronoi_dev(nnpp_ctx* c, double hp, double margin, double radius,
+                     unsigned long long* count, int64_t dims[3]) {
+  const int32_t n = c->n;
+  hipStream_t s = c->stream;
+  dims[0] = dims[1] = dims[2] = 0;
+  if (n == 0) return NNPP_OK;
+  std::vector<double> hx(n), hy(n), hz(n);
+  NNPP_HIP(c, hipMemcpyAsync(hx.data(), c->x, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(hy.data(), c->y, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(hz.data(), c->z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipStreamSynchronize(s));
+  double bbox[6] = {hx[0], hx[0], hy[0], hy[0], hz[0], hz[0]};
+  for (int32_t i = 0; i < n; i++) {
+    bbox[0] = hx[i] < bbox[0] ? hx[i] : bbox[0];
+    bbox[1] = hx[i] > bbox[1] ? hx[i] : bbox[1];
+    bbox[2] = hy[i] < bbox[2] ? hy[i] : bbox[2];
+    bbox[3] = hy[i] > bbox[3] ? hy[i] : bbox[3];
+    bbox[4] = hz[i] < bbox[4] ? hz[i] : bbox[4];
+    bbox[5] = hz[i] > bbox[5] ? hz[i] : bbox[5];
+  }
+  const double lo[3] = {bbox[0] - margin, bbox[2] - margin, bbox[4] - margin};
+  for (int a = 0; a < 3; a++) {
+    const double e = (bbox[2 * a + 1] + margin) - lo[a];
+    dims[a] = (int64_t)std::llround(e / hp);
+    if (dims[a] < 1) dims[a] = 1;
+  }
+  const double np_d = (double)dims[0] * (double)dims[1] * (double)dims[2];
+  if (np_d > 4e9)
+    return nnpp_set_error(c, NNPP_ERR_ARG, "voronoi volumes: more than 4e9 probes (raise h)");
+  const int64_t np = dims[0] * dims[1] * dims[2];
+  PointGrid G;
+  int st = build_point_grid(c, n, c->x, c->y, c->z, bbox, radius * 1.001, true, G);
+  if (st) return st;
+  NNPP_HIP(c, hipMemsetAsync(count, 0, sizeof(unsigned long long) * n, s));
+  k_vor_probe<<<(np + NNPP_BLOCK - 1) / NNPP_BLOCK, NNPP_BLOCK, 0, s>>>(
+      np, dims[0], dims[1], lo[0], lo[1], lo[2], hp, G.g, G.start, G.sx, G.sy, G.sz, G.sid,
+      radius * radius, count);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  G.release();
+  if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
+  return NNPP_OK;
 }

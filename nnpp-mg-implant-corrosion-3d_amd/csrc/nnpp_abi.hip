@@ -724,6 +724,29 @@ int nnpp_implant_classify(nnpp_ctx* c, int64_t nb, const double* bx, const doubl
   return done(st);
 }
 
+int nnpp_voronoi_volumes(nnpp_ctx* c, double h, double margin, double radius,
+                         double* vol_out) {
+  if (!c || !(h > 0) || !(margin >= 0) || !(radius > 0) || (!vol_out && c->n))
+    return NNPP_ERR_ARG;
+  if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
+  NNPP_HIP(c, hipSetDevice(c->device));
+  const int32_t n = c->n;
+  if (!n) return NNPP_OK;
+  unsigned long long* cnt = nullptr;
+  NNPP_HIP(c, hipMalloc((void**)&cnt, sizeof(unsigned long long) * n));
+  int64_t dims[3];
+  int st = nnpp_voronoi_dev(c, h, margin, radius, cnt, dims);
+  std::vector<unsigned long long> hc(n);
+  if (!st && hipMemcpy(hc.data(), cnt, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost) !=
+                 hipSuccess)
+    st = nnpp_set_error(c, NNPP_ERR_HIP, "count download");
+  hipFree(cnt);
+  if (st) return st;
+  const double pv = h * h * h;
+  for (int32_t i = 0; i < n; i++) vol_out[i] = (double)hc[i] * pv;
+  return NNPP_OK;
+}
+
 int nnpp_newton_defaults(nnpp_newton_opts* o) {
   if (!o) return NNPP_ERR_ARG;
   o->newton_tol = 10E-6;  // EPS, main.cpp:35

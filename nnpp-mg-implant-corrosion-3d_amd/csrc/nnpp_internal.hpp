@@ -377,6 +377,8 @@ int nnpp_build_bond_slices(nnpp_ctx* c);
 int nnpp_implant_count_dev(nnpp_ctx* c, int64_t nb, const double* bx, const double* by,
                            const double* bz, const double bbox[6], double r, int32_t max_count,
                            int32_t* counts);
+int nnpp_voronoi_dev(nnpp_ctx* c, double hp, double margin, double radius,
+                     unsigned long long* count, int64_t dims[3]);
 int nnpp_build_potential(nnpp_ctx* c);
 int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
                      const double* Un, double* out, uint16_t* passmask = nullptr);

@@ -91,6 +91,18 @@ class Problem:
                                                    dptr(ic) if want_ic else None))
         return counts, ic
 
+    def voronoi_volumes(self, h=None, margin=None, radius=None):
+        """Voronoi-approximate node volumes (probe lattice of spacing h, default dx/4; margin
+        default dx/2; radius default 2 delta)."""
+        p = self.params
+        d = min(p.dx, p.dy, p.dz)
+        h = d / 4 if h is None else h
+        margin = d / 2 if margin is None else margin
+        radius = 2 * p.delta if radius is None else radius
+        out = np.empty(self.n)
+        self._check(self.lib.nnpp_voronoi_volumes(self.ctx, h, margin, radius, dptr(out)))
+        return out
+
     def neighbors(self):
         rp = np.empty(self.n + 1, dtype=np.int64)
         col = np.empty(max(self.nnz or 0, 1), dtype=np.int32)

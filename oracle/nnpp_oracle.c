@@ -1231,3 +1231,53 @@ int or_implant_icvalues(int32_t n, const nnpp_params* p, const int32_t* counts, 
   }
   return NNPP_OK;
 }
+
+/* Voronoi-approximate node volumes (no reference counterpart): probes on a lattice of spacing h
+ * over the nodes' bounding box widened by margin (round(extent / h) per axis, centres at
+ * lo + (k + 1/2) h), each assigned to its nearest node within radius by brute force (squared
+ * distances, ties to the lowest index); vol[i] = probes of node i * h^3. */
+int or_voronoi_volumes(int32_t n, const double* x, const double* y, const double* z, double h,
+                       double margin, double radius, double* vol) {
+  if (n <= 0) return NNPP_OK;
+  double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+  for (int32_t i = 0; i < n; i++) {
+    const double v[3] = {x[i], y[i], z[i]};
+    for (int a = 0; a < 3; a++) {
+      lo[a] = v[a] < lo[a] ? v[a] : lo[a];
+      hi[a] = v[a] > hi[a] ? v[a] : hi[a];
+    }
+  }
+  int64_t dims[3];
+  for (int a = 0; a < 3; a++) {
+    const double l = lo[a] - margin;
+    dims[a] = llround(((hi[a] + margin) - l) / h);
+    if (dims[a] < 1) dims[a] = 1;
+    lo[a] = l;
+  }
+  const int64_t np = dims[0] * dims[1] * dims[2];
+  int64_t* cnt = (int64_t*)calloc((size_t)n, sizeof(int64_t));
+  const double r2 = radius * radius;
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < np; p++) {
+    const int64_t kx = p % dims[0], ky = (p / dims[0]) % dims[1], kz = p / (dims[0] * dims[1]);
+    const double qx = lo[0] + ((double)kx + 0.5) * h, qy = lo[1] + ((double)ky + 0.5) * h,
+                 qz = lo[2] + ((double)kz + 0.5) * h;
+    double best = r2;
+    int32_t who = -1;
+    for (int32_t j = 0; j < n; j++) { /* ascending j: the first of equal distances wins */
+      const double dx = x[j] - qx, dy = y[j] - qy, dz = z[j] - qz;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < best || (d2 == best && who < 0)) {
+        best = d2;
+        who = j;
+      }
+    }
+    if (who >= 0) {
+#pragma omp atomic
+      cnt[who]++;
+    }
+  }
+  for (int32_t i = 0; i < n; i++) vol[i] = (double)cnt[i] * (h * h * h);
+  free(cnt);
+  return NNPP_OK;
+}

@@ -86,6 +86,8 @@ int or_implant_classify(int32_t n, const double* x, const double* y, const doubl
                         int64_t nb, const double* bx, const double* by, const double* bz,
                         double r, int32_t max_count, int32_t* counts);
 int or_implant_icvalues(int32_t n, const nnpp_params* p, const int32_t* counts, double* ic);
+int or_voronoi_volumes(int32_t n, const double* x, const double* y, const double* z, double h,
+                       double margin, double radius, double* vol);
 
 #ifdef __cplusplus
 }

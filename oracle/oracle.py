@@ -58,6 +58,7 @@ class Oracle:
         L.or_implant_classify.argtypes = [_i32, _dp, _dp, _dp, C.c_int64, _dp, _dp, _dp, _d,
                                           _i32, _ip]
         L.or_implant_icvalues.argtypes = [_i32, P, _ip, _dp]
+        L.or_voronoi_volumes.argtypes = [_i32, _dp, _dp, _dp, _d, _d, _d, _dp]
         L.or_jacobian_apply.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _ip, _dp,
                                         _dp, _d, _dp, _dp, _dp]
         L.or_step_implicit.argtypes = [_i32, P, _dp, _dp, _dp, _dp, _lp, _ip, _ip, _dp, _dp, _dp,
@@ -152,6 +153,11 @@ class Oracle:
         st = self.lib.or_implant_classify(x.size, _d_(x), _d_(y), _d_(z), bx.size, _d_(bx),
                                           _d_(by), _d_(bz), r, max_count, _i_(counts))
         return st, counts
+
+    def voronoi_volumes(self, x, y, z, h, margin, radius):
+        vol = np.zeros(x.size)
+        self.lib.or_voronoi_volumes(x.size, _d_(x), _d_(y), _d_(z), h, margin, radius, _d_(vol))
+        return vol
 
     def implant_icvalues(self, params, counts):
         ic = np.zeros((11, counts.size))

@@ -123,3 +123,24 @@ def _oracle_nb(case):
                                         p.dx * 10., p.dy * 10., p.dz * 10.)
     assert st == 0
     return rp, col, ml
+
+
+@pytest.mark.parametrize("jitter,res", [(0.0, 3), (0.3, 3), (0.45, 2)])
+def test_voronoi_volumes_vs_oracle(jitter, res):
+    say("case voronoi", jitter, res)
+    p = reference_params()
+    dx = p.dx
+    ng = 9
+    g = (np.arange(ng) + 0.5) * dx
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([xx, yy, zz]).reshape(3, -1)
+    P = P + np.random.default_rng(5).uniform(-jitter * dx, jitter * dx, P.shape)
+    x, y, z = (a.copy() for a in P)
+    h = dx / res
+    vo = O.voronoi_volumes(x, y, z, h, dx / 2, 2 * dx)
+    pb, _ = _problem(x, y, z)
+    with pb:
+        v = pb.voronoi_volumes(h=h, margin=dx / 2, radius=2 * dx)
+    assert np.array_equal(v, vo)
+    if jitter == 0.0:
+        assert np.all(v == res ** 3 * h ** 3)

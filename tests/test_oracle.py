@@ -119,6 +119,48 @@ def test_ell_adapter_matches_reference_layout():
                                         nn._abi.iptr(ell), nn._abi.iptr(ln)) == nn._abi.ERR_OVERFLOW
 
 
+def _from_ell(E, ln):
+    n, ml = E.shape
+    flat = np.ascontiguousarray(E.T).ravel()  # column-major n x max_len
+    rp = np.zeros(n + 1, dtype=np.int64)
+    st = nn.lib.nnpp_neighbors_from_ell(n, ml, nn._abi.iptr(flat), nn._abi.iptr(ln),
+                                        nn._abi.lptr(rp), None)
+    if st:
+        return st, None, None
+    col = np.zeros(max(int(rp[-1]), 1), dtype=np.int32)
+    st = nn.lib.nnpp_neighbors_from_ell(n, ml, nn._abi.iptr(flat), nn._abi.iptr(ln),
+                                        nn._abi.lptr(rp), nn._abi.iptr(col))
+    return st, rp, col[: int(rp[-1])]
+
+
+def test_ell_to_csr_adapter():
+    """nnpp_neighbors_from_ell: the reference's horizons layout (self first, neighbours in
+    emission order, -1 padding) back to id-sorted CSR."""
+    d, case = load_golden("jit7")
+    n, ml = case.n, int(d["max_len"])
+    rp, col = d["row_ptr"], d["col"]
+    rng = np.random.default_rng(3)
+    E = np.full((n, ml), -1, dtype=np.int32)
+    E[:, 0] = np.arange(n)
+    for i in range(n):  # neighbours in an arbitrary (emission) order
+        E[i, 1:d["lengths"][i]] = rng.permutation(col[rp[i]:rp[i + 1]])
+    st, rp2, col2 = _from_ell(E, d["lengths"].astype(np.int32))
+    assert st == 0 and np.array_equal(rp2, rp) and np.array_equal(col2, col)
+    if orc.RefHarness.available():  # the reference's own emission order
+        p = case.params
+        st, ell, ln, _ = orc.RefHarness().build_clouds_hash(case.x, case.y, case.z,
+                                                            p.delta * (1.00 + 1e-5), p.dx * 10,
+                                                            p.dy * 10, p.dz * 10)
+        st, rp3, col3 = _from_ell(ell, ln)
+        assert st == 0 and np.array_equal(rp3, rp) and np.array_equal(col3, col)
+    bad = E.copy()
+    bad[5, 0] = 6  # self not in column 0
+    assert _from_ell(bad, d["lengths"].astype(np.int32))[0] == nn._abi.ERR_ARG
+    bad = E.copy()
+    bad[5, 2] = bad[5, 1]  # repeated id
+    assert _from_ell(bad, d["lengths"].astype(np.int32))[0] == nn._abi.ERR_ARG
+
+
 # ---------------------------------------------------------------- residuals (utils.h) -------
 @pytest.mark.parametrize("name", ["reg7", "jit7"])
 def test_residuals_match_reference(name):

@@ -47,3 +47,27 @@ def test_implant_initial_values_table():
         assert list(ic[:, i]) == (liquid if c == 0 else solid)
     # the phase rule of nnpp_init_state (U_Mg > EPS) turns them into the reference's phases
     assert np.array_equal((ic[0] > p.phase_eps).astype(int), (counts > 0).astype(int))
+
+
+def _lattice(ng, dx, jitter=0.0, seed=0):
+    g = (np.arange(ng) + 0.5) * dx
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([xx, yy, zz]).reshape(3, -1)
+    if jitter:
+        P = P + np.random.default_rng(seed).uniform(-jitter * dx, jitter * dx, P.shape)
+    return P[0].copy(), P[1].copy(), P[2].copy()
+
+
+def test_voronoi_volumes_closed_forms():
+    """A lattice with margin dx/2 and probes dx/3 apart: every node owns exactly its 27 probes
+    (its cube, dx^3); a jittered cloud with a large radius assigns every probe once."""
+    dx = 2e-5
+    x, y, z = _lattice(6, dx)
+    h = dx / 3
+    v = O.voronoi_volumes(x, y, z, h, dx / 2, dx)
+    assert np.all(v == 27 * h ** 3) and abs(v[0] - dx ** 3) <= 1e-12 * dx ** 3
+    x, y, z = _lattice(7, dx, jitter=0.3, seed=2)
+    v = O.voronoi_volumes(x, y, z, h, dx / 2, 10 * dx)
+    ext = [(a.max() + dx / 2) - (a.min() - dx / 2) for a in (x, y, z)]
+    nprobe = np.prod([round(e / h) for e in ext])
+    assert abs(v.sum() / h ** 3 - nprobe) < 1e-6 and np.all(v > 0)
