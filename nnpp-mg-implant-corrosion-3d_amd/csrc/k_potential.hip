@@ -589,6 +589,15 @@ __global__ void k_m_apply(int32_t n, double inv_sb, float* __restrict__ e,
 }
 
 // phi = x on free nodes, Dirichlet value on Dirichlet nodes, unchanged elsewhere.
+__global__ void k_phi_guess(int32_t n, const uint8_t* __restrict__ pfree,
+                            const double* __restrict__ phi, const double* __restrict__ prev,
+                            double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = phi[i];
+  out[i] = pfree[i] ? 2.0 * a - prev[i] : a;
+}
+
 __global__ void k_scatter_phi(int32_t n, int S, const uint8_t* __restrict__ pfree,
                               const uint16_t* __restrict__ bcmask, const double* __restrict__ bcphi,
                               const double* __restrict__ x, double* __restrict__ phi) {
@@ -875,6 +884,13 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   if (spmv64) *spmv64 += n64;
   if (spmv32) *spmv32 += n32;
   return status;
+}
+
+int nnpp_phi_guess_dev(nnpp_ctx* c, const double* phi, const double* prev, double* out) {
+  if (!c->n) return NNPP_OK;
+  k_phi_guess<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(c->n, c->pfree, phi, prev, out);
+  NNPP_HIP(c, hipGetLastError());
+  return NNPP_OK;
 }
 
 // One potential-operator application (with its dot product) on the solver's direction vector;

@@ -56,6 +56,8 @@ void free_nodes(nnpp_ctx* c) {
   dfree(c->U[0]);
   dfree(c->U[1]);
   dfree(c->Unew);
+  dfree(c->phi_prev);
+  c->phi_prev_ok = false;
   nnpp_implicit_free(c);
   dfree(c->Fres);
   dfree(c->aos);
@@ -137,11 +139,21 @@ int tape_checkpoint(nnpp_ctx* c) {
   }
   NNPP_HIP(c, hipMemcpyAsync(k.U, c->U[NNPP_SLOT_N], bytes, hipMemcpyDeviceToDevice, c->stream));
   NNPP_HIP(c, hipMemcpyAsync(k.phase, c->phase, m, hipMemcpyDeviceToDevice, c->stream));
+  k.phi_prev_ok = c->phi_prev_ok;
+  if (k.phi_prev_ok) {
+    if (hipMalloc((void**)&k.phi_prev, sizeof(double) * m) != hipSuccess) {
+      hipFree(k.U);
+      hipFree(k.phase);
+      return nnpp_set_error(c, NNPP_ERR_HIP, "tape checkpoint allocation");
+    }
+    NNPP_HIP(c, hipMemcpyAsync(k.phi_prev, c->phi_prev, sizeof(double) * m,
+                               hipMemcpyDeviceToDevice, c->stream));
+  }
   k.step = c->step;
   k.solid_count = c->solid_count;
   k.voln1rel = c->voln1rel;
   c->tape.ckpts.push_back(k);
-  c->tape.bytes += (int64_t)(bytes + m);
+  c->tape.bytes += (int64_t)(bytes + m + (k.phi_prev ? sizeof(double) * m : 0));
   return NNPP_OK;
 }
 
@@ -151,6 +163,7 @@ void nnpp_tape_free(nnpp_ctx* c) {
   for (nnpp_tape_ckpt& k : c->tape.ckpts) {
     dfree(k.U);
     dfree(k.phase);
+    dfree(k.phi_prev);
   }
   c->tape = nnpp_tape();
 }
@@ -388,6 +401,7 @@ int nnpp_set_state(nnpp_ctx* c, int32_t slot, const double* U) {
   if (c->n)
     NNPP_HIP(c, hipMemcpy(c->U[slot], U, sizeof(double) * c->n * fields(c), hipMemcpyHostToDevice));
   c->imp_first = true;  // an implicit step starts again from U^n with the Dirichlet values
+  c->phi_prev_ok = false;
   return NNPP_OK;
 }
 
@@ -546,6 +560,7 @@ int nnpp_poisson_solve(nnpp_ctx* c, int32_t precision, int32_t* iters, double* r
   int st = ensure_potential(c);
   if (st) return st;
   double* U = c->U[NNPP_SLOT_N];
+  c->phi_prev_ok = false;  // phi^n no longer follows the explicit steps' history
   return nnpp_potential_solve_dev(c, precision, U, U + (int64_t)c->p.n_species * c->n, iters,
                                   relres, nullptr, nullptr);
 }
@@ -576,11 +591,18 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
   const double BV = std::pow(10., -c->p.vol_red * (1. - c->voln1rel));
   st = nnpp_launch_flux(c, 0, precision, BV, U, U, c->Unew);
   if (st) return st;
-  // 2. phi^{n+1} from the charge of U^n, warm-started from phi^n, written into Unew
+  // 2. phi^{n+1} from the charge of U^n, written into Unew; warm start 2 phi^n - phi^{n-1}
+  //    on the free rows once the previous potential is known (phi^n before that)
   double* phi_new = c->Unew + (int64_t)S * n;
-  if (n)
-    NNPP_HIP(c, hipMemcpyAsync(phi_new, U + (int64_t)S * n, sizeof(double) * n,
-                               hipMemcpyDeviceToDevice, c->stream));
+  const double* phi_n = U + (int64_t)S * n;
+  if (n && !c->phi_prev)
+    NNPP_HIP(c, hipMalloc((void**)&c->phi_prev, sizeof(double) * n));
+  if (n && c->phi_prev_ok) {
+    if ((st = nnpp_phi_guess_dev(c, phi_n, c->phi_prev, phi_new))) return st;
+  } else if (n) {
+    NNPP_HIP(c, hipMemcpyAsync(phi_new, phi_n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                               c->stream));
+  }
   int32_t its = 0;
   double rr = 0.0;
   int status = NNPP_OK;
@@ -588,6 +610,10 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
                                 &loc.cg_spmv_fp32);
   if (st == NNPP_ERR_NOCONV) status = NNPP_ERR_NOCONV;
   else if (st) return st;
+  if (n)
+    NNPP_HIP(c, hipMemcpyAsync(c->phi_prev, phi_n, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                               c->stream));
+  c->phi_prev_ok = (n > 0);
   std::swap(c->U[NNPP_SLOT_N], c->Unew);
   // 3. front update and 4. volume cadence
   int32_t flips = 0;

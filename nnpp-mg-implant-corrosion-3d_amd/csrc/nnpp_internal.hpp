@@ -99,6 +99,8 @@ struct nnpp_tape_step {  // host record of one recorded step
 struct nnpp_tape_ckpt {  // state before recorded step `every * index`
   double* U = nullptr;   // F*n
   uint8_t* phase = nullptr;
+  double* phi_prev = nullptr;  // n: the previous potential (warm start), if phi_prev_ok
+  bool phi_prev_ok = false;
   int64_t step = 0, solid_count = 0;
   double voln1rel = 0.0;
 };
@@ -188,6 +190,11 @@ struct nnpp_ctx {
   int64_t imp_n = -1;          // n and S the buffers were allocated for
   int imp_S = 0;
   bool imp_first = true;       // next implicit step starts from U^n with the Dirichlet values
+
+  // potential of the previous explicit step, for the linear extrapolation that warm-starts
+  // the next potential solve; invalid after anything but an explicit step changed the state
+  double* phi_prev = nullptr;
+  bool phi_prev_ok = false;
 
   int64_t step = 1;
   double voln1rel = 0.0;
@@ -388,6 +395,8 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                              int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32,
                              const double* rhs = nullptr);
 int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out);
+// Warm start of the potential solve: out = 2 phi - prev on the free rows, phi elsewhere.
+int nnpp_phi_guess_dev(nnpp_ctx* c, const double* phi, const double* prev, double* out);
 // Species blocks of the frozen Jacobian (utils.h:844-1849) applied to the vectors staged in the
 // gather records c->aos ([v_0 .. v_{S-1}, phi^n] per node): out = d0 v + bond sums. With d0_out
 // set, d0 = 1/dt - R' at Un and the block diagonals are written first (d0 is then not read).
