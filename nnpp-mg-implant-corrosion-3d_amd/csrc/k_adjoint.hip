@@ -310,7 +310,7 @@ int nnpp_adjoint_run(nnpp_ctx* c, const double* lam_h, double* grad_h, double* l
       !B.alloc(&rec, sizeof(double) * m * RA) || !B.alloc(&pass, sizeof(uint16_t) * m) ||
       !B.alloc(&part, sizeof(double) * NG * NNPP_RED_GRID) ||
       !B.alloc(&acc, sizeof(double) * (NG + 1)) || !B.alloc(&Usave, fb) || !B.alloc(&Psave, m) ||
-      !B.alloc(&PhiSave, sizeof(double) * m) ||
+      !B.alloc(&PhiSave, sizeof(double) * 2 * m) ||
       !B.alloc(&seg, fb * segn) || !B.alloc(&segP, m * (segn + 1)))
     return nnpp_set_error(c, NNPP_ERR_HIP, "adjoint buffer allocation");
   st_.tape_bytes = T.bytes + (int64_t)(fb * segn + m * (segn + 1));
@@ -321,10 +321,10 @@ int nnpp_adjoint_run(nnpp_ctx* c, const double* lam_h, double* grad_h, double* l
   // the current state, restored at the end
   NNPP_HIP(c, hipMemcpyAsync(Usave, c->U[NNPP_SLOT_N], fb, hipMemcpyDeviceToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(Psave, c->phase, m, hipMemcpyDeviceToDevice, s));
-  const bool save_phi_ok = c->phi_prev_ok;
-  if (save_phi_ok)
-    NNPP_HIP(c, hipMemcpyAsync(PhiSave, c->phi_prev, sizeof(double) * m, hipMemcpyDeviceToDevice,
-                               s));
+  const int save_hist = c->phi_hist;
+  if (save_hist > 0)
+    NNPP_HIP(c, hipMemcpyAsync(PhiSave, c->phi_prev, sizeof(double) * 2 * m,
+                               hipMemcpyDeviceToDevice, s));
   const int64_t save_step = c->step, save_solid = c->solid_count;
   const double save_vr = c->voln1rel;
 
@@ -378,9 +378,9 @@ int nnpp_adjoint_run(nnpp_ctx* c, const double* lam_h, double* grad_h, double* l
     const nnpp_tape_ckpt& K = T.ckpts[ck];
     NNPP_HIP(c, hipMemcpyAsync(c->U[NNPP_SLOT_N], K.U, fb, hipMemcpyDeviceToDevice, s));
     NNPP_HIP(c, hipMemcpyAsync(c->phase, K.phase, m, hipMemcpyDeviceToDevice, s));
-    c->phi_prev_ok = K.phi_prev_ok;
-    if (K.phi_prev_ok)
-      NNPP_HIP(c, hipMemcpyAsync(c->phi_prev, K.phi_prev, sizeof(double) * m,
+    c->phi_hist = K.phi_hist;
+    if (K.phi_hist > 0)
+      NNPP_HIP(c, hipMemcpyAsync(c->phi_prev, K.phi_prev, sizeof(double) * 2 * m,
                                  hipMemcpyDeviceToDevice, s));
     c->step = K.step;
     c->voln1rel = K.voln1rel;
@@ -462,10 +462,10 @@ int nnpp_adjoint_run(nnpp_ctx* c, const double* lam_h, double* grad_h, double* l
   // leave the state as it was before the call
   NNPP_HIP(c, hipMemcpyAsync(c->U[NNPP_SLOT_N], Usave, fb, hipMemcpyDeviceToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(c->phase, Psave, m, hipMemcpyDeviceToDevice, s));
-  if (save_phi_ok)
-    NNPP_HIP(c, hipMemcpyAsync(c->phi_prev, PhiSave, sizeof(double) * m, hipMemcpyDeviceToDevice,
-                               s));
-  c->phi_prev_ok = save_phi_ok;
+  if (save_hist > 0)
+    NNPP_HIP(c, hipMemcpyAsync(c->phi_prev, PhiSave, sizeof(double) * 2 * m,
+                               hipMemcpyDeviceToDevice, s));
+  c->phi_hist = save_hist;
   NNPP_HIP(c, hipStreamSynchronize(s));
   c->step = save_step;
   c->solid_count = save_solid;

@@ -539,7 +539,7 @@ int nnpp_implicit_once(nnpp_ctx* c, int precision, const nnpp_newton_opts& o,
   c->U[NNPP_SLOT_N] = c->U[NNPP_SLOT_N1];
   c->U[NNPP_SLOT_N1] = old_prev;
   c->imp_first = false;
-  c->phi_prev_ok = false;  // the explicit step's potential history no longer applies
+  c->phi_hist = 0;  // the explicit step's potential history no longer applies
   st.steps++;
   st.flipped += flips;
   return status;

@@ -589,13 +589,16 @@ __global__ void k_m_apply(int32_t n, double inv_sb, float* __restrict__ e,
 }
 
 // phi = x on free nodes, Dirichlet value on Dirichlet nodes, unchanged elsewhere.
-__global__ void k_phi_guess(int32_t n, const uint8_t* __restrict__ pfree,
+__global__ void k_phi_guess(int32_t n, int hist, const uint8_t* __restrict__ pfree,
                             const double* __restrict__ phi, const double* __restrict__ prev,
-                            double* __restrict__ out) {
+                            const double* __restrict__ prev2, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double a = phi[i];
-  out[i] = pfree[i] ? 2.0 * a - prev[i] : a;
+  double g = a;
+  if (pfree[i] && hist >= 2) g = 3.0 * (a - prev[i]) + prev2[i];
+  else if (pfree[i] && hist == 1) g = 2.0 * a - prev[i];
+  out[i] = g;
 }
 
 __global__ void k_scatter_phi(int32_t n, int S, const uint8_t* __restrict__ pfree,
@@ -886,9 +889,11 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   return status;
 }
 
-int nnpp_phi_guess_dev(nnpp_ctx* c, const double* phi, const double* prev, double* out) {
+int nnpp_phi_guess_dev(nnpp_ctx* c, int hist, const double* phi, const double* prev,
+                       const double* prev2, double* out) {
   if (!c->n) return NNPP_OK;
-  k_phi_guess<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(c->n, c->pfree, phi, prev, out);
+  k_phi_guess<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(c->n, hist, c->pfree, phi, prev,
+                                                                 prev2, out);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }

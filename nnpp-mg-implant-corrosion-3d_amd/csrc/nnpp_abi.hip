@@ -57,7 +57,7 @@ void free_nodes(nnpp_ctx* c) {
   dfree(c->U[1]);
   dfree(c->Unew);
   dfree(c->phi_prev);
-  c->phi_prev_ok = false;
+  c->phi_hist = 0;
   nnpp_implicit_free(c);
   dfree(c->Fres);
   dfree(c->aos);
@@ -139,21 +139,21 @@ int tape_checkpoint(nnpp_ctx* c) {
   }
   NNPP_HIP(c, hipMemcpyAsync(k.U, c->U[NNPP_SLOT_N], bytes, hipMemcpyDeviceToDevice, c->stream));
   NNPP_HIP(c, hipMemcpyAsync(k.phase, c->phase, m, hipMemcpyDeviceToDevice, c->stream));
-  k.phi_prev_ok = c->phi_prev_ok;
-  if (k.phi_prev_ok) {
-    if (hipMalloc((void**)&k.phi_prev, sizeof(double) * m) != hipSuccess) {
+  k.phi_hist = c->phi_hist;
+  if (k.phi_hist > 0) {
+    if (hipMalloc((void**)&k.phi_prev, sizeof(double) * 2 * m) != hipSuccess) {
       hipFree(k.U);
       hipFree(k.phase);
       return nnpp_set_error(c, NNPP_ERR_HIP, "tape checkpoint allocation");
     }
-    NNPP_HIP(c, hipMemcpyAsync(k.phi_prev, c->phi_prev, sizeof(double) * m,
+    NNPP_HIP(c, hipMemcpyAsync(k.phi_prev, c->phi_prev, sizeof(double) * 2 * m,
                                hipMemcpyDeviceToDevice, c->stream));
   }
   k.step = c->step;
   k.solid_count = c->solid_count;
   k.voln1rel = c->voln1rel;
   c->tape.ckpts.push_back(k);
-  c->tape.bytes += (int64_t)(bytes + m + (k.phi_prev ? sizeof(double) * m : 0));
+  c->tape.bytes += (int64_t)(bytes + m + (k.phi_prev ? sizeof(double) * 2 * m : 0));
   return NNPP_OK;
 }
 
@@ -401,7 +401,7 @@ int nnpp_set_state(nnpp_ctx* c, int32_t slot, const double* U) {
   if (c->n)
     NNPP_HIP(c, hipMemcpy(c->U[slot], U, sizeof(double) * c->n * fields(c), hipMemcpyHostToDevice));
   c->imp_first = true;  // an implicit step starts again from U^n with the Dirichlet values
-  c->phi_prev_ok = false;
+  c->phi_hist = 0;
   return NNPP_OK;
 }
 
@@ -560,7 +560,7 @@ int nnpp_poisson_solve(nnpp_ctx* c, int32_t precision, int32_t* iters, double* r
   int st = ensure_potential(c);
   if (st) return st;
   double* U = c->U[NNPP_SLOT_N];
-  c->phi_prev_ok = false;  // phi^n no longer follows the explicit steps' history
+  c->phi_hist = 0;  // phi^n no longer follows the explicit steps' history
   return nnpp_potential_solve_dev(c, precision, U, U + (int64_t)c->p.n_species * c->n, iters,
                                   relres, nullptr, nullptr);
 }
@@ -591,18 +591,15 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
   const double BV = std::pow(10., -c->p.vol_red * (1. - c->voln1rel));
   st = nnpp_launch_flux(c, 0, precision, BV, U, U, c->Unew);
   if (st) return st;
-  // 2. phi^{n+1} from the charge of U^n, written into Unew; warm start 2 phi^n - phi^{n-1}
-  //    on the free rows once the previous potential is known (phi^n before that)
+  // 2. phi^{n+1} from the charge of U^n, written into Unew; warm start on the free rows by
+  //    extrapolation of phi^n and the previous potentials (quadratic once two are known)
   double* phi_new = c->Unew + (int64_t)S * n;
   const double* phi_n = U + (int64_t)S * n;
   if (n && !c->phi_prev)
-    NNPP_HIP(c, hipMalloc((void**)&c->phi_prev, sizeof(double) * n));
-  if (n && c->phi_prev_ok) {
-    if ((st = nnpp_phi_guess_dev(c, phi_n, c->phi_prev, phi_new))) return st;
-  } else if (n) {
-    NNPP_HIP(c, hipMemcpyAsync(phi_new, phi_n, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                               c->stream));
-  }
+    NNPP_HIP(c, hipMalloc((void**)&c->phi_prev, sizeof(double) * 2 * n));
+  if (n && (st = nnpp_phi_guess_dev(c, c->phi_hist, phi_n, c->phi_prev, c->phi_prev + n,
+                                    phi_new)))
+    return st;
   int32_t its = 0;
   double rr = 0.0;
   int status = NNPP_OK;
@@ -610,10 +607,13 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
                                 &loc.cg_spmv_fp32);
   if (st == NNPP_ERR_NOCONV) status = NNPP_ERR_NOCONV;
   else if (st) return st;
-  if (n)
+  if (n) {  // history: [phi^n, phi^{n-1}]
+    NNPP_HIP(c, hipMemcpyAsync(c->phi_prev + n, c->phi_prev, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, c->stream));
     NNPP_HIP(c, hipMemcpyAsync(c->phi_prev, phi_n, sizeof(double) * n, hipMemcpyDeviceToDevice,
                                c->stream));
-  c->phi_prev_ok = (n > 0);
+  }
+  c->phi_hist = n > 0 ? std::min(c->phi_hist + 1, 2) : 0;
   std::swap(c->U[NNPP_SLOT_N], c->Unew);
   // 3. front update and 4. volume cadence
   int32_t flips = 0;

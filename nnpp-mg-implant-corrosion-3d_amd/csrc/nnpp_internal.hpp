@@ -99,8 +99,8 @@ struct nnpp_tape_step {  // host record of one recorded step
 struct nnpp_tape_ckpt {  // state before recorded step `every * index`
   double* U = nullptr;   // F*n
   uint8_t* phase = nullptr;
-  double* phi_prev = nullptr;  // n: the previous potential (warm start), if phi_prev_ok
-  bool phi_prev_ok = false;
+  double* phi_prev = nullptr;  // 2n: the two previous potentials (warm start)
+  int phi_hist = 0;             // how many of them are valid
   int64_t step = 0, solid_count = 0;
   double voln1rel = 0.0;
 };
@@ -191,10 +191,11 @@ struct nnpp_ctx {
   int imp_S = 0;
   bool imp_first = true;       // next implicit step starts from U^n with the Dirichlet values
 
-  // potential of the previous explicit step, for the linear extrapolation that warm-starts
-  // the next potential solve; invalid after anything but an explicit step changed the state
+  // potentials of the two previous explicit steps ([phi^{n-1}, phi^{n-2}]), for the
+  // extrapolation that warm-starts the next potential solve; phi_hist of them are valid (0
+  // after anything but an explicit step changed the state)
   double* phi_prev = nullptr;
-  bool phi_prev_ok = false;
+  int phi_hist = 0;
 
   int64_t step = 1;
   double voln1rel = 0.0;
@@ -395,8 +396,10 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                              int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32,
                              const double* rhs = nullptr);
 int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out);
-// Warm start of the potential solve: out = 2 phi - prev on the free rows, phi elsewhere.
-int nnpp_phi_guess_dev(nnpp_ctx* c, const double* phi, const double* prev, double* out);
+// Warm start of the potential solve on the free rows (phi elsewhere): hist 2 -> quadratic
+// extrapolation 3 phi - 3 prev + prev2, hist 1 -> linear 2 phi - prev, hist 0 -> phi.
+int nnpp_phi_guess_dev(nnpp_ctx* c, int hist, const double* phi, const double* prev,
+                       const double* prev2, double* out);
 // Species blocks of the frozen Jacobian (utils.h:844-1849) applied to the vectors staged in the
 // gather records c->aos ([v_0 .. v_{S-1}, phi^n] per node): out = d0 v + bond sums. With d0_out
 // set, d0 = 1/dt - R' at Un and the block diagonals are written first (d0 is then not read).
