@@ -244,13 +244,22 @@ __device__ __forceinline__ T spmv_row(const Slices& sl, const T* __restrict__ w,
 // tiles without any entry skip the staging. All threads of the block call this.
 template <typename T>
 __device__ __forceinline__ T mask_weight(T w, int keep);
+// The weight is wave-uniform (a scalar load), the keep mask per lane: one v_and_b32 per 32 bits
+// with the weight as the scalar operand. Written out because the compiler otherwise turns the
+// AND with a sign-extended bit into a compare and a select (twice the instructions).
 template <>
 __device__ __forceinline__ float mask_weight<float>(float w, int keep) {
-  return __int_as_float(__float_as_int(w) & keep);
+  float r;
+  asm("v_and_b32 %0, %1, %2" : "=v"(r) : "s"(w), "v"(keep));
+  return r;
 }
 template <>
 __device__ __forceinline__ double mask_weight<double>(double w, int keep) {
-  return __longlong_as_double(__double_as_longlong(w) & (long long)keep);
+  const uint64_t b = (uint64_t)__double_as_longlong(w);
+  uint32_t lo, hi;
+  asm("v_and_b32 %0, %1, %2" : "=v"(lo) : "s"((uint32_t)b), "v"(keep));
+  asm("v_and_b32 %0, %1, %2" : "=v"(hi) : "s"((uint32_t)(b >> 32)), "v"(keep));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // ORDERED: the result returned to lane j is row r0 + j's (the column form's lane order, which
