@@ -18,8 +18,9 @@
 // RESID: F_k = (U1_k - Un_k)/dt + (sum_KR1 + sum_KR2) - R_k, phi row of utils.h:798-841.
 // !RESID: explicit update U_k^{n+1} = U_k^n - dt (sum_KR1 + sum_KR2 - R_k), Dirichlet rows,
 //         clamp of main.cpp:805-837.
-// MIXED: FP32 weights, FP32 differences taken from the FP64 state, Kahan-compensated FP32 sums
-// per node and species; reactions, the update and the output stay FP64.
+// MIXED: FP32 weights, FP32 differences taken from the FP64 state, FP32 sums over groups of 16
+// bond slots added with Kahan compensation per node and species; reactions, the update and the
+// output stay FP64.
 // JAC: the species blocks of the frozen Jacobian of utils.h:844-1849 applied to v (the gathered
 //      record holds v instead of the iterate): out_k = d0_k v_k + sum_KR_k1(v) + sum_KR_k2(v)
 //      with d0_k = 1/dt - R'_k (R' at U^n), identity rows where field k is Dirichlet. The
@@ -237,11 +238,11 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1 && !mixed) break;
     double sd[S], sm[S];
-    float fd[S], fm[S], cd[S], cm[S];
+    float fd[S], fm[S], cd[S], cm[S], pd[S], pm[S];
 #pragma unroll
     for (int k = 0; k < S; k++) {
       sd[k] = sm[k] = 0.0;
-      fd[k] = fm[k] = cd[k] = cm[k] = 0.0f;
+      fd[k] = fm[k] = cd[k] = cm[k] = pd[k] = pm[k] = 0.0f;
     }
     int32_t jr = WIDE ? 0 : a.first[i];
     for (int gq = 0; gq < G; gq++) {
@@ -300,11 +301,11 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
         const double dphi = v[S] - phii;
         if (JAC) sw[pass] += w;
         if (MIXED) {
-          const float wf = (float)w, dpf = (float)dphi;
+          const float wf = (float)w, wdp = wf * (float)dphi;
 #pragma unroll
           for (int k = 0; k < S; k++) {
-            kahan_add(fd[k], cd[k], wf * (float)(v[k] - ci[k]));
-            kahan_add(fm[k], cm[k], (wf * (float)v[k]) * dpf);
+            pd[k] = __fmaf_rn(wf, (float)(v[k] - ci[k]), pd[k]);
+            pm[k] = __fmaf_rn(wdp, (float)v[k], pm[k]);
           }
         } else {
           const double wdphi = w * dphi;
@@ -317,6 +318,14 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
         if (RESID && pass == 0 && pi == 0) {  // liquid-liquid bond of the phi row
           sphi += w * (v[S + 1] - phi1i);
           flagLL |= (w != 0.0);
+        }
+      }
+      if (MIXED && ((gq & 3) == 3 || gq == G - 1)) {  // fold the group of 16 bond slots
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+          kahan_add(fd[k], cd[k], pd[k]);
+          kahan_add(fm[k], cm[k], pm[k]);
+          pd[k] = pm[k] = 0.0f;
         }
       }
     }
