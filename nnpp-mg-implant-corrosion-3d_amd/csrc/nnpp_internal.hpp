@@ -66,7 +66,7 @@ struct nnpp_slices {
 // lds[l + lc[t]]. The staging walks the segments in pieces of at most 256 values, one value per
 // thread and piece.
 #define NNPP_STENCIL_MAX 128
-#define NNPP_STENCIL_CHUNK 16  // entries per step of the stencil SpMV; tables padded to it
+#define NNPP_STENCIL_CHUNK 32  // tables padded to a multiple of the SpMV chunks (32 / 16)
 struct nnpp_stencil {
   int state = 0;               // 0: not built for this geometry, 1: valid, -1: not a stencil
   int K = 0, R = 0, P = 0;     // offsets, segments, staging pieces
