@@ -74,7 +74,7 @@ class StepStats(C.Structure):
         ("cg_spmv_fp32", C.c_int64), ("last_relres", C.c_double), ("flipped", C.c_int64),
         ("solid_count", C.c_int64), ("voln1rel", C.c_double), ("nnz_flux", C.c_int64),
         ("nnz_ll", C.c_int64), ("pot_col_bytes", C.c_int32), ("pot_weight_bytes", C.c_int32),
-        ("pot_stencil", C.c_int32), ("reserved0", C.c_int32),
+        ("pot_stencil", C.c_int32), ("flux_stencil", C.c_int32),
     ]
 
     def as_dict(self):

@@ -488,32 +488,39 @@ __global__ void k_st_collect(int32_t n, const int64_t* __restrict__ row_ptr,
   }
 }
 
-// Weight of each offset from one bond that has it (the same arithmetic as k_pot_fill).
+// Weight of each offset from one bond that has it (the same arithmetic as k_pot_fill and
+// k_slice_fill): potential exponent (tw64/tw32) and flux exponent (tf64).
 __global__ void k_st_weights(int K, const int32_t* __restrict__ toff,
                              const int32_t* __restrict__ rep, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
-                             WeightArgs wa, double* __restrict__ tw64,
-                             float* __restrict__ tw32) {
+                             WeightArgs wa, WeightArgs wf, double* __restrict__ tw64,
+                             float* __restrict__ tw32, double* __restrict__ tf64) {
   const int t = threadIdx.x;
   if (t >= K) return;
   const int64_t i = rep[t];
   const double w = bond_weight(wa, x, y, z, i, (int32_t)(i + toff[t]));
   tw64[t] = w;
   tw32[t] = (float)w;
+  tf64[t] = bond_weight(wf, x, y, z, i, (int32_t)(i + toff[t]));
 }
 
 // Row masks of all bonds; fails if an entry's offset is not in the table, the entries of a
 // row are not in ascending order, or a bond's weight differs (in any bit) from its offset's.
+// fail_flux: some bond's flux weight differs from its offset's.
 __global__ void k_st_mask(int32_t n, int K, const int64_t* __restrict__ row_ptr,
                           const int32_t* __restrict__ col, const int32_t* __restrict__ toff,
-                          const double* __restrict__ tw64, const double* __restrict__ x,
-                          const double* __restrict__ y, const double* __restrict__ z,
-                          WeightArgs wa, uint4* __restrict__ bmask, int* __restrict__ fail) {
+                          const double* __restrict__ tw64, const double* __restrict__ tf64,
+                          const double* __restrict__ x, const double* __restrict__ y,
+                          const double* __restrict__ z, WeightArgs wa, WeightArgs wf,
+                          uint4* __restrict__ bmask, int* __restrict__ fail,
+                          int* __restrict__ fail_flux) {
   __shared__ int32_t t_off[NNPP_STENCIL_MAX];
   __shared__ double t_w[NNPP_STENCIL_MAX];
+  __shared__ double t_f[NNPP_STENCIL_MAX];
   for (int t = threadIdx.x; t < K; t += blockDim.x) {
     t_off[t] = toff[t];
     t_w[t] = tw64[t];
+    t_f[t] = tf64[t];
   }
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -538,6 +545,8 @@ __global__ void k_st_mask(int32_t n, int K, const int64_t* __restrict__ row_ptr,
       bad = 1;
       break;
     }
+    if (__double_as_longlong(bond_weight(wf, x, y, z, i, j)) != __double_as_longlong(t_f[lo]))
+      atomicExch(fail_flux, 1);
     m[lo >> 5] |= 1u << (lo & 31);
     prev = lo;
   }
@@ -753,7 +762,7 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
 }
 
 void nnpp_free_stencil(nnpp_stencil& t) {
-  void* ps[] = {t.toff, t.lc, t.seg_start, t.seg_len, t.seg_base, t.piece, t.tw64, t.tw32,
+  void* ps[] = {t.toff, t.lc, t.seg_start, t.seg_len, t.seg_base, t.piece, t.tw64, t.tw32, t.tf64,
                 t.bmask, t.pmask, t.perm};
   for (void* q : ps)
     if (q) hipFree(q);
@@ -771,10 +780,10 @@ static int build_stencil(nnpp_ctx* c) {
   if (n == 0 || c->nnz <= 0 || c->vol) return NNPP_OK;
   unsigned long long* table = nullptr;
   int32_t* rep = nullptr;
-  int* flags = nullptr;  // [0] count, [1] fail
+  int* flags = nullptr;  // [0] count, [1] fail, [2] flux weights fail
   NNPP_HIP(c, hipMalloc((void**)&table, sizeof(unsigned long long) * DICT_HASH));
   NNPP_HIP(c, hipMalloc((void**)&rep, sizeof(int32_t) * DICT_HASH));
-  NNPP_HIP(c, hipMalloc((void**)&flags, sizeof(int) * 2));
+  NNPP_HIP(c, hipMalloc((void**)&flags, sizeof(int) * 3));
   NNPP_HIP(c, hipMemsetAsync(table, 0, sizeof(unsigned long long) * DICT_HASH, s));
   NNPP_HIP(c, hipMemsetAsync(flags, 0, sizeof(int) * 2, s));
   k_st_collect<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->row_ptr, c->col, table, rep, flags,
@@ -782,7 +791,7 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipGetLastError());
   std::vector<unsigned long long> ht(DICT_HASH);
   std::vector<int32_t> hr(DICT_HASH);
-  int hf[2] = {0, 0};
+  int hf[3] = {0, 0, 0};
   NNPP_HIP(c, hipMemcpyAsync(ht.data(), table, sizeof(unsigned long long) * DICT_HASH,
                              hipMemcpyDeviceToHost, s));
   NNPP_HIP(c, hipMemcpyAsync(hr.data(), rep, sizeof(int32_t) * DICT_HASH, hipMemcpyDeviceToHost, s));
@@ -860,6 +869,8 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMalloc((void**)&T.tw32, sizeof(float) * Kp));
   NNPP_HIP(c, hipMemsetAsync(T.tw64, 0, sizeof(double) * Kp, s));
   NNPP_HIP(c, hipMemsetAsync(T.tw32, 0, sizeof(float) * Kp, s));
+  NNPP_HIP(c, hipMalloc((void**)&T.tf64, sizeof(double) * Kp));
+  NNPP_HIP(c, hipMemsetAsync(T.tf64, 0, sizeof(double) * Kp, s));
   NNPP_HIP(c, hipMalloc((void**)&T.bmask, sizeof(uint4) * n));
   NNPP_HIP(c, hipMalloc((void**)&T.pmask, sizeof(uint4) * n));
   NNPP_HIP(c, hipMalloc((void**)&T.perm, (size_t)nnpp_nblocks(n) * NNPP_BLOCK));
@@ -872,13 +883,17 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMemcpyAsync(T.piece, pc.data(), sizeof(int32_t) * 3 * P, hipMemcpyHostToDevice,
                              s));
   const WeightArgs wa = weight_args(c, 3.0 + 2.0 * c->p.s_phi);
-  k_st_weights<<<1, NNPP_STENCIL_MAX, 0, s>>>(K, T.toff, reps_d, c->x, c->y, c->z, wa, T.tw64,
-                                              T.tw32);
-  NNPP_HIP(c, hipMemsetAsync(flags, 0, sizeof(int) * 2, s));
+  const WeightArgs wf = weight_args(c, c->w_exponent);
+  k_st_weights<<<1, NNPP_STENCIL_MAX, 0, s>>>(K, T.toff, reps_d, c->x, c->y, c->z, wa, wf,
+                                              T.tw64, T.tw32, T.tf64);
+  NNPP_HIP(c, hipMemsetAsync(flags, 0, sizeof(int) * 3, s));
   k_st_mask<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, K, c->row_ptr, c->col, T.toff, T.tw64,
-                                                    c->x, c->y, c->z, wa, T.bmask, flags + 1);
+                                                    T.tf64, c->x, c->y, c->z, wa, wf, T.bmask,
+                                                    flags + 1, flags + 2);
   NNPP_HIP(c, hipGetLastError());
-  NNPP_HIP(c, hipMemcpyAsync(hf, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+  std::vector<double> tf(K);
+  NNPP_HIP(c, hipMemcpyAsync(hf, flags, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(tf.data(), T.tf64, sizeof(double) * K, hipMemcpyDeviceToHost, s));
   NNPP_HIP(c, hipStreamSynchronize(s));
   hipFree(reps_d);
   hipFree(flags);
@@ -887,6 +902,9 @@ static int build_stencil(nnpp_ctx* c) {
     T.state = -1;
     return NNPP_OK;
   }
+  T.flux_ok = hf[2] ? 0 : 1;
+  for (int t = 0; t < K; t++)
+    if (tf[t] != 0.0) T.fnz[t >> 5] |= 1u << (t & 31);
   T.state = 1;
   return NNPP_OK;
 }
@@ -915,6 +933,11 @@ static int build_potential_stencil(nnpp_ctx* c) {
   c->pot_nfree = (int64_t)c->h_scal[SC_R1];
   T.active = 1;
   c->pot_dirty = false;
+  return NNPP_OK;
+}
+
+int nnpp_stencil_ensure(nnpp_ctx* c) {
+  if (c->pst.state == 0 && c->n && !c->vol) return build_stencil(c);
   return NNPP_OK;
 }
 

@@ -55,14 +55,6 @@ struct Slices {
   int off_min, off_max;  // smallest and largest offset
 };
 
-// Load from a table that no kernel writes while it reads it, through the constant address
-// space: with a wave-uniform index it becomes a scalar load (batched by the compiler) instead of
-// a vector load of one address.
-template <typename T>
-__device__ __forceinline__ T ld_table(const T* p, int i) {
-  return ((const __attribute__((address_space(4))) T*)p)[i];
-}
-
 template <typename T>
 __device__ __forceinline__ const T* stencil_weights(const Slices& sl);
 template <>
@@ -243,26 +235,6 @@ __device__ __forceinline__ T spmv_row(const Slices& sl, const T* __restrict__ w,
 // form (bit for bit). FP32: two fused multiply-add chains per chunk (even and odd entries),
 // the chunk sums Kahan-compensated. The leading barrier orders the staging after the previous tile's reads;
 // tiles without any entry skip the staging. All threads of the block call this.
-template <typename T>
-__device__ __forceinline__ T mask_weight(T w, int keep);
-// The weight is wave-uniform (a scalar load), the keep mask per lane: one v_and_b32 per 32 bits
-// with the weight as the scalar operand. Written out because the compiler otherwise turns the
-// AND with a sign-extended bit into a compare and a select (twice the instructions).
-template <>
-__device__ __forceinline__ float mask_weight<float>(float w, int keep) {
-  float r;
-  asm("v_and_b32 %0, %1, %2" : "=v"(r) : "s"(w), "v"(keep));
-  return r;
-}
-template <>
-__device__ __forceinline__ double mask_weight<double>(double w, int keep) {
-  const uint64_t b = (uint64_t)__double_as_longlong(w);
-  uint32_t lo, hi;
-  asm("v_and_b32 %0, %1, %2" : "=v"(lo) : "s"((uint32_t)b), "v"(keep));
-  asm("v_and_b32 %0, %1, %2" : "=v"(hi) : "s"((uint32_t)(b >> 32)), "v"(keep));
-  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
 // ORDERED: the result returned to lane j is row r0 + j's (the column form's lane order, which
 // the FP64 dot products need to match it bit for bit); otherwise lane j gets the result of the
 // row it computed, *row_out.
@@ -289,21 +261,21 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
         T val[PB];
 #pragma unroll
         for (int k = 0; k < PB; k++) {
-          if (pb + k < P && (int)threadIdx.x < ld_table(sl.piece, 3 * (pb + k) + 2))
-            val[k] = v[r0 + ld_table(sl.piece, 3 * (pb + k)) + threadIdx.x];
+          if (pb + k < P && (int)threadIdx.x < nnpp_ld_table(sl.piece, 3 * (pb + k) + 2))
+            val[k] = v[r0 + nnpp_ld_table(sl.piece, 3 * (pb + k)) + threadIdx.x];
         }
 #pragma unroll
         for (int k = 0; k < PB; k++) {
-          if (pb + k < P && (int)threadIdx.x < ld_table(sl.piece, 3 * (pb + k) + 2))
-            lds[ld_table(sl.piece, 3 * (pb + k) + 1) + threadIdx.x] = val[k];
+          if (pb + k < P && (int)threadIdx.x < nnpp_ld_table(sl.piece, 3 * (pb + k) + 2))
+            lds[nnpp_ld_table(sl.piece, 3 * (pb + k) + 1) + threadIdx.x] = val[k];
         }
       }
     } else {
       const int R = sl.R;
       for (int r = 0; r < R; r++) {
-        const int64_t g0 = r0 + ld_table(sl.seg_start, r);
-        const int len = NNPP_BLOCK + ld_table(sl.seg_len, r) - 1;
-        T* dst = lds + ld_table(sl.seg_base, r);
+        const int64_t g0 = r0 + nnpp_ld_table(sl.seg_start, r);
+        const int len = NNPP_BLOCK + nnpp_ld_table(sl.seg_len, r) - 1;
+        T* dst = lds + nnpp_ld_table(sl.seg_base, r);
         for (int u = threadIdx.x; u < len; u += NNPP_BLOCK) {
           const int64_t g = g0 + u;
           dst[u] = (g >= 0 && g < n) ? v[g] : T(0);
@@ -333,8 +305,8 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
       T wv[BT], pv[BT];
 #pragma unroll
       for (int k = 0; k < BT; k++) {  // tables padded to a multiple of BT entries (weight 0)
-        lv[k] = ld_table(sl.lc, t0 + k);
-        wv[k] = ld_table(tw, t0 + k);
+        lv[k] = nnpp_ld_table(sl.lc, t0 + k);
+        wv[k] = nnpp_ld_table(tw, t0 + k);
       }
 #pragma unroll
       for (int k = 0; k < BT; k++) pv[k] = row[lv[k]];
@@ -349,7 +321,7 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
 #pragma unroll
         for (int k = 0; k < BT; k++) {
           const int keep = (int)(bits << (31 - k)) >> 31;  // bit k -> all ones, else 0
-          const T wk = mask_weight<T>(wv[k], keep);
+          const T wk = nnpp_mask_weight<T>(wv[k], keep);
           if constexpr (sizeof(T) == 8) acc.add(wk, vi - pv[k]);
           else part[k & 1] = __fmaf_rn(wk, vi - pv[k], part[k & 1]);
         }

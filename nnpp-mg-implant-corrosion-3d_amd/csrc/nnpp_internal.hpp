@@ -82,6 +82,10 @@ struct nnpp_stencil {
                                  // row, LDS position, value count (1..256)
   double* tw64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] potential weight of
   float* tw32 = nullptr;       // offset t (FP64 / FP32; padding: 0)
+  double* tf64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] flux weight of offset t
+                               // (flux exponent; padding: 0)
+  int flux_ok = 0;             // every bond's flux weight equals its offset's (bit for bit)
+  uint32_t fnz[4] = {0u, 0u, 0u, 0u};  // offsets with a nonzero flux weight
   uint4* bmask = nullptr;      // [n] bonds of each row
   uint4* pmask = nullptr;      // [n] potential-operator entries of each row
   uint8_t* perm = nullptr;     // [256 per tile] row of each lane: rows with every entry first,
@@ -300,6 +304,35 @@ __device__ __forceinline__ double nnpp_ld_scalar(const double* __restrict__ p) {
 // [x*per, (x+1)*per) of node blocks, so neighbour data stays in its L2.
 #define NNPP_RED_GRID 2048
 
+// Load from a table that no kernel writes while it reads it, through the constant address
+// space: with a wave-uniform index it becomes a scalar load (batched by the compiler) instead of
+// a vector load of one address.
+template <typename T>
+__device__ __forceinline__ T nnpp_ld_table(const T* p, int i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// w where keep is all ones, +0 where keep is 0. The weight must be wave-uniform (a table value
+// read with nnpp_ld_table), the keep mask is per lane: one v_and_b32 per 32 bits with the weight
+// as the scalar operand. Written out because the compiler otherwise turns the AND with a
+// sign-extended bit into a compare and a select (twice the instructions).
+template <typename T>
+__device__ __forceinline__ T nnpp_mask_weight(T w, int keep);
+template <>
+__device__ __forceinline__ float nnpp_mask_weight<float>(float w, int keep) {
+  float r;
+  asm("v_and_b32 %0, %1, %2" : "=v"(r) : "s"(w), "v"(keep));
+  return r;
+}
+template <>
+__device__ __forceinline__ double nnpp_mask_weight<double>(double w, int keep) {
+  const uint64_t b = (uint64_t)__double_as_longlong(w);
+  uint32_t lo, hi;
+  asm("v_and_b32 %0, %1, %2" : "=v"(lo) : "s"((uint32_t)b), "v"(keep));
+  asm("v_and_b32 %0, %1, %2" : "=v"(hi) : "s"((uint32_t)(b >> 32)), "v"(keep));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 struct nnpp_block_iter {
   int64_t cur, end, stride;
 };
@@ -427,3 +460,9 @@ int nnpp_adjoint_run(nnpp_ctx* c, const double* lam, double* grad, double* lam0,
                      nnpp_adjoint_stats* stats);
 void nnpp_tape_free(nnpp_ctx* c);
 void nnpp_free_stencil(nnpp_stencil& t);
+// Builds the stencil tables of the current geometry if not yet known (c->pst.state 0 -> 1 or
+// -1); NNPP_OK either way.
+int nnpp_stencil_ensure(nnpp_ctx* c);
+// 1 where the explicit flux of this precision takes the lattice stencil form for the current
+// geometry.
+int nnpp_flux_stencil_active(nnpp_ctx* c, int precision);

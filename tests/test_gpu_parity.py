@@ -237,7 +237,8 @@ def test_mixed_step_close_to_fp64():
 
 @pytest.mark.parametrize("kind", ["regular", "screw"])
 def test_storage_formats_bit_identical(kind):
-    """Stencil form (lattices), 16-bit column steps vs 32-bit columns, dictionary weight classes
+    """Stencil forms (lattices: the potential operator and the explicit flux), 16-bit column
+    steps vs 32-bit columns, dictionary weight classes
     vs stored weights: the same additions in the same order, so the trajectories agree bit for
     bit."""
     say("case storage formats", kind)
@@ -266,6 +267,8 @@ def test_storage_formats_bit_identical(kind):
         for fmt, (U, st) in out.items():
             stencil = kind == "regular" and not (fmt & T)
             assert st["pot_stencil"] == int(stencil), fmt
+            # the FP64 explicit flux takes the stencil form wherever the operator does
+            assert st["flux_stencil"] == int(stencil and prec == nn.FP64), fmt
             if stencil:
                 assert st["pot_col_bytes"] == 0 and st["pot_weight_bytes"] == 0
             else:
