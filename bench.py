@@ -477,15 +477,22 @@ def main():
         bytes32 = nll * (cb + wb32) + n * (4 + 3 * 4)
         bytes64 = nll * (cb + wb64) + n * (4 + 3 * 8)
     S = case.S
-    fbytes = lambda wb: nnz * (4 + wb) + n * (8 * (S + 1) + 1 + 2 + 8 * S)  # noqa: E731
+    # flux bytes per launch: column form, per bond a column and a weight; stencil form (lattices,
+    # FP64 in both precisions), a 16-byte bond mask per row; both: the fields read once, phase,
+    # Dirichlet mask and the updated fields
+    fstencil = bool(st_["flux_stencil"])
+    fbytes = lambda wb: ((n * 16 if fstencil else nnz * (4 + wb))  # noqa: E731
+                         + n * (8 * (S + 1) + 1 + 2 + 8 * S))
     ach = alg_bytes / (t_spmv32 * 1e-3) / 1e9
     kern = {
         "pot_spmv_fp32": {"ms": t_spmv32, "alg_GBps": alg_bytes / (t_spmv32 * 1e-3) / 1e9,
                           "format_GBps": bytes32 / (t_spmv32 * 1e-3) / 1e9},
         "pot_spmv_fp64": {"ms": t_spmv64, "alg_GBps": alg_bytes / (t_spmv64 * 1e-3) / 1e9,
                           "format_GBps": bytes64 / (t_spmv64 * 1e-3) / 1e9},
-        "flux_fp64": {"ms": t_flux64, "GBps": fbytes(8) / (t_flux64 * 1e-3) / 1e9},
-        "flux_mixed": {"ms": t_flux32, "GBps": fbytes(4) / (t_flux32 * 1e-3) / 1e9},
+        "flux_fp64": {"ms": t_flux64, "form": "stencil" if fstencil else "columns",
+                      "GBps": fbytes(8) / (t_flux64 * 1e-3) / 1e9},
+        "flux_mixed": {"ms": t_flux32, "form": "stencil (fp64)" if fstencil else "columns (fp32)",
+                       "GBps": fbytes(4) / (t_flux32 * 1e-3) / 1e9},
     }
     for r_ in res.values():
         r_["pb"].close()
@@ -506,7 +513,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 bond sums (Kahan) / f64 state, dots, residual checks",
+        "dtype": ("f32 CG inner iterations (Kahan bond sums); f64 state, dots, residual checks; "
+                  "flux " + ("f64 (lattice stencil form)" if fstencil else "f32 bond sums (Kahan)")),
         "data": "synthetic (seeded lattice case in the reference's input layout)",
         "config": {"workload": (f"regular {ng}^3 lattice, 11 species + phi, delta=3dx, Mg sphere "
                                 f"r=0.3, Dirichlet faces, CG rtol {a.rtol:g}" if a.case == "regular"

@@ -126,7 +126,8 @@ typedef struct nnpp_step_stats {
                                class) or 4 (FP32 value); 0 for the stencil form */
   int32_t pot_stencil;      /* the operator used the stencil form (16 bytes per row, no
                                per-entry stream; pot_col_bytes = 0) */
-  int32_t flux_stencil;     /* the explicit flux used the lattice stencil form (FP64 steps) */
+  int32_t flux_stencil;     /* the explicit flux used the lattice stencil form (FP64 in both
+                               precisions) */
 } nnpp_step_stats;
 
 typedef struct nnpp_ctx nnpp_ctx;

@@ -763,7 +763,7 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
 
 void nnpp_free_stencil(nnpp_stencil& t) {
   void* ps[] = {t.toff, t.lc, t.seg_start, t.seg_len, t.seg_base, t.piece, t.tw64, t.tw32, t.tf64,
-                t.bmask, t.pmask, t.perm};
+                t.bmask, t.fperm, t.pmask, t.perm};
   for (void* q : ps)
     if (q) hipFree(q);
   t = nnpp_stencil();
@@ -874,6 +874,7 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMalloc((void**)&T.bmask, sizeof(uint4) * n));
   NNPP_HIP(c, hipMalloc((void**)&T.pmask, sizeof(uint4) * n));
   NNPP_HIP(c, hipMalloc((void**)&T.perm, (size_t)nnpp_nblocks(n) * NNPP_BLOCK));
+  NNPP_HIP(c, hipMalloc((void**)&T.fperm, (size_t)nnpp_nblocks(n) * NNPP_BLOCK));
   NNPP_HIP(c, hipMemcpyAsync(T.toff, toff.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(T.lc, lc.data(), sizeof(int32_t) * Kp, hipMemcpyHostToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(reps_d, reps.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice, s));
@@ -903,6 +904,8 @@ static int build_stencil(nnpp_ctx* c) {
     return NNPP_OK;
   }
   T.flux_ok = hf[2] ? 0 : 1;
+  k_st_perm<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, K, T.bmask, T.fperm);
+  NNPP_HIP(c, hipGetLastError());
   for (int t = 0; t < K; t++)
     if (tf[t] != 0.0) T.fnz[t >> 5] |= 1u << (t & 31);
   T.state = 1;

@@ -87,6 +87,8 @@ struct nnpp_stencil {
   int flux_ok = 0;             // every bond's flux weight equals its offset's (bit for bit)
   uint32_t fnz[4] = {0u, 0u, 0u, 0u};  // offsets with a nonzero flux weight
   uint4* bmask = nullptr;      // [n] bonds of each row
+  uint8_t* fperm = nullptr;    // [256 per tile] lane order of the stencil flux: rows with every
+                               // bond first (as perm, by bmask)
   uint4* pmask = nullptr;      // [n] potential-operator entries of each row
   uint8_t* perm = nullptr;     // [256 per tile] row of each lane: rows with every entry first,
                                // then the other rows with entries, then the rest (stable)
@@ -463,6 +465,6 @@ void nnpp_free_stencil(nnpp_stencil& t);
 // Builds the stencil tables of the current geometry if not yet known (c->pst.state 0 -> 1 or
 // -1); NNPP_OK either way.
 int nnpp_stencil_ensure(nnpp_ctx* c);
-// 1 where the explicit flux of this precision takes the lattice stencil form for the current
-// geometry.
-int nnpp_flux_stencil_active(nnpp_ctx* c, int precision);
+// 1 where the explicit flux takes the lattice stencil form (FP64 in both precisions) for the
+// current geometry.
+int nnpp_flux_stencil_active(nnpp_ctx* c);
