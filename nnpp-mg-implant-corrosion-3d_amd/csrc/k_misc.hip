@@ -278,17 +278,12 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_nonfinite(int64_t m, const doubl
 __global__ __launch_bounds__(NNPP_BLOCK) void k_finalize(int op, const double* __restrict__ part,
                                                          int64_t nblk, double* __restrict__ scal,
                                                          double a0) {
-  if ((op == FIN_PQ || op == FIN_UPDATE || op == FIN_M_UPDATE) && scal[SC_DONE] != 0.0) return;
-  if (op == FIN_BNORM2 || op == FIN_PQ || op == FIN_NONFINITE) {
+  if (op == FIN_BNORM2 || op == FIN_NONFINITE) {
     double v[1];
     nnpp_final_sum<1>(v, part, nblk);
     if (threadIdx.x != 0) return;
     if (op == FIN_BNORM2) scal[SC_BNORM2] = v[0];
-    else if (op == FIN_NONFINITE) scal[SC_R3] = v[0];
-    else {
-      scal[SC_PQ] = v[0];
-      scal[SC_ALPHA] = scal[SC_RHO] / v[0];
-    }
+    else scal[SC_R3] = v[0];
   } else if (op == FIN_FRONT) {
     double v[3];
     nnpp_final_sum<3>(v, part, nblk);
@@ -313,13 +308,6 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_finalize(int op, const double* _
         scal[SC_DONE] = (v[1] <= t2) ? 1.0 : 0.0;
         break;
       }
-      case FIN_UPDATE:
-        scal[SC_BETA] = v[0] / scal[SC_RHO];
-        scal[SC_RHO] = v[0];
-        scal[SC_RR] = v[1];
-        scal[SC_ITERS] += 1.0;
-        if (v[1] <= scal[SC_TARGET2]) scal[SC_DONE] = 1.0;
-        break;
       case FIN_M_REPLACE:
       case FIN_M_REPLACE_START:
         scal[SC_RHO] = v[0];
@@ -330,16 +318,6 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_finalize(int op, const double* _
         // after a replacement the FP64 test has just failed: run at least one more iteration
         scal[SC_DONE] = (op == FIN_M_REPLACE_START && v[1] <= a0) ? 1.0 : 0.0;
         break;
-      case FIN_M_UPDATE: {
-        scal[SC_BETA] = v[0] / scal[SC_RHO];
-        scal[SC_RHO] = v[0];
-        scal[SC_RR] = v[1];
-        scal[SC_ITERS] += 1.0;
-        const double mx = scal[SC_RRMAX] > v[1] ? scal[SC_RRMAX] : v[1];
-        scal[SC_RRMAX] = mx;
-        if (v[1] <= scal[SC_TARGET2] || v[1] <= a0 * mx) scal[SC_DONE] = 1.0;
-        break;
-      }
       default:
         break;
     }

@@ -10,8 +10,10 @@
 // for smooth vectors.
 //
 // FP64: Jacobi-preconditioned CG, three kernels per iteration (SpMV + dot, update + dots,
-// direction update); all scalars stay on the device and a converged flag makes the remaining
-// launches of a batch exit at once, so the host synchronises once per batch.
+// direction update). The scalars stay on the device: each kernel that needs alpha or beta sums
+// the previous kernel's block partials itself (every block in the same order, so all agree; no
+// one-block reduction launch in between), and a converged flag makes the remaining launches of
+// a batch exit at once, so the host synchronises once per batch.
 // MIXED: CG with reliable updates. The iteration runs in FP32 (FP32 weights and vectors,
 // Kahan-compensated bond sums, FP64 dot products) on the residual normalised by ||b||; the
 // solution is kept in FP64. Whenever the FP32 recursive residual has dropped by a factor 100
@@ -440,14 +442,57 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
   nnpp_block_partial<1>(v, part);
 }
 
+// Slot of rho in iteration parity par.
+__host__ __device__ constexpr int rho_slot(int par) { return par ? SC_RHO1 : SC_RHO; }
+
+// alpha = rho / p.Ap from the SpMV's block partials (every block sums them in the same order;
+// block 0 records PQ and ALPHA).
+__device__ __forceinline__ double cg_alpha(const double* __restrict__ part_s, int64_t nblk_s,
+                                           double* scal, int par) {
+  double pq[1];
+  nnpp_final_sum_all<1>(pq, part_s, nblk_s);
+  const double alpha = nnpp_ld_scalar(scal + rho_slot(par)) / pq[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[SC_PQ] = pq[0];
+    scal[SC_ALPHA] = alpha;
+  }
+  return alpha;
+}
+
+// beta, ||r||^2 and the stop test from the update's block partials (r.z, r.r). Block 0 records
+// BETA, RR, RRMAX, ITERS, DONE and the new rho in the other slot (no block reads that slot in
+// this kernel). reliable > 0 (mixed): also stop once ||r||^2 <= reliable * RRMAX; RRMAX is a
+// running maximum, so a block that reads it after block 0 updated it takes the same decision.
+__device__ __forceinline__ bool cg_beta(const double* __restrict__ part_u, int64_t nblk_u,
+                                        double* scal, int par, double reliable, double& beta) {
+  double v[2];
+  nnpp_final_sum_all<2>(v, part_u, nblk_u);
+  beta = v[0] / nnpp_ld_scalar(scal + rho_slot(par));
+  const double mx = fmax(nnpp_ld_scalar(scal + SC_RRMAX), v[1]);
+  const bool done = v[1] <= nnpp_ld_scalar(scal + SC_TARGET2) || (reliable > 0.0 && v[1] <= reliable * mx);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[SC_BETA] = beta;
+    scal[rho_slot(par ^ 1)] = v[0];
+    scal[SC_RR] = v[1];
+    scal[SC_ITERS] += 1.0;
+    if (reliable > 0.0) scal[SC_RRMAX] = mx;
+    if (done) scal[SC_DONE] = 1.0;
+  }
+  return done;
+}
+
+// x += alpha p, r -= alpha q, z = D^-1 r; partials r.z, r.r into part_u.
 __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_update(int32_t n, const double* __restrict__ pinv,
                                                           const double* __restrict__ p,
                                                           const double* __restrict__ q,
                                                           double* __restrict__ x,
                                                           double* __restrict__ r,
-                                                          double* __restrict__ zv, double* __restrict__ part, const double* __restrict__ scal) {
+                                                          double* __restrict__ zv,
+                                                          double* __restrict__ part_u,
+                                                          const double* __restrict__ part_s,
+                                                          int64_t nblk_s, double* scal, int par) {
   if (cg_done(scal)) return;
-  const double alpha = nnpp_ld_scalar(scal + SC_ALPHA);
+  const double alpha = cg_alpha(part_s, nblk_s, scal, par);
   double v[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * NNPP_BLOCK) {
@@ -459,15 +504,20 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_update(int32_t n, const doubl
     v[0] += ri * zi;
     v[1] += ri * ri;
   }
-  nnpp_block_partial<2>(v, part);
+  nnpp_block_partial<2>(v, part_u);
 }
 
-__global__ void k_cg_dir(int32_t n, const double* __restrict__ zv, double* __restrict__ p,
-                         const double* scal) {
+// p = z + beta p, after the stop test (the iteration that stops skips it).
+__global__ __launch_bounds__(NNPP_BLOCK) void k_cg_dir(int32_t n, const double* __restrict__ zv,
+                                                       double* __restrict__ p,
+                                                       const double* __restrict__ part_u,
+                                                       int64_t nblk_u, double* scal, int par) {
   if (cg_done(scal)) return;
-  const double beta = nnpp_ld_scalar(scal + SC_BETA);
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = zv[i] + beta * p[i];
+  double beta;
+  if (cg_beta(part_u, nblk_u, scal, par, 0.0, beta)) return;
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK)
+    p[i] = zv[i] + beta * p[i];
 }
 
 // ---- mixed precision: CG with reliable updates ----
@@ -532,10 +582,11 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_update(int32_t n, const float*
                                                          float* __restrict__ e,
                                                          float* __restrict__ r,
                                                          float* __restrict__ zv,
-                                                         double* __restrict__ part,
-                                                         const double* __restrict__ scal) {
+                                                         double* __restrict__ part_u,
+                                                         const double* __restrict__ part_s,
+                                                         int64_t nblk_s, double* scal, int par) {
   if (cg_done(scal)) return;
-  const float alpha = (float)nnpp_ld_scalar(scal + SC_ALPHA);
+  const float alpha = (float)cg_alpha(part_s, nblk_s, scal, par);
   double v[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * NNPP_BLOCK) {
@@ -547,17 +598,31 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_update(int32_t n, const float*
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
   }
-  nnpp_block_partial<2>(v, part);
+  nnpp_block_partial<2>(v, part_u);
 }
 
-// p = z + beta p. The iteration that stops a batch skips it (the flag is already set), so the
-// host runs it once with FORCE before a residual replacement: continuing along a direction that
-// missed its update would take the step length r.z / p.Ap, which only holds for r orthogonal
-// to the previous direction, and diverges.
-template <bool FORCE>
-__global__ void k_m_dir(int32_t n, const float* __restrict__ zv, float* __restrict__ p,
-                        const double* scal) {
-  if (!FORCE && cg_done(scal)) return;
+// p = z + beta p after the stop test (beta, the test and the scalars from the update's
+// partials). The iteration that stops a batch skips it, so the host runs k_m_dir_force once
+// before a residual replacement: continuing along a direction that missed its update would take
+// the step length r.z / p.Ap, which only holds for r orthogonal to the previous direction, and
+// diverges.
+__global__ __launch_bounds__(NNPP_BLOCK) void k_m_dir(int32_t n, const float* __restrict__ zv,
+                                                      float* __restrict__ p,
+                                                      const double* __restrict__ part_u,
+                                                      int64_t nblk_u, double* scal, int par,
+                                                      double reliable) {
+  if (cg_done(scal)) return;
+  double b;
+  if (cg_beta(part_u, nblk_u, scal, par, reliable, b)) return;
+  const float beta = (float)b;
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK)
+    p[i] = zv[i] + beta * p[i];
+}
+
+// p = z + BETA p with the BETA of the iteration that stopped (before a residual replacement).
+__global__ void k_m_dir_force(int32_t n, const float* __restrict__ zv, float* __restrict__ p,
+                              const double* scal) {
   const float beta = (float)nnpp_ld_scalar(scal + SC_BETA);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = zv[i] + beta * p[i];
@@ -765,6 +830,11 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
     if (e) return e;
     return read_scalars(c);
   };
+  // CG iterations: SpMV (partials p.Ap into part_s), update (alpha from them, partials r.z and
+  // r.r into part), direction (beta and the stop test from those); rho alternates between two
+  // device slots by iteration parity, reset to the first slot by every (re)start
+  double* part_s = part + 2 * NNPP_RED_GRID;
+  int par = 0;
   if (bnorm2 == 0.0) {  // zero right-hand side: the solution is 0 on the free nodes
     NNPP_HIP(c, hipMemsetAsync(c->cg_x, 0, sizeof(double) * n, s));
   } else if (precision != NNPP_MIXED) {
@@ -780,14 +850,12 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         break;
       }
       const int batch = plan.next(rr_b, c->h_scal[SC_TARGET2], left);
-      for (int it = 0; it < batch; it++) {
-        launch_cg_spmv(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q, part,
-                       scal);
-        if ((st = nnpp_finalize(c, FIN_PQ, op.g64, 0.0))) return st;
+      for (int it = 0; it < batch; it++, par ^= 1) {
+        launch_cg_spmv(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q,
+                       part_s, scal);
         k_cg_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
-                                             c->cg_z, part, scal);
-        if ((st = nnpp_finalize(c, FIN_UPDATE, g, 0.0))) return st;
-        k_cg_dir<<<nb, NNPP_BLOCK, 0, s>>>(n, c->cg_z, c->cg_p, scal);
+                                             c->cg_z, part, part_s, op.g64, c->scal, par);
+        k_cg_dir<<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_z, c->cg_p, part, g, c->scal, par);
       }
       NNPP_HIP(c, hipGetLastError());
       if ((st = read_scalars(c))) return st;
@@ -812,6 +880,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                                                  c->f_z, c->f_p, part);
       NNPP_HIP(c, hipGetLastError());
       if ((st = nnpp_finalize(c, FIN_M_REPLACE_START, g, rtol2))) return st;
+      par = 0;
       BatchPlan plan;
       double rr_b = rr / bnorm2, stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
       int it_b = 0;
@@ -822,14 +891,13 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
           break;
         }
         const int batch = plan.next(rr_b, stop_b, left);
-        for (int it = 0; it < batch; it++) {
-          launch_m_spmv(var, op.lds32, op.g32, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, part,
-                        scal);
-          if ((st = nnpp_finalize(c, FIN_PQ, op.g32, 0.0))) return st;
+        for (int it = 0; it < batch; it++, par ^= 1) {
+          launch_m_spmv(var, op.lds32, op.g32, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q,
+                        part_s, scal);
           k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
-                                              c->f_z, part, scal);
-          if ((st = nnpp_finalize(c, FIN_M_UPDATE, g, RELIABLE_DROP2))) return st;
-          k_m_dir<false><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, scal);
+                                              c->f_z, part, part_s, op.g32, c->scal, par);
+          k_m_dir<<<g, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, part, g, c->scal, par,
+                                           RELIABLE_DROP2);
         }
         NNPP_HIP(c, hipGetLastError());
         if ((st = read_scalars(c))) return st;
@@ -849,11 +917,12 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         if (!std::isfinite(rr))
           return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
         if (rr <= target2) break;
-        k_m_dir<true><<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, scal);
+        k_m_dir_force<<<nb, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, scal);
         k_m_replace<false><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, c->pinv32, c->f_e, c->f_r,
                                                     c->f_z, c->f_p, part);
         NNPP_HIP(c, hipGetLastError());
         if ((st = nnpp_finalize(c, FIN_M_REPLACE, g, rtol2))) return st;
+        par = 0;
         rr_b = rr / bnorm2;  // the replaced residual starts the next phase
         stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
       }

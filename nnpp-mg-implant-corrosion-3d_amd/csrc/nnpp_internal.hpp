@@ -214,7 +214,9 @@ struct nnpp_ctx {
 // Device scalars used by the CG kernels and reductions.
 enum CgScalar {
   SC_RHO = 0, SC_ALPHA, SC_BETA, SC_PQ, SC_RR, SC_TARGET2, SC_DONE, SC_ITERS, SC_BNORM2,
-  SC_R0, SC_R1, SC_R2, SC_R3, SC_RRMAX, SC_COUNT
+  SC_R0, SC_R1, SC_R2, SC_R3, SC_RRMAX,
+  SC_RHO1,  // second slot of rho: CG iterations alternate between SC_RHO and SC_RHO1
+  SC_COUNT
 };
 
 int nnpp_set_error(nnpp_ctx* c, int code, const std::string& msg);
@@ -397,16 +399,28 @@ __device__ __forceinline__ void nnpp_final_sum(double (&v)[K], const double* __r
   nnpp_block_sum<K>(v, lds);
 }
 
+// As nnpp_final_sum, with the totals in every thread of the block. Every block of a kernel that
+// calls this on the same partials obtains the same totals (same order of additions).
+template <int K>
+__device__ __forceinline__ void nnpp_final_sum_all(double (&v)[K],
+                                                   const double* __restrict__ part, int64_t nblk) {
+  __shared__ double tot[K];
+  nnpp_final_sum<K>(v, part, nblk);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = tot[k];
+}
+
 // Finalize operations (what the one-block kernel does with the summed partials).
 enum NnppFin {
   FIN_BNORM2 = 0,     // K=1: BNORM2
   FIN_RESID,          // K=2: RR
   FIN_RESID_START,    // K=2: RHO, RR, TARGET2 = a0 * BNORM2, ITERS = 0, DONE
-  FIN_PQ,             // K=1: PQ, ALPHA = RHO / PQ            (skipped when DONE)
-  FIN_UPDATE,         // K=2: BETA, RHO, RR, ITERS++, DONE at TARGET2 (skipped when DONE)
   FIN_M_REPLACE,      // K=2: RHO, RR, RRMAX, TARGET2 = a0, DONE = 0
   FIN_M_REPLACE_START,// K=2: as above, ITERS = 0, DONE = (RR <= a0)
-  FIN_M_UPDATE,       // K=2: as FIN_UPDATE, RRMAX, DONE also at RR <= a0 * RRMAX
   FIN_FRONT,          // K=3: R0, R1, R2
   FIN_NONFINITE       // K=1: R3
 };
