@@ -760,19 +760,35 @@ static int read_scalars(nnpp_ctx* c) {
 static constexpr double RELIABLE_DROP2 = 1e-4;  // replace the residual every 2 decades
 
 // Iterations of the next launch batch: the host sees the residual only between batches, and
-// iterations launched after the stop exit at once but still cost their launches. From the
-// squared-residual reduction per iteration of the batches so far, predict the iterations to the
-// stop threshold (plus one); the first batch of a solve is CG_BATCH long.
+// iterations launched after the stop exit at once but still cost their launches (about 16 us
+// for the three of an iteration at 128^3, against 30-50 us for a host round trip). A phase
+// (solve start or residual replacement up to the next stop) is first sized by the same phase of
+// the previous solve, plus one: successive steps repeat the phase lengths closely. Beyond that,
+// from the squared-residual reduction per iteration of the batches so far, predict the
+// iterations to the stop threshold (plus one); without either, CG_BATCH.
 struct BatchPlan {
   double rate = 0.0;
+  const std::vector<int32_t>* prev = nullptr;  // phase lengths of the previous solve
+  std::vector<int32_t> lens;                   // phase lengths of this solve
+  int phase = 0, phase_start = 0;
   void observe(double rr0, double rr1, int its) {
     if (its > 0 && rr0 > 0.0 && rr1 > 0.0 && rr1 < rr0) rate = std::pow(rr1 / rr0, 1.0 / its);
   }
-  int next(double rr, double stop, int left) const {
-    int k = CG_BATCH;
-    if (rate > 0.0 && rate < 1.0 && stop > 0.0 && rr > stop)
-      k = (int)std::ceil(std::log(stop / rr) / std::log(rate)) + 1;
-    k = k < 2 ? 2 : (k > 2 * CG_BATCH ? 2 * CG_BATCH : k);
+  // a phase ended after `total` iterations of the solve
+  void end_phase(int total) {
+    lens.push_back(total - phase_start);
+    phase++;
+    phase_start = total;
+  }
+  int next(double rr, double stop, int left, int total) const {
+    int k = 0;
+    if (prev && phase < (int)prev->size()) k = (*prev)[phase] - (total - phase_start) + 1;
+    if (k < 2) {
+      k = CG_BATCH;
+      if (rate > 0.0 && rate < 1.0 && stop > 0.0 && rr > stop)
+        k = (int)std::ceil(std::log(stop / rr) / std::log(rate)) + 1;
+    }
+    k = k < 2 ? 2 : (k > 4 * CG_BATCH ? 4 * CG_BATCH : k);
     return k < left ? k : left;
   }
 };
@@ -841,6 +857,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
     if ((st = resid(true))) return st;
     bool done = c->h_scal[SC_DONE] != 0.0;
     BatchPlan plan;
+    if (!rhs) plan.prev = &c->cg_phase_len[0];
     double rr_b = c->h_scal[SC_RR];
     int it_b = 0;
     while (!done) {
@@ -849,7 +866,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         status = NNPP_ERR_NOCONV;
         break;
       }
-      const int batch = plan.next(rr_b, c->h_scal[SC_TARGET2], left);
+      const int batch = plan.next(rr_b, c->h_scal[SC_TARGET2], left, total);
       for (int it = 0; it < batch; it++, par ^= 1) {
         launch_cg_spmv(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q,
                        part_s, scal);
@@ -868,6 +885,10 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       it_b = total;
     }
     n64 += total;
+    if (!rhs && status == NNPP_OK) {
+      plan.end_phase(total);
+      c->cg_phase_len[0] = plan.lens;
+    }
     if ((st = resid(false))) return st;  // FP64 residual check of the final iterate
     rr = c->h_scal[SC_RR];
   } else {
@@ -882,6 +903,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       if ((st = nnpp_finalize(c, FIN_M_REPLACE_START, g, rtol2))) return st;
       par = 0;
       BatchPlan plan;
+      if (!rhs) plan.prev = &c->cg_phase_len[1];
       double rr_b = rr / bnorm2, stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
       int it_b = 0;
       while (true) {
@@ -890,7 +912,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
           status = NNPP_ERR_NOCONV;
           break;
         }
-        const int batch = plan.next(rr_b, stop_b, left);
+        const int batch = plan.next(rr_b, stop_b, left, total);
         for (int it = 0; it < batch; it++, par ^= 1) {
           launch_m_spmv(var, op.lds32, op.g32, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q,
                         part_s, scal);
@@ -909,6 +931,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         stop_b = std::max(c->h_scal[SC_TARGET2], RELIABLE_DROP2 * c->h_scal[SC_RRMAX]);
         it_b = total;
         if (c->h_scal[SC_DONE] == 0.0) continue;
+        plan.end_phase(total);
         // reliable update: fold the correction into x, recompute the residual in FP64
         k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
         NNPP_HIP(c, hipGetLastError());
@@ -927,6 +950,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
       }
       n32 += total;
+      if (!rhs && status == NNPP_OK) c->cg_phase_len[1] = plan.lens;
       if (status != NNPP_OK) {  // keep the progress made so far
         k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
         NNPP_HIP(c, hipGetLastError());

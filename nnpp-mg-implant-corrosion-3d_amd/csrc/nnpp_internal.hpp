@@ -184,6 +184,10 @@ struct nnpp_ctx {
 
   double rtol = 1e-10;
   int32_t maxit = 20000;
+  // iterations of each phase (between residual replacements) of the last potential solve of
+  // the step kind, per precision (0 FP64, 1 mixed): the next solve sizes its launch batches by
+  // them (successive steps repeat the pattern closely)
+  std::vector<int32_t> cg_phase_len[2];
   double inner_rtol = 1e-4;
 
   // implicit (reference Newton) steps, allocated on first use
