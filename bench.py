@@ -303,10 +303,11 @@ def summarize_rocprof(trace_db, fetch_db=None, write_db=None, copyf_db=None, cop
 
 
 def roofline_entry(ach, alg_bytes, fmt_bytes, t_ms, stencil, cb, wb):
-    traffic = measured_traffic("k_m_spmv")
+    kname = "k_m_spmv_st" if stencil else "k_m_spmv"
+    traffic = measured_traffic(kname)
     out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-           "kernel": "k_m_spmv (FP32 potential SpMV + p.q)",
+           "kernel": f"{kname} (FP32 potential SpMV + p.q)",
            "algorithmic_bytes_per_launch": alg_bytes,
            "format_bytes_per_launch": fmt_bytes,
            "format_GBps": fmt_bytes / (t_ms * 1e-3) / 1e9,

@@ -108,3 +108,65 @@ def test_small_lattices_tile_edges(ng):
         tol = 1e-10 if prec == nn.FP64 else 1e-6
         for f in range(4):
             assert rel_err(U[f], ref["U"][f]) < tol, (ng, prec, f)
+
+
+def exact_lattice(nx, ny, nz, S=3, seed=0):
+    """nx x ny x nz lattice with spacing 1/16: every bond vector is exact in binary, so each
+    offset j - i has one weight and the stencil forms (potential operator and flux) apply when
+    no two bond vectors share an offset (nx, ny >= 7 for delta = 3 spacings). A solid ball in
+    the middle (interface bonds in both passes of the flux), Dirichlet faces."""
+    h = 1.0 / 16
+    p = nn.synth.synthetic_params(16, S=S, seed=seed)
+    p.n_nodes = nx * ny * nz
+    zz, yy, xx = np.meshgrid((np.arange(nz) + 0.5) * h, (np.arange(ny) + 0.5) * h,
+                             (np.arange(nx) + 0.5) * h, indexing="ij")
+    x, y, z = xx.ravel(), yy.ravel(), zz.ravel()
+    n, F = x.size, S + 1
+    rng = np.random.default_rng(seed + 7)
+    ic = np.empty((F, n))
+    ic[:S] = rng.uniform(0.9, 1.1, size=(S, n))
+    ic[S] = 0.0
+    c = 0.5 * h * np.array([nx, ny, nz])
+    r = 0.3 * h * min(nx, ny, nz)
+    solid = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r * r
+    ic[0, ~solid] = 0.0
+    I = np.stack(np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij"),
+                 0).reshape(3, -1)
+    lim = np.array([nz, ny, nx])[:, None] - 1
+    face = ((I == 0) | (I == lim)).any(axis=0)
+    bct = np.where(face, 0, 1).astype(np.int32)
+    bcv = np.full((F, n), -1.0)
+    bcv[:S, face] = 1.0
+    bcv[S, face] = 0.0
+    return nn.synth.Case(p, x, y, z, None, bct, bcv, ic, name=f"exact{nx}x{ny}x{nz}")
+
+
+def _exact_run(shape, stencil):
+    case = exact_lattice(*shape, seed=sum(shape))
+    case.params.front_threshold = 1.0
+    assert (case.icvalues[0] > 0).any()
+    for prec in (nn.FP64, nn.MIXED):
+        ref, U, ph, st = run_both(case, 4, prec)
+        assert st["status"] == 0
+        assert st["flux_stencil"] == int(stencil) and st["pot_stencil"] == int(stencil), \
+            (shape, prec)
+        assert np.array_equal(ph, ref["phase"])
+        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        for f in range(case.params.S + 1):
+            assert rel_err(U[f], ref["U"][f]) < tol, (shape, prec, f)
+
+
+@pytest.mark.parametrize("shape", [(7, 7, 5), (7, 7, 7), (9, 9, 9), (16, 9, 7), (13, 13, 13)])
+def test_stencil_forms_small_lattices(shape):
+    """The stencil forms on lattices inside one 256-row tile (7x7x5) or ending inside one:
+    staging windows clipped at both ends, partial waves, interface nodes; against the oracle."""
+    say("case exact lattice", shape)
+    _exact_run(shape, True)
+
+
+def test_stencil_refused_when_offsets_alias():
+    """In a lattice 4 nodes wide, (3,0,0) and (-1,1,0) are both bonds with the offset j - i = 3
+    and different weights: the per-offset weight check rejects the stencil forms, and the
+    column forms run."""
+    say("case aliasing lattice")
+    _exact_run((4, 4, 4), False)
