@@ -236,6 +236,32 @@ __device__ __forceinline__ void explicit_update(const FluxArgs& a, int64_t i, in
   if (a.passmask) a.passmask[i] = (uint16_t)pass_bits;
 }
 
+// Jacobian rows of node i from its bond sums (acc) and weight sums (sw: same-phase and
+// solid-liquid passes): out_k = d0_k v_k - acc_k, identity rows where field k is Dirichlet; with
+// d0_out set, d0 = 1/dt - R'_k at U^n (un) and the block diagonal are written first.
+template <int S>
+__device__ __forceinline__ void jac_update(const FluxArgs& a, int64_t i, int pi, int intf,
+                                           const double (&v)[S], const double (&acc)[S],
+                                           const double (&sw)[2], const double* un) {
+  const int64_t n = a.n;
+  const unsigned bm = a.bcmask[i];
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    const bool dir = (bm >> k) & 1u;
+    double d0;
+    if (a.d0_out) {
+      d0 = (1. / a.dt) - jac_reaction(k, a.chemistry, a.rx, a.r_diss, pi, intf, un);
+      a.d0_out[(int64_t)k * n + i] = d0;
+      const double A0 = pi ? a.A_SS[k] : a.A_LL[k];
+      const double dg = d0 + __fma_rn(A0, sw[0], a.A_SL[k] * sw[1]);
+      a.diag_out[(int64_t)k * n + i] = dir ? 1.0 : dg;
+    } else {
+      d0 = a.d0[(int64_t)k * n + i];
+    }
+    a.out[(int64_t)k * n + i] = dir ? v[k] : __fma_rn(d0, v[k], -acc[k]);
+  }
+}
+
 template <int S, bool RESID, bool MIXED, bool WIDE, bool JAC = false>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
   static_assert(!JAC || (!RESID && !MIXED), "the Jacobian apply is FP64 and not a residual");
@@ -393,21 +419,8 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux(const FluxArgs a) {
   const unsigned bm = a.bcmask[i];
   (void)U1;
   if constexpr (JAC) {
-#pragma unroll
-    for (int k = 0; k < S; k++) {
-      const bool dir = (bm >> k) & 1u;
-      double d0;
-      if (a.d0_out) {
-        d0 = (1. / a.dt) - jac_reaction(k, a.chemistry, a.rx, a.r_diss, pi, intf, un);
-        a.d0_out[(int64_t)k * n + i] = d0;
-        const double A0 = pi ? a.A_SS[k] : a.A_LL[k];
-        const double dg = d0 + (A0 * sw[0] + a.A_SL[k] * sw[1]);
-        a.diag_out[(int64_t)k * n + i] = dir ? 1.0 : dg;
-      } else {
-        d0 = a.d0[(int64_t)k * n + i];
-      }
-      a.out[(int64_t)k * n + i] = dir ? ci[k] : d0 * ci[k] - acc[k];
-    }
+    (void)bm;
+    jac_update<S>(a, i, pi, intf, ci, acc, sw, un);
     return;
   }
 
@@ -600,6 +613,21 @@ __device__ __forceinline__ void st_sums(const FluxSt& st, const double2* rcp,
   }
 }
 
+// sw = sum of the weights of the bonds of mask m in ascending order (the column form's weight
+// sums of the Jacobian diagonal; absent bonds add +0).
+__device__ __forceinline__ double st_wsum(const FluxSt& st, const uint32_t (&m)[4]) {
+  const int K = st.K;
+  double sw = 0.0;
+  for (int t0 = 0; t0 < K; t0 += 32) {
+    const int wd = t0 >> 5;
+    const uint32_t bits = wd == 0 ? m[0] : (wd == 1 ? m[1] : (wd == 2 ? m[2] : m[3]));
+    const int cnt = K - t0 < 32 ? K - t0 : 32;
+    for (int k = 0; k < cnt; k++)
+      sw += nnpp_mask_weight<double>(nnpp_ld_table(st.tf, t0 + k), (int)(bits << (31 - k)) >> 31);
+  }
+  return sw;
+}
+
 // Per chunk of 16 entries: some lane has a bond of mask m there (anyc), every lane has every
 // bond of nonzero weight there (fullc). Wave-uniform.
 __device__ __forceinline__ void st_chunks(const FluxSt& st, const uint32_t (&m)[4],
@@ -620,7 +648,9 @@ __device__ __forceinline__ void st_chunks(const FluxSt& st, const uint32_t (&m)[
   }
 }
 
-template <int S>
+// JAC: the species blocks of the frozen Jacobian applied to v (a.U1, species-major) with the
+// lagged phi^n, as the column kernel's JAC mode (identical additions and epilogue).
+template <int S, bool JAC>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_flux_st(const FluxArgs a, const FluxSt st) {
   extern __shared__ double2 fst_lds[];
   double2* lcp = fst_lds;                                         // [E] (c_j, phi_j)
@@ -632,15 +662,16 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux_st(const FluxArgs a, const 
   const int l = st.perm[r0 + threadIdx.x];
   const int64_t i = r0 + l;
   const bool in = i < n;
-  const double* __restrict__ U = a.Un;  // explicit update: the iterate is U^n
+  // explicit update: the fields of U^n; JAC: the vector v
+  const double* __restrict__ U = JAC ? a.U1 : a.Un;
   StSlots sl;
   sl.init(st);
-  sl.stage<true>(st, U + (int64_t)S * n, n, w0, lcp);
+  sl.stage<true>(st, a.Un + (int64_t)S * n, n, w0, lcp);
   sl.stage_u8(st, a.phase, n, w0, lph);
   double ci[S];
 #pragma unroll
   for (int k = 0; k < S; k++) ci[k] = in ? U[(int64_t)k * n + i] : 0.0;
-  const double phii = in ? U[(int64_t)S * n + i] : 0.0;
+  const double phii = in ? a.Un[(int64_t)S * n + i] : 0.0;
   const int pi = in ? a.phase[i] : 0;
   const uint4 bq = in ? st.bmask[i] : make_uint4(0u, 0u, 0u, 0u);
   __syncthreads();
@@ -683,15 +714,27 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_flux_st(const FluxArgs a, const 
       if (mixed) acc[k] += pass_sum(a.A_SL[k], sd1, a.M_SL[k], sm1);
     }
   }
-  if (in) explicit_update<S>(a, i, pi, intf, ci, acc, phii);
+  if constexpr (JAC) {
+    double sw[2] = {0.0, 0.0};
+    double un[S];
+    if (a.d0_out) {  // Jacobian set-up: weight sums and U^n for the diagonal
+      sw[0] = st_wsum(st, m0);
+      sw[1] = mixed ? st_wsum(st, m1) : 0.0;
+#pragma unroll
+      for (int k = 0; k < S; k++) un[k] = in ? a.Un[(int64_t)k * n + i] : 0.0;
+    }
+    if (in) jac_update<S>(a, i, pi, intf, ci, acc, sw, un);
+  } else {
+    if (in) explicit_update<S>(a, i, pi, intf, ci, acc, phii);
+  }
 }
 
-template <int S>
+template <int S, bool JAC>
 hipError_t launch_st(const FluxArgs& a, const FluxSt& st, size_t lds, hipStream_t s) {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_flux_st<S>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_flux_st<S, JAC>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  k_flux_st<S><<<nnpp_nblocks(a.n), NNPP_BLOCK, lds, s>>>(a, st);
+  k_flux_st<S, JAC><<<nnpp_nblocks(a.n), NNPP_BLOCK, lds, s>>>(a, st);
   return hipGetLastError();
 }
 
@@ -789,6 +832,39 @@ static size_t flux_stencil_lds(nnpp_ctx* c) {
   return (bytes + 7) & ~(size_t)7;
 }
 
+// The stencil form of the explicit update (JAC = false) or of the Jacobian apply (JAC = true).
+template <bool JAC>
+static int launch_stencil(nnpp_ctx* c, const FluxArgs& a, size_t st_lds) {
+  const int S = c->p.n_species;
+  const nnpp_stencil& T = c->pst;
+  FluxSt st;
+  st.bmask = T.bmask;
+  st.perm = T.fperm;
+  st.lc = T.lc;
+  st.tf = T.tf64;
+  st.seg_start = T.seg_start;
+  st.seg_len = T.seg_len;
+  st.seg_base = T.seg_base;
+  st.K = T.K;
+  st.R = T.R;
+  st.E = T.lds_elems;
+  st.off_min = T.off_min;
+  for (int w = 0; w < 4; w++) st.nz[w] = T.fnz[w];
+  hipError_t e = hipSuccess;
+  switch (S) {
+#define NNPP_CASE(SS) \
+  case SS: e = launch_st<SS, JAC>(a, st, st_lds, c->stream); break;
+    NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
+    NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
+    NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)
+#undef NNPP_CASE
+    default:
+      return nnpp_set_error(c, NNPP_ERR_ARG, "n_species out of range");
+  }
+  if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
+  return NNPP_OK;
+}
+
 int nnpp_flux_stencil_active(nnpp_ctx* c) { return flux_stencil_lds(c) != 0; }
 
 int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
@@ -802,34 +878,7 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   // explicit update on a lattice: the stencil form, FP64 in both precisions (on a 128^3 lattice
   // 2.4 ms against 4.65 ms for the FP64 and 2.45 ms for the FP32 column form on MI355X)
   const size_t st_lds = resid ? 0 : flux_stencil_lds(c);
-  if (st_lds) {
-    const nnpp_stencil& T = c->pst;
-    FluxSt st;
-    st.bmask = T.bmask;
-    st.perm = T.fperm;
-    st.lc = T.lc;
-    st.tf = T.tf64;
-    st.seg_start = T.seg_start;
-    st.seg_len = T.seg_len;
-    st.seg_base = T.seg_base;
-    st.K = T.K;
-    st.R = T.R;
-    st.E = T.lds_elems;
-    st.off_min = T.off_min;
-    for (int w = 0; w < 4; w++) st.nz[w] = T.fnz[w];
-    switch (S) {
-#define NNPP_CASE(SS) \
-  case SS: e = launch_st<SS>(a, st, st_lds, c->stream); break;
-      NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
-      NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
-      NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)
-#undef NNPP_CASE
-      default:
-        return nnpp_set_error(c, NNPP_ERR_ARG, "n_species out of range");
-    }
-    if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
-    return NNPP_OK;
-  }
+  if (st_lds) return launch_stencil<false>(c, a, st_lds);
   switch (S) {
 #define NNPP_CASE(SS) \
   case SS: e = launch_s<SS>(a, resid, mixed, c->bonds.wide, nb, c->stream); break;
@@ -844,14 +893,17 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   return NNPP_OK;
 }
 
-int nnpp_launch_jac(nnpp_ctx* c, double BV, const double* Un, const double* d0, double* out,
-                    double* d0_out, double* diag_out) {
+int nnpp_launch_jac(nnpp_ctx* c, double BV, const double* Un, const double* vin,
+                    const double* d0, double* out, double* d0_out, double* diag_out) {
   const int S = c->p.n_species;
-  FluxArgs a = flux_args(c, BV, nullptr, Un, out, nullptr);
+  FluxArgs a = flux_args(c, BV, vin, Un, out, nullptr);
   a.d0 = d0;
   a.d0_out = d0_out;
   a.diag_out = diag_out;
   if (!c->n) return NNPP_OK;
+  // lattice: the stencil form reads v species-major
+  const size_t st_lds = vin ? flux_stencil_lds(c) : 0;
+  if (st_lds) return launch_stencil<true>(c, a, st_lds);
   const int64_t nb = nnpp_nblocks(c->n);
   hipError_t e = hipSuccess;
   switch (S) {

@@ -63,7 +63,8 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_bs_start(int32_t n, int S,
 __global__ void k_bs_dir(int32_t n, int S, uint32_t mask, Coef beta, Coef omega,
                          const double* __restrict__ r, double* __restrict__ p,
                          const double* __restrict__ v, const double* __restrict__ diag,
-                         const double* __restrict__ phin, double* __restrict__ rec, int R) {
+                         const double* __restrict__ phin, double* __restrict__ rec, int R,
+                         double* __restrict__ hat) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double* o = rec + i * R;
@@ -78,6 +79,7 @@ __global__ void k_bs_dir(int32_t n, int S, uint32_t mask, Coef beta, Coef omega,
         ph = pn / diag[q];
       }
       o[k] = ph;
+      hat[(int64_t)k * n + i] = ph;
     }
   }
   o[S] = phin[i];
@@ -92,6 +94,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_bs_s(int32_t n, int S, uint32_t 
                                                      const double* __restrict__ diag,
                                                      const double* __restrict__ phin,
                                                      double* __restrict__ rec, int R,
+                                                     double* __restrict__ hat,
                                                      double* __restrict__ part) {
   double acc[NV];
 #pragma unroll
@@ -113,6 +116,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_bs_s(int32_t n, int S, uint32_t 
           acc[k] += sv * sv;
         }
         o[k] = sh;
+        hat[(int64_t)k * n + i] = sh;
       }
     }
     o[S] = phin[i];
@@ -285,7 +289,7 @@ int imp_alloc(nnpp_ctx* c) {
   if (c->imp && c->imp_n == n && c->imp_S == S) return NNPP_OK;
   nnpp_implicit_free(c);
   const int64_t m = n > 0 ? n : 1;
-  NNPP_HIP(c, hipMalloc((void**)&c->imp, sizeof(double) * 9 * S * m));
+  NNPP_HIP(c, hipMalloc((void**)&c->imp, sizeof(double) * 10 * S * m));
   NNPP_HIP(c, hipMalloc((void**)&c->imp_prev, sizeof(double) * (S + 1) * m));
   NNPP_HIP(c, hipMalloc((void**)&c->imp_dphi, sizeof(double) * 3 * m));
   NNPP_HIP(c, hipMalloc((void**)&c->imp_part, sizeof(double) * NNPP_RED_GRID * NV));
@@ -309,14 +313,16 @@ int imp_reduce(nnpp_ctx* c, int64_t g, double* out) {
 }
 
 // Species vectors inside c->imp
+// hat: the vector the Jacobian is applied to (p or s over the diagonal), species-major; the
+// records (c->aos) hold the same values for the column form.
 struct ImpVecs {
-  double *x, *r, *rh, *p, *v, *s, *t, *d0, *diag;
+  double *x, *r, *rh, *p, *v, *s, *t, *d0, *diag, *hat;
 };
 ImpVecs imp_vecs(nnpp_ctx* c) {
   const int64_t sn = (int64_t)c->p.n_species * c->n;
   double* b = c->imp;
   return ImpVecs{b, b + sn, b + 2 * sn, b + 3 * sn, b + 4 * sn, b + 5 * sn, b + 6 * sn,
-                 b + 7 * sn, b + 8 * sn};
+                 b + 7 * sn, b + 8 * sn, b + 9 * sn};
 }
 
 // K_k x_k = -F_k (species fields of c->Fres) for every species together by Jacobi-
@@ -358,9 +364,9 @@ int imp_bicgstab(nnpp_ctx* c, double BV, double rtol, int32_t maxit, int32_t* it
       rho_prev[k] = rho[k];
     }
     k_bs_dir<<<nb, NNPP_BLOCK, 0, st>>>(n, S, mask, beta, omega, V.r, V.p, V.v, V.diag, phin,
-                                        c->aos, R);
+                                        c->aos, R, V.hat);
     NNPP_HIP(c, hipGetLastError());
-    if ((e = nnpp_launch_jac(c, BV, Un, V.d0, V.v))) return e;
+    if ((e = nnpp_launch_jac(c, BV, Un, V.hat, V.d0, V.v))) return e;
     k_bs_dots<<<g, NNPP_BLOCK, 0, st>>>(n, S, V.rh, V.v, nullptr, nullptr, c->imp_part);
     NNPP_HIP(c, hipGetLastError());
     if ((e = imp_reduce(c, g, red))) return e;
@@ -374,13 +380,13 @@ int imp_bicgstab(nnpp_ctx* c, double BV, double rtol, int32_t maxit, int32_t* it
       alpha.v[k] = al[k];
     }
     k_bs_s<<<g, NNPP_BLOCK, 0, st>>>(n, S, mask, alpha, V.r, V.v, V.s, V.diag, phin, c->aos, R,
-                                     c->imp_part);
+                                     V.hat, c->imp_part);
     NNPP_HIP(c, hipGetLastError());
     if ((e = imp_reduce(c, g, red))) return e;
     uint32_t fin = 0;  // ||s|| already small: x += alpha p^, r = s (omega = 0)
     for (int k = 0; k < S; k++)
       if (((mask >> k) & 1u) && red[k] <= rtol * rtol * bn2[k]) fin |= 1u << k;
-    if ((e = nnpp_launch_jac(c, BV, Un, V.d0, V.t))) return e;
+    if ((e = nnpp_launch_jac(c, BV, Un, V.hat, V.d0, V.t))) return e;
     k_bs_dots<<<g, NNPP_BLOCK, 0, st>>>(n, S, V.t, V.s, V.t, V.t, c->imp_part);
     NNPP_HIP(c, hipGetLastError());
     if ((e = imp_reduce(c, g, red))) return e;
@@ -422,9 +428,9 @@ int imp_setup(nnpp_ctx* c, double BV) {
   const double* Un = c->U[NNPP_SLOT_N];
   k_bs_dir<<<nnpp_nblocks(n), NNPP_BLOCK, 0, c->stream>>>(n, S, 0u, Coef{}, Coef{}, V.r, V.p,
                                                           V.v, V.diag, Un + (int64_t)S * n,
-                                                          c->aos, jac_rec_len(S));
+                                                          c->aos, jac_rec_len(S), V.hat);
   NNPP_HIP(c, hipGetLastError());
-  return nnpp_launch_jac(c, BV, Un, nullptr, V.t, V.d0, V.diag);
+  return nnpp_launch_jac(c, BV, Un, V.hat, nullptr, V.t, V.d0, V.diag);
 }
 
 }  // namespace
@@ -558,7 +564,7 @@ int nnpp_jacobian_apply_dev(nnpp_ctx* c, double voln1rel, const double* in, doub
   k_jac_stage<<<nnpp_nblocks(n), NNPP_BLOCK, 0, c->stream>>>(n, S, in, Un + (int64_t)S * n,
                                                              c->aos, jac_rec_len(S));
   NNPP_HIP(c, hipGetLastError());
-  if ((e = nnpp_launch_jac(c, BV, Un, nullptr, out, V.d0, V.diag))) return e;
+  if ((e = nnpp_launch_jac(c, BV, Un, in, nullptr, out, V.d0, V.diag))) return e;
   if (diag)
     NNPP_HIP(c, hipMemcpyAsync(diag, V.diag, sizeof(double) * S * n, hipMemcpyDeviceToDevice,
                                c->stream));

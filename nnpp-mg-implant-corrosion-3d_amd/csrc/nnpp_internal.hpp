@@ -453,11 +453,14 @@ int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out);
 // extrapolation 3 phi - 3 prev + prev2, hist 1 -> linear 2 phi - prev, hist 0 -> phi.
 int nnpp_phi_guess_dev(nnpp_ctx* c, int hist, const double* phi, const double* prev,
                        const double* prev2, double* out);
-// Species blocks of the frozen Jacobian (utils.h:844-1849) applied to the vectors staged in the
-// gather records c->aos ([v_0 .. v_{S-1}, phi^n] per node): out = d0 v + bond sums. With d0_out
-// set, d0 = 1/dt - R' at Un and the block diagonals are written first (d0 is then not read).
-int nnpp_launch_jac(nnpp_ctx* c, double BV, const double* Un, const double* d0, double* out,
-                    double* d0_out = nullptr, double* diag_out = nullptr);
+// Species blocks of the frozen Jacobian (utils.h:844-1849) applied to v: out = d0 v + bond
+// sums. v is given species-major (vin, S*n; the lattice stencil form reads it) and staged in the
+// gather records c->aos ([v_0 .. v_{S-1}, phi^n] per node; the column form reads those). With
+// d0_out set, d0 = 1/dt - R' at Un and the block diagonals are written first (d0 is then not
+// read).
+int nnpp_launch_jac(nnpp_ctx* c, double BV, const double* Un, const double* vin,
+                    const double* d0, double* out, double* d0_out = nullptr,
+                    double* diag_out = nullptr);
 // phi block K11 (utils.h:1794-1848) with its identity rows: out = K11 in, diag = its diagonal.
 int nnpp_jac_phi_dev(nnpp_ctx* c, const double* in, double* out, double* diag);
 // Implicit (reference Newton) time step; the potential operator must be current.
