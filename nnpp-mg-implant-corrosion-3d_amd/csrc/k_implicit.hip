@@ -25,68 +25,93 @@ struct Coef {
 // stride of the Jacobian's gather records: [v_0 .. v_{S-1}, phi^n], padded to even
 static inline int jac_rec_len(int S) { return (S + 2) & ~1; }
 
+// The species kernels run on a (gx, S) grid: blockIdx.y is the species, so a thread keeps two
+// accumulators instead of 2 S (the all-species form held one wave per SIMD). gx is a multiple
+// of the XCD count, so the XCD of a block is blockIdx.x % 8 as in nnpp_blocks_of.
+static inline int64_t bs_grid(int64_t nb, int S) {
+  int64_t g = (NNPP_RED_GRID / S) & ~(int64_t)(NNPP_NXCD - 1);
+  if (g < NNPP_NXCD) g = NNPP_NXCD;
+  return nb < g ? nb : g;
+}
+
+// Block partials of species blockIdx.y: a in column k, b in column NNPP_MAX_SPECIES + k of row
+// blockIdx.x (k_bs_final sums each column over the rows in row order).
+__device__ __forceinline__ void bs_partial(double a, double b, double* __restrict__ part) {
+  __shared__ double lds[2 * (NNPP_BLOCK / NNPP_WAVE)];
+  double v[2] = {a, b};
+  nnpp_block_sum<2>(v, lds);
+  if (threadIdx.x == 0) {
+    part[(int64_t)blockIdx.x * NV + blockIdx.y] = v[0];
+    part[(int64_t)blockIdx.x * NV + NNPP_MAX_SPECIES + blockIdx.y] = v[1];
+  }
+}
+
+// Gather record of node i for the column form: species k's value, and (species 0's block) phi^n
+// and the padding.
+__device__ __forceinline__ void bs_record(double* __restrict__ rec, int R, int S, int64_t i,
+                                          int k, double val, const double* __restrict__ phin) {
+  double* o = rec + i * R;
+  o[k] = val;
+  if (k == 0) {
+    o[S] = phin[i];
+    for (int q = S + 1; q < R; q++) o[q] = 0.0;
+  }
+}
+
 // b = -F (species fields of the residual), r = rh = b, x = p = v = 0; per species b.b
-__global__ __launch_bounds__(NNPP_BLOCK) void k_bs_start(int32_t n, int S,
-                                                         const double* __restrict__ F,
+__global__ __launch_bounds__(NNPP_BLOCK) void k_bs_start(int32_t n, const double* __restrict__ F,
                                                          double* __restrict__ x,
                                                          double* __restrict__ r,
                                                          double* __restrict__ rh,
                                                          double* __restrict__ p,
                                                          double* __restrict__ v,
                                                          double* __restrict__ part) {
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  const int64_t o = (int64_t)blockIdx.y * n;
+  double acc = 0.0;
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-#pragma unroll
-    for (int k = 0; k < NNPP_MAX_SPECIES; k++) {
-      if (k < S) {
-        const int64_t q = (int64_t)k * n + i;
-        const double b = -F[q];
-        r[q] = b;
-        rh[q] = b;
-        x[q] = 0.0;
-        p[q] = 0.0;
-        v[q] = 0.0;
-        acc[k] += b * b;
-      }
-    }
+    const int64_t q = o + i;
+    const double b = -F[q];
+    r[q] = b;
+    rh[q] = b;
+    x[q] = 0.0;
+    p[q] = 0.0;
+    v[q] = 0.0;
+    acc += b * b;
   }
-  nnpp_block_partial<NV>(acc, part);
+  bs_partial(acc, 0.0, part);
 }
 
-// p = r + beta (p - omega v) for the active species; the records get p / diag (0 for inactive
-// species) and phi^n. With mask 0 this only stages zeros and phi^n.
-__global__ void k_bs_dir(int32_t n, int S, uint32_t mask, Coef beta, Coef omega,
-                         const double* __restrict__ r, double* __restrict__ p,
-                         const double* __restrict__ v, const double* __restrict__ diag,
-                         const double* __restrict__ phin, double* __restrict__ rec, int R,
-                         double* __restrict__ hat) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// p = r + beta (p - omega v) for the active species; hat (and the records, if given) get
+// p / diag, 0 for inactive species. With mask 0 this only stages zeros (and phi^n).
+// Grid (node blocks, S).
+__global__ __launch_bounds__(NNPP_BLOCK) void k_bs_dir(int32_t n, int S, uint32_t mask,
+                                                       Coef beta, Coef omega,
+                                                       const double* __restrict__ r,
+                                                       double* __restrict__ p,
+                                                       const double* __restrict__ v,
+                                                       const double* __restrict__ diag,
+                                                       const double* __restrict__ phin,
+                                                       double* __restrict__ rec, int R,
+                                                       double* __restrict__ hat) {
+  const int k = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x;
   if (i >= n) return;
-  double* o = rec + i * R;
-#pragma unroll
-  for (int k = 0; k < NNPP_MAX_SPECIES; k++) {
-    if (k < S) {
-      double ph = 0.0;
-      if ((mask >> k) & 1u) {
-        const int64_t q = (int64_t)k * n + i;
-        const double pn = r[q] + beta.v[k] * (p[q] - omega.v[k] * v[q]);
-        p[q] = pn;
-        ph = pn / diag[q];
-      }
-      o[k] = ph;
-      hat[(int64_t)k * n + i] = ph;
-    }
+  const int64_t q = (int64_t)k * n + i;
+  double ph = 0.0;
+  if ((mask >> k) & 1u) {
+    const double pn = r[q] + beta.v[k] * (p[q] - omega.v[k] * v[q]);
+    p[q] = pn;
+    ph = pn / diag[q];
   }
-  o[S] = phin[i];
-  for (int k = S + 1; k < R; k++) o[k] = 0.0;
+  hat[q] = ph;
+  if (rec) bs_record(rec, R, S, i, k, ph, phin);
 }
 
-// s = r - alpha v for the active species; the records get s / diag; per species s.s
+// s = r - alpha v for the active species; hat (and the records, if given) get s / diag;
+// per species s.s
 __global__ __launch_bounds__(NNPP_BLOCK) void k_bs_s(int32_t n, int S, uint32_t mask, Coef alpha,
                                                      const double* __restrict__ r,
                                                      const double* __restrict__ v,
@@ -96,92 +121,77 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_bs_s(int32_t n, int S, uint32_t 
                                                      double* __restrict__ rec, int R,
                                                      double* __restrict__ hat,
                                                      double* __restrict__ part) {
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  const int k = blockIdx.y;
+  const bool on = (mask >> k) & 1u;
+  const double al = alpha.v[k];
+  const int64_t o = (int64_t)k * n;
+  double acc = 0.0;
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-    double* o = rec + i * R;
-#pragma unroll
-    for (int k = 0; k < NNPP_MAX_SPECIES; k++) {
-      if (k < S) {
-        double sh = 0.0;
-        if ((mask >> k) & 1u) {
-          const int64_t q = (int64_t)k * n + i;
-          const double sv = r[q] - alpha.v[k] * v[q];
-          s[q] = sv;
-          sh = sv / diag[q];
-          acc[k] += sv * sv;
-        }
-        o[k] = sh;
-        hat[(int64_t)k * n + i] = sh;
-      }
+    const int64_t q = o + i;
+    double sh = 0.0;
+    if (on) {
+      const double sv = r[q] - al * v[q];
+      s[q] = sv;
+      sh = sv / diag[q];
+      acc += sv * sv;
     }
-    o[S] = phin[i];
-    for (int k = S + 1; k < R; k++) o[k] = 0.0;
+    hat[q] = sh;
+    if (rec) bs_record(rec, R, S, i, k, sh, phin);
   }
-  nnpp_block_partial<NV>(acc, part);
+  bs_partial(acc, 0.0, part);
 }
 
 // per species a0.b0 (first half) and a1.b1 (second half, if a1)
-__global__ __launch_bounds__(NNPP_BLOCK) void k_bs_dots(int32_t n, int S,
+__global__ __launch_bounds__(NNPP_BLOCK) void k_bs_dots(int32_t n,
                                                         const double* __restrict__ a0,
                                                         const double* __restrict__ b0,
                                                         const double* __restrict__ a1,
                                                         const double* __restrict__ b1,
                                                         double* __restrict__ part) {
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  const int64_t o = (int64_t)blockIdx.y * n;
+  double acc0 = 0.0, acc1 = 0.0;
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-#pragma unroll
-    for (int k = 0; k < NNPP_MAX_SPECIES; k++) {
-      if (k < S) {
-        const int64_t q = (int64_t)k * n + i;
-        acc[k] += a0[q] * b0[q];
-        if (a1) acc[NNPP_MAX_SPECIES + k] += a1[q] * b1[q];
-      }
-    }
+    const int64_t q = o + i;
+    acc0 += a0[q] * b0[q];
+    if (a1) acc1 += a1[q] * b1[q];
   }
-  nnpp_block_partial<NV>(acc, part);
+  bs_partial(acc0, acc1, part);
 }
 
 // x += alpha p/diag + omega s/diag, r = s - omega t for the active species; per species rh.r
 // (first half) and r.r (second half) of the (possibly unchanged) residual
 __global__ __launch_bounds__(NNPP_BLOCK) void k_bs_update(
-    int32_t n, int S, uint32_t mask, Coef alpha, Coef omega, double* __restrict__ x,
+    int32_t n, uint32_t mask, Coef alpha, Coef omega, double* __restrict__ x,
     double* __restrict__ r, const double* __restrict__ p, const double* __restrict__ s,
     const double* __restrict__ t, const double* __restrict__ diag,
     const double* __restrict__ rh, double* __restrict__ part) {
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  const int k = blockIdx.y;
+  const bool on = (mask >> k) & 1u;
+  const double al = alpha.v[k], om = omega.v[k];
+  const int64_t o = (int64_t)k * n;
+  double acc0 = 0.0, acc1 = 0.0;
   const int64_t nb = nnpp_nblocks(n);
   for (nnpp_block_iter it = nnpp_blocks_of(nb); it.cur < it.end; it.cur += it.stride) {
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     if (i >= n) continue;
-#pragma unroll
-    for (int k = 0; k < NNPP_MAX_SPECIES; k++) {
-      if (k < S) {
-        const int64_t q = (int64_t)k * n + i;
-        double rv = r[q];
-        if ((mask >> k) & 1u) {
-          const double dg = diag[q];
-          x[q] += alpha.v[k] * (p[q] / dg) + omega.v[k] * (s[q] / dg);
-          rv = s[q] - omega.v[k] * t[q];
-          r[q] = rv;
-        }
-        acc[k] += rh[q] * rv;
-        acc[NNPP_MAX_SPECIES + k] += rv * rv;
-      }
+    const int64_t q = o + i;
+    double rv = r[q];
+    if (on) {
+      const double dg = diag[q];
+      x[q] += al * (p[q] / dg) + om * (s[q] / dg);
+      rv = s[q] - om * t[q];
+      r[q] = rv;
     }
+    acc0 += rh[q] * rv;
+    acc1 += rv * rv;
   }
-  nnpp_block_partial<NV>(acc, part);
+  bs_partial(acc0, acc1, part);
 }
 
 // Sum of the NV-wide partials of nblk blocks in a fixed order: thread t adds column t % NV of
@@ -248,11 +258,16 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_newton_update(
 }
 
 // Potential block: the Dirichlet part of the update (identity rows: dU = -F) as a vector
+// vector; *any is set to 1 if an entry is nonzero (it must be 0 before the launch; lanes that
+// set it all store the same value)
 __global__ void k_newton_phi_dir(int32_t n, int S, const uint16_t* __restrict__ bcmask,
-                                 const double* __restrict__ Fphi, double* __restrict__ dd) {
+                                 const double* __restrict__ Fphi, double* __restrict__ dd,
+                                 int32_t* __restrict__ any) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  dd[i] = ((bcmask[i] >> S) & 1u) ? -Fphi[i] : 0.0;
+  const double d = ((bcmask[i] >> S) & 1u) ? -Fphi[i] : 0.0;
+  dd[i] = d;
+  if (d != 0.0) *any = 1;
 }
 
 // Right-hand side of the free rows, psc (-F - K11_fD dU_D) (coup = the operator applied to the
@@ -294,6 +309,7 @@ int imp_alloc(nnpp_ctx* c) {
   NNPP_HIP(c, hipMalloc((void**)&c->imp_dphi, sizeof(double) * 3 * m));
   NNPP_HIP(c, hipMalloc((void**)&c->imp_part, sizeof(double) * NNPP_RED_GRID * NV));
   NNPP_HIP(c, hipMalloc((void**)&c->imp_red, sizeof(double) * NV));
+  NNPP_HIP(c, hipMalloc((void**)&c->imp_flag, sizeof(int32_t)));
   NNPP_HIP(c, hipHostMalloc((void**)&c->h_imp, sizeof(double) * NV, hipHostMallocDefault));
   c->imp_n = n;
   c->imp_S = S;
@@ -335,14 +351,17 @@ int imp_bicgstab(nnpp_ctx* c, double BV, double rtol, int32_t maxit, int32_t* it
   hipStream_t st = c->stream;
   const int R = jac_rec_len(S);
   const int64_t nb = nnpp_nblocks(n);
-  const int64_t g = nnpp_red_grid(nb);
+  const int64_t gx = bs_grid(nb, S);
+  const dim3 g((unsigned)gx, (unsigned)S), gn((unsigned)nb, (unsigned)S);
   const ImpVecs V = imp_vecs(c);
   const double* Un = c->U[NNPP_SLOT_N];
   const double* phin = Un + (int64_t)S * n;
+  // the column form of the Jacobian reads the gather records, the stencil form reads hat
+  double* rec = nnpp_flux_stencil_active(c) ? nullptr : c->aos;
   double red[NV];
-  k_bs_start<<<g, NNPP_BLOCK, 0, st>>>(n, S, c->Fres, V.x, V.r, V.rh, V.p, V.v, c->imp_part);
+  k_bs_start<<<g, NNPP_BLOCK, 0, st>>>(n, c->Fres, V.x, V.r, V.rh, V.p, V.v, c->imp_part);
   NNPP_HIP(c, hipGetLastError());
-  int e = imp_reduce(c, g, red);
+  int e = imp_reduce(c, gx, red);
   if (e) return e;
   double bn2[NNPP_MAX_SPECIES], rr[NNPP_MAX_SPECIES], rho[NNPP_MAX_SPECIES];
   double rho_prev[NNPP_MAX_SPECIES], al[NNPP_MAX_SPECIES], om[NNPP_MAX_SPECIES];
@@ -363,13 +382,13 @@ int imp_bicgstab(nnpp_ctx* c, double BV, double rtol, int32_t maxit, int32_t* it
       omega.v[k] = om[k];
       rho_prev[k] = rho[k];
     }
-    k_bs_dir<<<nb, NNPP_BLOCK, 0, st>>>(n, S, mask, beta, omega, V.r, V.p, V.v, V.diag, phin,
-                                        c->aos, R, V.hat);
+    k_bs_dir<<<gn, NNPP_BLOCK, 0, st>>>(n, S, mask, beta, omega, V.r, V.p, V.v, V.diag, phin,
+                                        rec, R, V.hat);
     NNPP_HIP(c, hipGetLastError());
     if ((e = nnpp_launch_jac(c, BV, Un, V.hat, V.d0, V.v))) return e;
-    k_bs_dots<<<g, NNPP_BLOCK, 0, st>>>(n, S, V.rh, V.v, nullptr, nullptr, c->imp_part);
+    k_bs_dots<<<g, NNPP_BLOCK, 0, st>>>(n, V.rh, V.v, nullptr, nullptr, c->imp_part);
     NNPP_HIP(c, hipGetLastError());
-    if ((e = imp_reduce(c, g, red))) return e;
+    if ((e = imp_reduce(c, gx, red))) return e;
     for (int k = 0; k < S; k++) {
       if (!((mask >> k) & 1u)) continue;
       if (red[k] == 0.0 || !std::isfinite(red[k])) {  // breakdown: keep the current x_k
@@ -379,27 +398,27 @@ int imp_bicgstab(nnpp_ctx* c, double BV, double rtol, int32_t maxit, int32_t* it
       al[k] = rho[k] / red[k];
       alpha.v[k] = al[k];
     }
-    k_bs_s<<<g, NNPP_BLOCK, 0, st>>>(n, S, mask, alpha, V.r, V.v, V.s, V.diag, phin, c->aos, R,
+    k_bs_s<<<g, NNPP_BLOCK, 0, st>>>(n, S, mask, alpha, V.r, V.v, V.s, V.diag, phin, rec, R,
                                      V.hat, c->imp_part);
     NNPP_HIP(c, hipGetLastError());
-    if ((e = imp_reduce(c, g, red))) return e;
+    if ((e = imp_reduce(c, gx, red))) return e;
     uint32_t fin = 0;  // ||s|| already small: x += alpha p^, r = s (omega = 0)
     for (int k = 0; k < S; k++)
       if (((mask >> k) & 1u) && red[k] <= rtol * rtol * bn2[k]) fin |= 1u << k;
     if ((e = nnpp_launch_jac(c, BV, Un, V.hat, V.d0, V.t))) return e;
-    k_bs_dots<<<g, NNPP_BLOCK, 0, st>>>(n, S, V.t, V.s, V.t, V.t, c->imp_part);
+    k_bs_dots<<<g, NNPP_BLOCK, 0, st>>>(n, V.t, V.s, V.t, V.t, c->imp_part);
     NNPP_HIP(c, hipGetLastError());
-    if ((e = imp_reduce(c, g, red))) return e;
+    if ((e = imp_reduce(c, gx, red))) return e;
     for (int k = 0; k < S; k++) {
       if (!((mask >> k) & 1u)) continue;
       const double tt = red[NNPP_MAX_SPECIES + k];
       om[k] = ((fin >> k) & 1u) || !(tt > 0.0) ? 0.0 : red[k] / tt;
       omega.v[k] = om[k];
     }
-    k_bs_update<<<g, NNPP_BLOCK, 0, st>>>(n, S, mask, alpha, omega, V.x, V.r, V.p, V.s, V.t,
+    k_bs_update<<<g, NNPP_BLOCK, 0, st>>>(n, mask, alpha, omega, V.x, V.r, V.p, V.s, V.t,
                                           V.diag, V.rh, c->imp_part);
     NNPP_HIP(c, hipGetLastError());
-    if ((e = imp_reduce(c, g, red))) return e;
+    if ((e = imp_reduce(c, gx, red))) return e;
     it++;
     for (int k = 0; k < S; k++) {
       if (!((mask >> k) & 1u)) continue;
@@ -419,16 +438,17 @@ int imp_bicgstab(nnpp_ctx* c, double BV, double rtol, int32_t maxit, int32_t* it
   return mask ? NNPP_ERR_NOCONV : NNPP_OK;
 }
 
-// Frozen Jacobian at U^n: d0 = 1/dt - R' and the species block diagonals (records: zeros and
-// phi^n; the output goes to the t vector, unused).
+// Frozen Jacobian at U^n: d0 = 1/dt - R' and the species block diagonals (hat and the column
+// form's records: zeros and phi^n; the output goes to the t vector, unused).
 int imp_setup(nnpp_ctx* c, double BV) {
   const int S = c->p.n_species;
   const int32_t n = c->n;
   const ImpVecs V = imp_vecs(c);
   const double* Un = c->U[NNPP_SLOT_N];
-  k_bs_dir<<<nnpp_nblocks(n), NNPP_BLOCK, 0, c->stream>>>(n, S, 0u, Coef{}, Coef{}, V.r, V.p,
-                                                          V.v, V.diag, Un + (int64_t)S * n,
-                                                          c->aos, jac_rec_len(S), V.hat);
+  double* rec = nnpp_flux_stencil_active(c) ? nullptr : c->aos;
+  const dim3 gn((unsigned)nnpp_nblocks(n), (unsigned)S);
+  k_bs_dir<<<gn, NNPP_BLOCK, 0, c->stream>>>(n, S, 0u, Coef{}, Coef{}, V.r, V.p, V.v, V.diag,
+                                             Un + (int64_t)S * n, rec, jac_rec_len(S), V.hat);
   NNPP_HIP(c, hipGetLastError());
   return nnpp_launch_jac(c, BV, Un, V.hat, nullptr, V.t, V.d0, V.diag);
 }
@@ -436,9 +456,10 @@ int imp_setup(nnpp_ctx* c, double BV) {
 }  // namespace
 
 void nnpp_implicit_free(nnpp_ctx* c) {
-  void* ps[] = {c->imp, c->imp_prev, c->imp_dphi, c->imp_part, c->imp_red};
+  void* ps[] = {c->imp, c->imp_prev, c->imp_dphi, c->imp_part, c->imp_red, c->imp_flag};
   for (void* q : ps)
     if (q) hipFree(q);
+  c->imp_flag = nullptr;
   if (c->h_imp) hipHostFree(c->h_imp);
   c->imp = c->imp_prev = c->imp_dphi = c->imp_part = c->imp_red = c->h_imp = nullptr;
   c->imp_n = -1;
@@ -495,9 +516,12 @@ int nnpp_implicit_once(nnpp_ctx* c, int precision, const nnpp_newton_opts& o,
     st.lin_relres_max = std::fmax(st.lin_relres_max, lrel);
     // potential block: K11 dU = -F with the Dirichlet rows' update moved to the right-hand side
     const double* Fphi = c->Fres + (int64_t)S * n;
-    k_newton_phi_dir<<<nb, NNPP_BLOCK, 0, s>>>(n, S, c->bcmask, Fphi, dtmp);
+    // the coupling is 0 (exactly: each bond adds w (0 - 0)) unless a Dirichlet row still
+    // moves; the iterate carries the Dirichlet values after the first Newton update
+    NNPP_HIP(c, hipMemsetAsync(c->imp_flag, 0, sizeof(int32_t), s));
+    k_newton_phi_dir<<<nb, NNPP_BLOCK, 0, s>>>(n, S, c->bcmask, Fphi, dtmp, c->imp_flag);
     NNPP_HIP(c, hipGetLastError());
-    if ((e = nnpp_potential_apply_dev(c, dtmp, coup))) return e;
+    if ((e = nnpp_potential_apply_dev(c, dtmp, coup, c->imp_flag))) return e;
     k_newton_phi_rhs<<<nb, NNPP_BLOCK, 0, s>>>(n, c->pfree, c->psc, Fphi, coup, dtmp, dphi);
     NNPP_HIP(c, hipGetLastError());
     const double rtol_keep = c->rtol;

@@ -325,16 +325,21 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_finalize(int op, const double* _
 }
 
 // Poisson operator action for nnpp_poisson_apply: K11 rows of utils.h:1794-1848 without the
-// identity rows, weights computed from the coordinates.
+// identity rows, weights computed from the coordinates. A flag `any` that reads 0 says `in` is
+// all zeros: every sum is then +0 (each bond adds w (0 - 0)) and is stored without the bonds.
 __global__ void k_pot_apply_csr(int32_t n, const int64_t* __restrict__ row_ptr,
                                 const int32_t* __restrict__ col,
                                 const uint8_t* __restrict__ phase, const double* __restrict__ x,
                                 const double* __restrict__ y, const double* __restrict__ z,
                                 WeightArgs wa, const double* __restrict__ in,
-                                double* __restrict__ out) {
+                                double* __restrict__ out, const int32_t* __restrict__ any) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
+  if (any && *any == 0) {
+    out[i] = acc;
+    return;
+  }
   if (phase[i] == 0)
     for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; b++) {
       const int32_t j = col[b];
@@ -1050,11 +1055,11 @@ int nnpp_jac_phi_dev(nnpp_ctx* c, const double* in, double* out, double* diag) {
   return NNPP_OK;
 }
 
-int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out) {
+int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out, const int32_t* any) {
   if (!c->n) return NNPP_OK;
   k_pot_apply_csr<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(
       c->n, c->row_ptr, c->col, c->phase, c->x, c->y, c->z,
-      weight_args(c, 3.0 + 2.0 * c->p.s_phi), in, out);
+      weight_args(c, 3.0 + 2.0 * c->p.s_phi), in, out, any);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }

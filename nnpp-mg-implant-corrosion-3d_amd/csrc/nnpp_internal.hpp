@@ -197,6 +197,7 @@ struct nnpp_ctx {
   double* imp_part = nullptr;  // [NNPP_RED_GRID * 32] block partials
   double* imp_red = nullptr;   // [32] totals
   double* h_imp = nullptr;     // pinned host copy of the totals
+  int32_t* imp_flag = nullptr;  // set when the Dirichlet part of the potential update is nonzero
   int64_t imp_n = -1;          // n and S the buffers were allocated for
   int imp_S = 0;
   bool imp_first = true;       // next implicit step starts from U^n with the Dirichlet values
@@ -448,7 +449,9 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
 int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, double* phi,
                              int32_t* iters, double* relres, int64_t* spmv64, int64_t* spmv32,
                              const double* rhs = nullptr);
-int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out);
+// any (device flag, optional): when it reads 0, `in` is all zeros and out is set to 0.
+int nnpp_potential_apply_dev(nnpp_ctx* c, const double* in, double* out,
+                             const int32_t* any = nullptr);
 // Warm start of the potential solve on the free rows (phi elsewhere): hist 2 -> quadratic
 // extrapolation 3 phi - 3 prev + prev2, hist 1 -> linear 2 phi - prev, hist 0 -> phi.
 int nnpp_phi_guess_dev(nnpp_ctx* c, int hist, const double* phi, const double* prev,
