@@ -125,7 +125,9 @@ typedef struct nnpp_step_stats {
   int32_t pot_weight_bytes; /* bytes per stored weight read in mixed mode: 1 (dictionary
                                class) or 4 (FP32 value); 0 for the stencil form */
   int32_t pot_stencil;      /* the operator used the stencil form (16 bytes per row, no
-                               per-entry stream; pot_col_bytes = 0) */
+                               per-entry stream; pot_col_bytes = 0): 1 with the table-driven
+                               loop, 3 with the loop specialised for the radius-3 lattice
+                               ball; 0 for the column form */
   int32_t flux_stencil;     /* the explicit flux used the lattice stencil form (FP64 in both
                                precisions) */
 } nnpp_step_stats;
@@ -315,6 +317,8 @@ int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol
 #define NNPP_FORMAT_WIDE_COLUMNS 1
 #define NNPP_FORMAT_NO_DICTIONARY 2
 #define NNPP_FORMAT_NO_STENCIL 4
+#define NNPP_FORMAT_NO_BALL 8 /* stencil form: the table-driven SpMV loop even where the
+                                 offsets are the radius-3 lattice ball (same results) */
 int nnpp_set_format(nnpp_ctx* ctx, int32_t flags);
 
 /* Solve the potential for the charge of slot N: phi^{n+1} on free nodes (liquid, with a

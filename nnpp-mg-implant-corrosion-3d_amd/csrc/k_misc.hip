@@ -752,6 +752,45 @@ void nnpp_free_stencil(nnpp_stencil& t) {
   t = nnpp_stencil();
 }
 
+// Whether the stencil is the NNPP_BALL_M lattice ball, for the potential SpMV's specialised
+// form: offset t is entry t of nnpp_ball's enumeration at dz*SZ + dy*SY + dx (SY, SZ read off
+// the entries (0, 1, 0) and (1, 0, 0)), a line's entries sit at consecutive LDS positions, and
+// the weights depend on dx^2 + dy^2 + dz^2 only, bit for bit. Fills T.ball and T.ball_args.
+static void detect_ball(nnpp_stencil& T, const std::vector<int32_t>& toff,
+                        const std::vector<int32_t>& lc, const std::vector<float>& w32,
+                        const std::vector<double>& w64) {
+  constexpr nnpp_ball<NNPP_BALL_M> B{};
+  T.ball = 0;
+  if (T.K != B.K) return;
+  int ty = -1, tz = -1;
+  for (int t = 0; t < B.K; t++) {
+    const int l = B.line[t];
+    if (B.ldz[l] == 0 && B.ldy[l] == 1 && B.dx[t] == 0) ty = t;
+    if (B.ldz[l] == 1 && B.ldy[l] == 0 && B.dx[t] == 0) tz = t;
+  }
+  if (ty < 0 || tz < 0) return;
+  const int64_t SY = toff[ty], SZ = toff[tz];
+  nnpp_ball_args a = {};
+  bool seen[NNPP_BALL_CLASSES] = {};
+  for (int t = 0; t < B.K; t++) {
+    const int l = B.line[t];
+    if ((int64_t)toff[t] != B.ldz[l] * SZ + B.ldy[l] * SY + B.dx[t]) return;
+    if (B.pos[t] == 0) a.lc[l] = lc[t];
+    else if (lc[t] != a.lc[l] + B.pos[t]) return;
+    const int d2 = B.d2[t];
+    if (!seen[d2]) {
+      seen[d2] = true;
+      a.w32[d2] = w32[t];
+      a.w64[d2] = w64[t];
+    } else if (std::memcmp(&a.w32[d2], &w32[t], sizeof(float)) ||
+               std::memcmp(&a.w64[d2], &w64[t], sizeof(double))) {
+      return;
+    }
+  }
+  T.ball_args = a;
+  T.ball = NNPP_BALL_M;
+}
+
 // Detect the stencil form of the potential operator for the current geometry (sets
 // c->pst.state to 1 or -1).
 static int build_stencil(nnpp_ctx* c) {
@@ -875,9 +914,12 @@ static int build_stencil(nnpp_ctx* c) {
                                                     T.tf64, c->x, c->y, c->z, wa, wf, T.bmask,
                                                     flags + 1, flags + 2);
   NNPP_HIP(c, hipGetLastError());
-  std::vector<double> tf(K);
+  std::vector<double> tf(K), pw64(K);
+  std::vector<float> pw32(K);
   NNPP_HIP(c, hipMemcpyAsync(hf, flags, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
   NNPP_HIP(c, hipMemcpyAsync(tf.data(), T.tf64, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(pw64.data(), T.tw64, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+  NNPP_HIP(c, hipMemcpyAsync(pw32.data(), T.tw32, sizeof(float) * K, hipMemcpyDeviceToHost, s));
   NNPP_HIP(c, hipStreamSynchronize(s));
   hipFree(reps_d);
   hipFree(flags);
@@ -891,6 +933,7 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipGetLastError());
   for (int t = 0; t < K; t++)
     if (tf[t] != 0.0) T.fnz[t >> 5] |= 1u << (t & 31);
+  detect_ball(T, toff, lc, pw32, pw64);
   T.state = 1;
   return NNPP_OK;
 }

@@ -55,7 +55,19 @@ struct Slices {
   const uint8_t* perm;  // row of each lane within its tile
   int K, R, P;
   int off_min, off_max;  // smallest and largest offset
+  nnpp_ball_args ball;   // the lattice-ball form's line positions and class weights
 };
+
+template <typename T>
+__device__ __forceinline__ T ball_weight(const Slices& sl, int d2);
+template <>
+__device__ __forceinline__ float ball_weight<float>(const Slices& sl, int d2) {
+  return sl.ball.w32[d2];
+}
+template <>
+__device__ __forceinline__ double ball_weight<double>(const Slices& sl, int d2) {
+  return sl.ball.w64[d2];
+}
 
 template <typename T>
 __device__ __forceinline__ const T* stencil_weights(const Slices& sl);
@@ -240,7 +252,55 @@ __device__ __forceinline__ T spmv_row(const Slices& sl, const T* __restrict__ w,
 // ORDERED: the result returned to lane j is row r0 + j's (the column form's lane order, which
 // the FP64 dot products need to match it bit for bit); otherwise lane j gets the result of the
 // row it computed, *row_out.
-template <typename T, bool ORDERED>
+// The chunk loop of stencil_tile for the radius-M lattice ball (M > 0): the same chunks, masks
+// and order of additions as the table-driven loop, so the same sums bit for bit (that loop's
+// padding entries add w = +0 terms to partial sums that are never -0, which changes nothing),
+// with the entries' layout known at compile time: one LDS address per line, the line's entries
+// read at immediate offsets from it, and the weights from the per-class kernel arguments, so no
+// table loads and no per-entry address arithmetic.
+template <typename T, int M>
+__device__ __forceinline__ void ball_rows(const Slices& sl, const T* row, const uint4 m,
+                                          const T vi, RowAcc<T>& acc) {
+  constexpr nnpp_ball<M> B{};
+  constexpr int K = nnpp_ball<M>::K;
+  constexpr int BT = sizeof(T) == 4 ? 32 : 16;
+#pragma unroll
+  for (int t0 = 0; t0 < K; t0 += BT) {
+    const int wd = t0 >> 5;
+    const uint32_t bits =
+        (wd == 0 ? m.x : (wd == 1 ? m.y : (wd == 2 ? m.z : m.w))) >> (t0 & 31);
+    const int nk = K - t0 < BT ? K - t0 : BT;
+    const uint32_t valid = nk >= 32 ? 0xffffffffu : (1u << nk) - 1u;
+    if (!__any(bits & valid)) continue;
+    const bool full = __all((bits & valid) == valid);
+    T pv[BT];
+#pragma unroll
+    for (int k = 0; k < BT; k++)
+      if (k < nk) pv[k] = row[sl.ball.lc[B.line[t0 + k]] + B.pos[t0 + k]];
+    T part[2] = {T(0), T(0)};
+    if (full) {
+#pragma unroll
+      for (int k = 0; k < BT; k++) {
+        if (k >= nk) break;
+        const T w = ball_weight<T>(sl, B.d2[t0 + k]);
+        if constexpr (sizeof(T) == 8) acc.add(w, vi - pv[k]);
+        else part[k & 1] = __fmaf_rn(w, vi - pv[k], part[k & 1]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < BT; k++) {
+        if (k >= nk) break;
+        const int keep = (int)(bits << (31 - k)) >> 31;
+        const T wk = nnpp_mask_weight<T>(ball_weight<T>(sl, B.d2[t0 + k]), keep);
+        if constexpr (sizeof(T) == 8) acc.add(wk, vi - pv[k]);
+        else part[k & 1] = __fmaf_rn(wk, vi - pv[k], part[k & 1]);
+      }
+    }
+    if constexpr (sizeof(T) == 4) acc.add(T(1), part[0] + part[1]);
+  }
+}
+
+template <typename T, bool ORDERED, int M = 0>
 __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict__ dd,
                                           const T* __restrict__ v, int32_t n, int64_t tile,
                                           T* lds, int64_t* row_out) {
@@ -294,7 +354,9 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
   const T* row = lds + l;
   RowAcc<T> acc;
   const int K = sl.K;
-  if (__any(any != 0u)) {
+  if constexpr (M > 0) {
+    if (__any(any != 0u)) ball_rows<T, M>(sl, row, m, vi, acc);
+  } else if (__any(any != 0u)) {
     for (int t0 = 0; t0 < K; t0 += BT) {  // a chunk never straddles a 32-bit mask word
       const int wd = t0 >> 5;
       const uint32_t bits =
@@ -417,7 +479,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_resid(int32_t n, Slices sl,
   nnpp_block_partial<2>(v, part);
 }
 
-template <bool WIDE, bool CLS, bool STEN>
+template <bool WIDE, bool CLS, bool STEN, int M = 0>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
                                                         const double* __restrict__ w,
                                                         const double* __restrict__ dd,
@@ -433,7 +495,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_spmv(int32_t n, Slices sl,
     const int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     double qi = 0.0;
     int64_t ir = i;
-    if constexpr (STEN) qi = stencil_tile<double, true>(sl, dd, p, n, it.cur, st_lds, &ir);
+    if constexpr (STEN) qi = stencil_tile<double, true, M>(sl, dd, p, n, it.cur, st_lds, &ir);
     if (i >= n) continue;
     if constexpr (!STEN) qi = spmv_row<double, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
@@ -549,7 +611,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_replace(int32_t n, const doubl
   nnpp_block_partial<2>(v, part);
 }
 
-template <bool WIDE, bool CLS, bool STEN>
+template <bool WIDE, bool CLS, bool STEN, int M = 0>
 __device__ __forceinline__ void m_spmv(int32_t n, const Slices& sl, const float* __restrict__ w,
                                        const float* __restrict__ dd, const float* __restrict__ p,
                                        float* __restrict__ q, double* __restrict__ part,
@@ -564,7 +626,7 @@ __device__ __forceinline__ void m_spmv(int32_t n, const Slices& sl, const float*
     int64_t i = it.cur * NNPP_BLOCK + threadIdx.x;
     float qi = 0.0f;
     // FP32: the stencil rows stay in their lane order (the dot is FP64 over FP32 products)
-    if constexpr (STEN) qi = stencil_tile<float, false>(sl, dd, p, n, it.cur, st_lds32, &i);
+    if constexpr (STEN) qi = stencil_tile<float, false, M>(sl, dd, p, n, it.cur, st_lds32, &i);
     if (i >= n) continue;
     if constexpr (!STEN) qi = spmv_row<float, WIDE, CLS>(sl, w, tab, dd, p, i);
     q[i] = qi;
@@ -585,7 +647,8 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_spmv(int32_t n, Slices sl,
 }
 
 // Stencil form, at most 72 VGPRs (7 waves per SIMD): measured 4% faster than the compiler's 76
-// (6 waves), with two registers spilled outside the row sums.
+// (6 waves), with two registers spilled outside the row sums. M > 0: the lattice-ball form.
+template <int M>
 __global__ __launch_bounds__(NNPP_BLOCK, 7) void k_m_spmv_st(int32_t n, Slices sl,
                                                              const float* __restrict__ w,
                                                              const float* __restrict__ dd,
@@ -593,7 +656,7 @@ __global__ __launch_bounds__(NNPP_BLOCK, 7) void k_m_spmv_st(int32_t n, Slices s
                                                              float* __restrict__ q,
                                                              double* __restrict__ part,
                                                              const double* __restrict__ scal) {
-  m_spmv<false, false, true>(n, sl, w, dd, p, q, part, scal);
+  m_spmv<false, false, true, M>(n, sl, w, dd, p, q, part, scal);
 }
 
 // e += alpha p, r -= alpha q, z = D^-1 r; stops the batch (DONE) at the target or when the
@@ -701,6 +764,7 @@ void launch_cg_spmv(int var, size_t lds, int64_t g, hipStream_t s, A... a) {
     case 1: k_cg_spmv<false, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
     case 2: k_cg_spmv<true, false, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
     case 3: k_cg_spmv<true, true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
+    case 5: k_cg_spmv<false, false, true, NNPP_BALL_M><<<g, NNPP_BLOCK, lds, s>>>(a...); break;
     default: k_cg_spmv<false, false, true><<<g, NNPP_BLOCK, lds, s>>>(a...); break;
   }
 }
@@ -712,7 +776,8 @@ void launch_m_spmv(int var, size_t lds, int64_t g, hipStream_t s, A... a) {
     case 1: k_m_spmv<false, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
     case 2: k_m_spmv<true, false><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
     case 3: k_m_spmv<true, true><<<g, NNPP_BLOCK, 0, s>>>(a...); break;
-    default: k_m_spmv_st<<<g, NNPP_BLOCK, lds, s>>>(a...); break;
+    case 5: k_m_spmv_st<NNPP_BALL_M><<<g, NNPP_BLOCK, lds, s>>>(a...); break;
+    default: k_m_spmv_st<0><<<g, NNPP_BLOCK, lds, s>>>(a...); break;
   }
 }
 
@@ -749,9 +814,10 @@ PotOp pot_op(const nnpp_ctx* c) {
   const nnpp_stencil& T = c->pst;
   o.sl = Slices{P.off, P.len, P.col, P.d16, P.first, P.cls, P.tab64, P.tab32,
                 T.pmask, T.lc, T.seg_start, T.seg_len, T.seg_base, T.piece, T.tw64, T.tw32,
-                T.perm, T.K, T.R, T.P, T.off_min, T.off_max};
+                T.perm, T.K, T.R, T.P, T.off_min, T.off_max, T.ball_args};
+  const bool ball = nnpp_pot_form(c) == NNPP_BALL_M;
   if (T.active) {
-    o.var = 4;
+    o.var = ball ? 5 : 4;
     o.lds32 = sizeof(float) * T.lds_elems;
     o.lds64 = sizeof(double) * T.lds_elems;
     o.w64 = T.tw64;
@@ -764,8 +830,11 @@ PotOp pot_op(const nnpp_ctx* c) {
   const int64_t cap = nnpp_red_grid(nnpp_nblocks(c->n));
   o.g32 = o.g64 = cap;
   if (T.active) {
-    o.g32 = resident_grid(k_m_spmv_st, o.lds32, c->device, cap);
-    o.g64 = resident_grid(k_cg_spmv<false, false, true>, o.lds64, c->device, cap);
+    o.g32 = ball ? resident_grid(k_m_spmv_st<NNPP_BALL_M>, o.lds32, c->device, cap)
+                 : resident_grid(k_m_spmv_st<0>, o.lds32, c->device, cap);
+    o.g64 = ball ? resident_grid(k_cg_spmv<false, false, true, NNPP_BALL_M>, o.lds64, c->device,
+                                 cap)
+                 : resident_grid(k_cg_spmv<false, false, true>, o.lds64, c->device, cap);
   }
   return o;
 }

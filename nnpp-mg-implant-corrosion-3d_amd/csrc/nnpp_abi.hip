@@ -546,7 +546,7 @@ int nnpp_set_solver(nnpp_ctx* c, double rtol, int32_t maxit, double inner_rtol) 
 
 int nnpp_set_format(nnpp_ctx* c, int32_t flags) {
   if (!c || (flags & ~(NNPP_FORMAT_WIDE_COLUMNS | NNPP_FORMAT_NO_DICTIONARY |
-                       NNPP_FORMAT_NO_STENCIL)))
+                       NNPP_FORMAT_NO_STENCIL | NNPP_FORMAT_NO_BALL)))
     return NNPP_ERR_ARG;
   if (flags != c->format) c->geom_dirty = c->pot_dirty = true;
   c->format = flags;
@@ -634,7 +634,7 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
   loc.flipped += flips;
   loc.nnz_flux = c->bonds.nnz;
   loc.nnz_ll = c->pot.nnz;
-  loc.pot_stencil = c->pst.active;
+  loc.pot_stencil = nnpp_pot_form(c);
   loc.flux_stencil = nnpp_flux_stencil_active(c);
   loc.pot_col_bytes = c->pst.active ? 0 : (c->pot.wide ? 4 : 2);
   loc.pot_weight_bytes = c->pst.active ? 0 : (c->pot.ncls > 0 ? 1 : 4);

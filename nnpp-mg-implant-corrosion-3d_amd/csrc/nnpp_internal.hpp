@@ -67,6 +67,59 @@ struct nnpp_slices {
 // thread and piece.
 #define NNPP_STENCIL_MAX 128
 #define NNPP_STENCIL_CHUNK 32  // tables padded to a multiple of the SpMV chunks (32 / 16)
+
+// Lattice ball of radius M: the offsets (dz, dy, dx) with dx^2 + dy^2 + dz^2 <= M^2 other than
+// (0, 0, 0), in lexicographic order, which is ascending order of dz*SZ + dy*SY + dx on a
+// lattice with row strides SY > 2M and SZ > 2M (SY + 1). Entry t lies on line `line[t]` (the
+// offsets of one (dz, dy), consecutive), `pos[t]` places after the line's first entry.
+template <int M>
+struct nnpp_ball {
+  static constexpr int count(bool lines) {
+    int k = 0;
+    for (int z = -M; z <= M; z++)
+      for (int y = -M; y <= M; y++) {
+        if (z * z + y * y > M * M) continue;
+        if (lines) k++;
+        else
+          for (int x = -M; x <= M; x++)
+            if (z * z + y * y + x * x <= M * M && (x | y | z) != 0) k++;
+      }
+    return k;
+  }
+  static constexpr int K = count(false), L = count(true);
+  int line[K], pos[K], dx[K], d2[K];
+  int ldz[L], ldy[L];
+  constexpr nnpp_ball() : line(), pos(), dx(), d2(), ldz(), ldy() {
+    int t = 0, l = 0;
+    for (int z = -M; z <= M; z++)
+      for (int y = -M; y <= M; y++) {
+        if (z * z + y * y > M * M) continue;
+        ldz[l] = z;
+        ldy[l] = y;
+        int x0 = M + 1;
+        for (int x = -M; x <= M; x++) {
+          if (z * z + y * y + x * x > M * M || (x | y | z) == 0) continue;
+          if (x0 > M) x0 = x;
+          line[t] = l;
+          pos[t] = x - x0;
+          dx[t] = x;
+          d2[t] = z * z + y * y + x * x;
+          t++;
+        }
+        l++;
+      }
+  }
+};
+// the radius the potential SpMV is specialised for (the lattice case with horizon 3 spacings)
+#define NNPP_BALL_M 3
+#define NNPP_BALL_LINES (nnpp_ball<NNPP_BALL_M>::L)
+#define NNPP_BALL_CLASSES (NNPP_BALL_M * NNPP_BALL_M + 1)
+// LDS position of each line's first entry and the weights by dx^2 + dy^2 + dz^2
+struct nnpp_ball_args {
+  int32_t lc[NNPP_BALL_LINES];
+  float w32[NNPP_BALL_CLASSES];
+  double w64[NNPP_BALL_CLASSES];
+};
 struct nnpp_stencil {
   int state = 0;               // 0: not built for this geometry, 1: valid, -1: not a stencil
   int K = 0, R = 0, P = 0;     // offsets, segments, staging pieces
@@ -93,6 +146,8 @@ struct nnpp_stencil {
   uint8_t* perm = nullptr;     // [256 per tile] row of each lane: rows with every entry first,
                                // then the other rows with entries, then the rest (stable)
   int active = 0;              // the current potential operator uses the stencil form
+  int ball = 0;  // the offsets, LDS positions and weights fit the NNPP_BALL_M ball (bit for bit)
+  nnpp_ball_args ball_args = {};
 };
 
 // Differentiation tape (nnpp_tape_begin .. nnpp_tape_end).
@@ -492,3 +547,9 @@ int nnpp_stencil_ensure(nnpp_ctx* c);
 // 1 where the explicit flux takes the lattice stencil form (FP64 in both precisions) for the
 // current geometry.
 int nnpp_flux_stencil_active(nnpp_ctx* c);
+// The potential SpMV's form: 0 column slices, 1 stencil masks with the table-driven loop,
+// NNPP_BALL_M stencil masks with the lattice-ball loop.
+inline int nnpp_pot_form(const nnpp_ctx* c) {
+  if (!c->pst.active) return 0;
+  return (c->pst.ball == NNPP_BALL_M && !(c->format & NNPP_FORMAT_NO_BALL)) ? NNPP_BALL_M : 1;
+}

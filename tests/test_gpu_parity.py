@@ -237,10 +237,10 @@ def test_mixed_step_close_to_fp64():
 
 @pytest.mark.parametrize("kind", ["regular", "screw"])
 def test_storage_formats_bit_identical(kind):
-    """Stencil forms (lattices: the potential operator and the explicit flux), 16-bit column
-    steps vs 32-bit columns, dictionary weight classes
-    vs stored weights: the same additions in the same order, so the trajectories agree bit for
-    bit."""
+    """Stencil forms (lattices: the potential operator and the explicit flux; the SpMV's
+    lattice-ball and table-driven loops), 16-bit column steps vs 32-bit columns, dictionary
+    weight classes vs stored weights: the same additions in the same order, so the trajectories
+    agree bit for bit."""
     say("case storage formats", kind)
     # 16^3 regular: spacing 1/16 makes every bond vector exact (dictionary and stencil apply);
     # the screw case has per-node volumes (column form only)
@@ -248,7 +248,8 @@ def test_storage_formats_bit_identical(kind):
     case.params.front_threshold = 1.0  # flips: the operator is rebuilt on the way
     rp, col, _ = oracle_nb(case)
     W, D, T = nn.FORMAT_WIDE_COLUMNS, nn.FORMAT_NO_DICTIONARY, nn.FORMAT_NO_STENCIL
-    formats = [nn.FORMAT_AUTO, W, D, W | D, T, T | W, T | D, T | W | D]
+    B = nn.FORMAT_NO_BALL  # the table-driven stencil loop instead of the lattice-ball one
+    formats = [nn.FORMAT_AUTO, W, D, W | D, T, T | W, T | D, T | W | D, B, B | T]
     res = {}
     for prec in (nn.FP64, nn.MIXED):
         out = res[prec] = {}
@@ -267,7 +268,8 @@ def test_storage_formats_bit_identical(kind):
         assert st0["flipped"] > 0
         for fmt, (U, st) in out.items():
             stencil = kind == "regular" and not (fmt & T)
-            assert st["pot_stencil"] == int(stencil), fmt
+            # 16^3 lattice with delta = 3 spacings: the offsets are the radius-3 ball
+            assert st["pot_stencil"] == (0 if not stencil else (1 if fmt & B else 3)), fmt
             # the explicit flux takes the stencil form (FP64) wherever the operator does
             assert st["flux_stencil"] == int(stencil), fmt
             if stencil:
@@ -288,7 +290,7 @@ def test_storage_formats_bit_identical(kind):
                     assert rel_err(U[f], res[nn.FP64][nn.FORMAT_AUTO][0][f]) < 1e-9, (fmt, f)
     with problem(case) as pb:
         with pytest.raises(nn.NnppError):
-            pb.set_format(8)
+            pb.set_format(16)
 
 
 @pytest.mark.parametrize("kind", ["regular", "screw"])
@@ -313,7 +315,8 @@ def test_potential_operator_kernel(kind):
     yo = np.where(free, yo * sc, 0.0)
     T = nn.FORMAT_NO_STENCIL
     out = {}
-    for fmt in (nn.FORMAT_AUTO, T, T | nn.FORMAT_NO_DICTIONARY, T | nn.FORMAT_WIDE_COLUMNS):
+    B = nn.FORMAT_NO_BALL
+    for fmt in (nn.FORMAT_AUTO, B, T, T | nn.FORMAT_NO_DICTIONARY, T | nn.FORMAT_WIDE_COLUMNS):
         with problem(case) as pb:
             pb.set_format(fmt)
             pb.set_neighbors(rp, col)
@@ -327,7 +330,8 @@ def test_potential_operator_kernel(kind):
     yc32 = out[T][1]
     for fmt, (a, b) in out.items():
         assert np.array_equal(a, y64), fmt  # FP64: the same additions in every form
-        assert np.array_equal(b, yc32) or fmt == nn.FORMAT_AUTO, fmt  # FP32 column forms
+        # FP32: the column forms agree among themselves, the two stencil loops between them
+        assert np.array_equal(b, y32 if fmt in (nn.FORMAT_AUTO, B) else yc32), fmt
     # FP32 stencil rows (chunk sums, compensated across chunks) against FP32 column rows
     assert np.abs(y32 - yc32).max() <= 4e-7 * np.abs(y64).max()
 
