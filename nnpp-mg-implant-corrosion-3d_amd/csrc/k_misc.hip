@@ -886,7 +886,9 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMalloc((void**)&T.seg_start, sizeof(int32_t) * R));
   NNPP_HIP(c, hipMalloc((void**)&T.seg_len, sizeof(int32_t) * R));
   NNPP_HIP(c, hipMalloc((void**)&T.seg_base, sizeof(int32_t) * R));
-  NNPP_HIP(c, hipMalloc((void**)&T.piece, sizeof(int32_t) * 3 * P));
+  const int Ppad = (P + NNPP_PIECE_BATCH - 1) / NNPP_PIECE_BATCH * NNPP_PIECE_BATCH;
+  pc.resize(3 * (size_t)Ppad, 0);  // empty pieces: the SpMV reads whole batches of the table
+  NNPP_HIP(c, hipMalloc((void**)&T.piece, sizeof(int32_t) * 3 * Ppad));
   NNPP_HIP(c, hipMalloc((void**)&T.tw64, sizeof(double) * Kp));
   NNPP_HIP(c, hipMalloc((void**)&T.tw32, sizeof(float) * Kp));
   NNPP_HIP(c, hipMemsetAsync(T.tw64, 0, sizeof(double) * Kp, s));
@@ -903,7 +905,7 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMemcpyAsync(T.seg_start, rs.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(T.seg_len, rl.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
   NNPP_HIP(c, hipMemcpyAsync(T.seg_base, rb.data(), sizeof(int32_t) * R, hipMemcpyHostToDevice, s));
-  NNPP_HIP(c, hipMemcpyAsync(T.piece, pc.data(), sizeof(int32_t) * 3 * P, hipMemcpyHostToDevice,
+  NNPP_HIP(c, hipMemcpyAsync(T.piece, pc.data(), sizeof(int32_t) * 3 * Ppad, hipMemcpyHostToDevice,
                              s));
   const WeightArgs wa = weight_args(c, 3.0 + 2.0 * c->p.s_phi);
   const WeightArgs wf = weight_args(c, c->w_exponent);

@@ -316,21 +316,23 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
     // and last few); a piece's source and destination are wave-uniform
     const bool inner = r0 + sl.off_min >= 0 && r0 + NNPP_BLOCK - 1 + sl.off_max < n;
     if (inner) {
-      // the loads of 8 pieces are issued together, then stored: one memory latency per 8 pieces
-      constexpr int PB = 8;
+      // the loads of 8 pieces are issued together, then stored: one memory latency per 8
+      // pieces. The batch's table words are read first, unconditionally (the table is padded
+      // to whole batches with empty pieces), so they arrive in one scalar round trip instead
+      // of two dependent ones per piece.
+      constexpr int PB = NNPP_PIECE_BATCH;
       const int P = sl.P;
       for (int pb = 0; pb < P; pb += PB) {
+        int32_t pt[3 * PB];
+#pragma unroll
+        for (int k = 0; k < 3 * PB; k++) pt[k] = nnpp_ld_table(sl.piece, 3 * pb + k);
         T val[PB];
 #pragma unroll
-        for (int k = 0; k < PB; k++) {
-          if (pb + k < P && (int)threadIdx.x < nnpp_ld_table(sl.piece, 3 * (pb + k) + 2))
-            val[k] = v[r0 + nnpp_ld_table(sl.piece, 3 * (pb + k)) + threadIdx.x];
-        }
+        for (int k = 0; k < PB; k++)
+          if ((int)threadIdx.x < pt[3 * k + 2]) val[k] = v[r0 + pt[3 * k] + threadIdx.x];
 #pragma unroll
-        for (int k = 0; k < PB; k++) {
-          if (pb + k < P && (int)threadIdx.x < nnpp_ld_table(sl.piece, 3 * (pb + k) + 2))
-            lds[nnpp_ld_table(sl.piece, 3 * (pb + k) + 1) + threadIdx.x] = val[k];
-        }
+        for (int k = 0; k < PB; k++)
+          if ((int)threadIdx.x < pt[3 * k + 2]) lds[pt[3 * k + 1] + threadIdx.x] = val[k];
       }
     } else {
       const int R = sl.R;

@@ -67,6 +67,8 @@ struct nnpp_slices {
 // thread and piece.
 #define NNPP_STENCIL_MAX 128
 #define NNPP_STENCIL_CHUNK 32  // tables padded to a multiple of the SpMV chunks (32 / 16)
+#define NNPP_PIECE_BATCH 8     // staging pieces per batch; the piece table is padded to whole
+                               // batches with empty pieces (count 0)
 
 // Lattice ball of radius M: the offsets (dz, dy, dx) with dx^2 + dy^2 + dz^2 <= M^2 other than
 // (0, 0, 0), in lexicographic order, which is ascending order of dz*SZ + dy*SY + dx on a
@@ -131,8 +133,9 @@ struct nnpp_stencil {
   int32_t* seg_start = nullptr;  // [R] first offset of segment r
   int32_t* seg_len = nullptr;    // [R] last offset - first offset + 1
   int32_t* seg_base = nullptr;   // [R] LDS base of segment r
-  int32_t* piece = nullptr;      // [3P] per piece: vector position relative to the tile's first
-                                 // row, LDS position, value count (1..256)
+  int32_t* piece = nullptr;      // [3P, padded to whole NNPP_PIECE_BATCH batches] per piece:
+                                 // vector position relative to the tile's first row, LDS
+                                 // position, value count (1..256; 0 in the padding)
   double* tw64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] potential weight of
   float* tw32 = nullptr;       // offset t (FP64 / FP32; padding: 0)
   double* tf64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] flux weight of offset t
