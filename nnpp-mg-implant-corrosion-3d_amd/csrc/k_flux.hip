@@ -507,18 +507,25 @@ struct FluxSt {
 struct StSlots {
   uint32_t off[ST_SLOTS];
 
+  // Segments outermost: a segment's three table words are read together (one scalar round
+  // trip per segment; per slot and segment, with the reads behind the range tests, they were
+  // several hundred dependent round trips per block), then every slot is tested against it.
   __device__ __forceinline__ void init(const FluxSt& st) {
+    int32_t pos[ST_SLOTS];
 #pragma unroll
-    for (int k = 0; k < ST_SLOTS; k++) {
-      const int e = (int)threadIdx.x + k * NNPP_BLOCK;
-      int32_t pos = st.off_min;
-      for (int r = 0; r < st.R; r++) {
-        const int b = nnpp_ld_table(st.seg_base, r);
-        if (e >= b && e < b + NNPP_BLOCK + nnpp_ld_table(st.seg_len, r) - 1)
-          pos = nnpp_ld_table(st.seg_start, r) + (e - b);
+    for (int k = 0; k < ST_SLOTS; k++) pos[k] = st.off_min;
+    for (int r = 0; r < st.R; r++) {
+      const int b = nnpp_ld_table(st.seg_base, r);
+      const int end = b + NNPP_BLOCK + nnpp_ld_table(st.seg_len, r) - 1;
+      const int s0 = nnpp_ld_table(st.seg_start, r);
+#pragma unroll
+      for (int k = 0; k < ST_SLOTS; k++) {
+        const int e = (int)threadIdx.x + k * NNPP_BLOCK;
+        if (e >= b && e < end) pos[k] = s0 + (e - b);
       }
-      off[k] = (uint32_t)(pos - st.off_min);
     }
+#pragma unroll
+    for (int k = 0; k < ST_SLOTS; k++) off[k] = (uint32_t)(pos[k] - st.off_min);
   }
 
   template <typename T>
