@@ -446,17 +446,25 @@ __device__ __forceinline__ void nnpp_final_sum(double (&v)[K], const double* __r
                                                int64_t nblk) {
   __shared__ double lds[K * (NNPP_BLOCK / NNPP_WAVE)];
   constexpr int PER = NNPP_RED_GRID / NNPP_BLOCK;
+  // every load is issued (from block 0 past nblk, then replaced by 0) before any sum, so the
+  // K * PER loads take one memory round trip; loads guarded by b < nblk became one branch each
+  // and the first sum waited inside it
+  double t[K][PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int64_t b = threadIdx.x + (int64_t)q * NNPP_BLOCK;
+    const int64_t bb = b < nblk ? b : 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) t[k][q] = part[bb * K + k];
+  }
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    double t[PER];
+    double a = 0.0;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       const int64_t b = threadIdx.x + (int64_t)q * NNPP_BLOCK;
-      t[q] = (b < nblk) ? part[b * K + k] : 0.0;
+      a += (b < nblk) ? t[k][q] : 0.0;
     }
-    double a = 0.0;
-#pragma unroll
-    for (int q = 0; q < PER; q++) a += t[q];
     v[k] = a;
   }
   nnpp_block_sum<K>(v, lds);
