@@ -45,13 +45,13 @@ def field_errors(a, b):
     return out
 
 
-def run_mode(nn, case, rp, col, precision, warmup, steps, rtol):
+def run_mode(nn, case, rp, col, precision, warmup, steps, rtol, inner_rtol=1e-3):
     import torch
     pb = nn.Problem(case.params, case.x, case.y, case.z, case.vol, device=0)
     pb.set_neighbors(rp, col)
     pb.set_bc(case.bctypes, case.bcvalues)
     pb.init_state(case.icvalues)
-    pb.set_solver(rtol=rtol, maxit=50000, inner_rtol=1e-4)
+    pb.set_solver(rtol=rtol, maxit=50000, inner_rtol=inner_rtol)
     if warmup:
         pb.step(warmup, precision)
     torch.cuda.synchronize()
@@ -371,6 +371,8 @@ def main():
                     help="regular lattice with a Mg sphere (default) or the jittered screw "
                          "geometry with per-node volumes")
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--inner-rtol", type=float, default=1e-3,
+                    help="mixed mode: residual-norm reduction of each FP32 refinement pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--kernels-only", action="store_true",
@@ -436,7 +438,7 @@ def main():
     res = {}
     for name, prec in (("fp64", nn.FP64), ("mixed", nn.MIXED)):
         log(f"[bench] {name}: warmup {a.warmup}, timed {a.steps}")
-        pbm, st, dt = run_mode(nn, case, rp, col, prec, a.warmup, a.steps, a.rtol)
+        pbm, st, dt = run_mode(nn, case, rp, col, prec, a.warmup, a.steps, a.rtol, a.inner_rtol)
         bu_flux = st["steps"] * st["nnz_flux"]
         bu_cg = (st["cg_spmv_fp64"] + st["cg_spmv_fp32"]) * st["nnz_ll"]
         res[name] = {"time_s": dt, "stats": st, "bu": bu_flux + bu_cg, "bu_flux": bu_flux,

@@ -302,7 +302,8 @@ int nnpp_potential_operator_apply(nnpp_ctx* ctx, int32_t precision, const double
 
 /* Solver controls for the potential (replaces Eigen GMRES at main.cpp:764-779):
  * relative residual rtol (default 1e-10), maxit (default 20000); for NNPP_MIXED the inner
- * FP32 solve reduces the residual by inner_rtol per refinement pass (default 1e-4). */
+ * FP32 solve reduces the residual norm by inner_rtol per refinement pass before the FP64
+ * residual replaces it (default 1e-3). */
 int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol);
 
 /* Storage format of the bond slices and the potential operator. NNPP_FORMAT_AUTO (default):

@@ -16,7 +16,7 @@
 // a batch exit at once, so the host synchronises once per batch.
 // MIXED: CG with reliable updates. The iteration runs in FP32 (FP32 weights and vectors,
 // Kahan-compensated bond sums, FP64 dot products) on the residual normalised by ||b||; the
-// solution is kept in FP64. Whenever the FP32 recursive residual has dropped by a factor 100
+// solution is kept in FP64. Whenever the FP32 recursive residual has dropped by inner_rtol (1e-3)
 // (or reached the target), the FP32 correction is added to the FP64 solution, the residual is
 // recomputed in FP64 (FP64 weights and vectors) and replaces the FP32 one, and the iteration
 // continues with the same search direction. The FP64 residual is the stopping test.
@@ -850,7 +850,6 @@ static int read_scalars(nnpp_ctx* c) {
   return NNPP_OK;
 }
 
-static constexpr double RELIABLE_DROP2 = 1e-4;  // replace the residual every 2 decades
 
 // Iterations of the next launch batch: the host sees the residual only between batches, and
 // iterations launched after the stop exit at once but still cost their launches (about 16 us
@@ -987,6 +986,9 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   } else {
     const double sb = 1.0 / std::sqrt(bnorm2);  // FP32 vectors live in units of ||b||
     const double target2 = rtol2 * bnorm2;
+    // a refinement pass ends (reliable update) once the FP32 residual norm has dropped by
+    // inner_rtol: its square by inner_rtol^2
+    const double drop2 = c->inner_rtol * c->inner_rtol;
     if ((st = resid(false))) return st;
     rr = c->h_scal[SC_RR];
     if (rr > target2) {
@@ -997,7 +999,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       par = 0;
       BatchPlan plan;
       if (!rhs) plan.prev = &c->cg_phase_len[1];
-      double rr_b = rr / bnorm2, stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
+      double rr_b = rr / bnorm2, stop_b = std::max(rtol2, drop2 * rr_b);
       int it_b = 0;
       while (true) {
         const int left = c->maxit - total;
@@ -1012,7 +1014,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
           k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
                                               c->f_z, part, part_s, op.g32, c->scal, par);
           k_m_dir<<<g, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, part, g, c->scal, par,
-                                           RELIABLE_DROP2);
+                                           drop2);
         }
         NNPP_HIP(c, hipGetLastError());
         if ((st = read_scalars(c))) return st;
@@ -1021,7 +1023,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
           return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
         plan.observe(rr_b, c->h_scal[SC_RR], total - it_b);
         rr_b = c->h_scal[SC_RR];
-        stop_b = std::max(c->h_scal[SC_TARGET2], RELIABLE_DROP2 * c->h_scal[SC_RRMAX]);
+        stop_b = std::max(c->h_scal[SC_TARGET2], drop2 * c->h_scal[SC_RRMAX]);
         it_b = total;
         if (c->h_scal[SC_DONE] == 0.0) continue;
         plan.end_phase(total);
@@ -1040,7 +1042,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         if ((st = nnpp_finalize(c, FIN_M_REPLACE, g, rtol2))) return st;
         par = 0;
         rr_b = rr / bnorm2;  // the replaced residual starts the next phase
-        stop_b = std::max(rtol2, RELIABLE_DROP2 * rr_b);
+        stop_b = std::max(rtol2, drop2 * rr_b);
       }
       n32 += total;
       if (!rhs && status == NNPP_OK) c->cg_phase_len[1] = plan.lens;

@@ -246,7 +246,7 @@ struct nnpp_ctx {
   // the step kind, per precision (0 FP64, 1 mixed): the next solve sizes its launch batches by
   // them (successive steps repeat the pattern closely)
   std::vector<int32_t> cg_phase_len[2];
-  double inner_rtol = 1e-4;
+  double inner_rtol = 1e-3;  // residual-norm reduction per FP32 refinement pass
 
   // implicit (reference Newton) steps, allocated on first use
   double* imp = nullptr;       // species vectors, S*n each: x r rh p v s t d0 diag

@@ -172,7 +172,7 @@ class Problem:
         self._check(self.lib.nnpp_potential_operator_apply(self.ctx, precision, dptr(x), dptr(y)))
         return y
 
-    def set_solver(self, rtol=1e-10, maxit=20000, inner_rtol=1e-4):
+    def set_solver(self, rtol=1e-10, maxit=20000, inner_rtol=1e-3):
         self._check(self.lib.nnpp_set_solver(self.ctx, rtol, maxit, inner_rtol))
 
     def set_format(self, flags=0):
