@@ -149,6 +149,33 @@ def test_poisson_solve_vs_oracle(precision):
         assert abs(it - it_o) <= 2
 
 
+def test_mixed_inner_rtol_sets_refinement_passes():
+    """inner_rtol is the residual-norm reduction of each FP32 pass of the mixed solve: a larger
+    one ends passes sooner (more FP64 residual replacements); both stop at the same FP64
+    relative residual, so the states agree to the solver tolerance. Values outside (0, 1) are
+    refused."""
+    say("case mixed inner_rtol")
+    case = nn.synth.regular(20, S=11, seed=2)
+    rp, col, _ = oracle_nb(case)
+    out = {}
+    for ir in (1e-1, 1e-4):
+        with problem(case) as pb:
+            pb.set_neighbors(rp, col)
+            pb.set_bc(case.bctypes, case.bcvalues)
+            pb.init_state(case.icvalues)
+            pb.set_solver(rtol=1e-12, maxit=100000, inner_rtol=ir)
+            st = pb.step(3, nn.MIXED)
+            assert st["status"] == 0 and st["last_relres"] <= 1e-12
+            out[ir] = (st["cg_spmv_fp64"], pb.get_state())
+    assert out[1e-1][0] > out[1e-4][0], (out[1e-1][0], out[1e-4][0])
+    for f in range(case.params.S + 1):
+        assert rel_err(out[1e-1][1][f], out[1e-4][1][f]) < 1e-9, f
+    with problem(case) as pb:
+        for bad in (0.0, 1.0, -1e-3):
+            with pytest.raises(nn.NnppError):
+                pb.set_solver(rtol=1e-10, maxit=100, inner_rtol=bad)
+
+
 def test_step_fp64_matches_oracle():
     say("case step fp64 regular16")
     case = nn.synth.regular(16, S=11, seed=0)
