@@ -333,7 +333,7 @@ def measured_traffic(kernel_prefix):
     return None
 
 
-def implicit_workload(nn, kind, ng, steps, warmup, lin_rtol, newton_tol):
+def implicit_workload(nn, kind, ng, steps, warmup, lin_rtol, newton_tol, precision="fp64"):
     """The reference's implicit step (modified Newton with the frozen Jacobian, main.cpp's
     tolerances unless overridden) on the same synthetic case: per-step time and the iteration
     counts behind it. Not the headline metric (that is the explicit step)."""
@@ -344,14 +344,16 @@ def implicit_workload(nn, kind, ng, steps, warmup, lin_rtol, newton_tol):
     pb.set_bc(case.bctypes, case.bcvalues)
     pb.init_state(case.icvalues)
     opts = dict(lin_rtol=lin_rtol, newton_tol=newton_tol)
+    prec = nn.MIXED if precision == "mixed" else nn.FP64  # the potential solves' precision
     if warmup:
-        pb.step_implicit(warmup, nn.FP64, **opts)
+        pb.step_implicit(warmup, prec, **opts)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    st = pb.step_implicit(steps, nn.FP64, **opts)
+    st = pb.step_implicit(steps, prec, **opts)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     out = {"workload": f"implicit (reference Newton) step, {kind} {ng}^3, 11 species + phi",
+           "potential_precision": precision,
            "nodes": case.n, "bonds": nnz, "steps": st["steps"], "warmup": warmup,
            "ms_per_step": dt / max(st["steps"], 1) * 1e3, "newton_tol": newton_tol,
            "lin_rtol": lin_rtol, "stats": st}
@@ -390,6 +392,8 @@ def main():
                          "checkpoints every 20; loss against a +5%% diffusivity target)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-grid", type=int, default=64)
+    ap.add_argument("--implicit-precision", choices=["fp64", "mixed"], default="fp64",
+                    help="--implicit: precision of the potential solves (species stay FP64)")
     ap.add_argument("--implicit", action="store_true",
                     help="only time the reference's implicit (Newton) step on the case")
     ap.add_argument("--lin-rtol", type=float, default=1e-5)
@@ -421,7 +425,8 @@ def main():
         adjoint_workload(nn, a.adjoint_grid, a.adjoint_steps)
         return
     if a.implicit:
-        implicit_workload(nn, a.case, ng, a.steps, a.warmup, a.lin_rtol, a.newton_tol)
+        implicit_workload(nn, a.case, ng, a.steps, a.warmup, a.lin_rtol, a.newton_tol,
+                          a.implicit_precision)
         return
     log(f"[bench] building {a.case} {ng}^3 case")
     case = (nn.synth.regular if a.case == "regular" else nn.synth.screw)(ng, S=11, seed=0)
