@@ -171,6 +171,24 @@ def test_stencil_forms_small_lattices(shape):
     _exact_run(shape, True)
 
 
+def test_stencil_table_loop_when_not_the_radius3_ball():
+    """delta = 2 spacings: the offsets are the radius-2 ball (32 of them), so the stencil forms
+    apply but the SpMV's lattice-ball loop (radius 3) does not; the table-driven loop runs and
+    matches the oracle."""
+    say("case radius-2 lattice")
+    case = exact_lattice(9, 9, 9, seed=5)
+    case.params.delta = 2.0 / 16
+    case.params.front_threshold = 1.0
+    for prec in (nn.FP64, nn.MIXED):
+        ref, U, ph, st = run_both(case, 3, prec)
+        assert st["status"] == 0
+        assert st["pot_stencil"] == 1 and st["flux_stencil"] == 1, prec
+        assert np.array_equal(ph, ref["phase"])
+        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        for f in range(case.params.S + 1):
+            assert rel_err(U[f], ref["U"][f]) < tol, (prec, f)
+
+
 def test_stencil_refused_when_offsets_alias():
     """In a lattice 4 nodes wide, (3,0,0) and (-1,1,0) are both bonds with the offset j - i = 3
     and different weights: the per-offset weight check rejects the stencil forms, and the
