@@ -675,8 +675,9 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_update(int32_t n, const float*
   if (cg_done(scal)) return;
   const float alpha = (float)cg_alpha(part_s, nblk_s, scal, par);
   double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * NNPP_BLOCK) {
+  // four consecutive nodes per thread and 16-byte accesses (every vector is its own 256-byte
+  // aligned allocation); the last n % 4 nodes go to the first lanes of block 0
+  const auto node = [&](int64_t i) {
     e[i] += alpha * p[i];
     const float ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -684,7 +685,41 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_update(int32_t n, const float*
     zv[i] = zi;
     v[0] += (double)ri * (double)zi;
     v[1] += (double)ri * (double)ri;
+  };
+  const int64_t n4 = n >> 2;
+  for (int64_t j = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; j < n4;
+       j += (int64_t)gridDim.x * NNPP_BLOCK) {
+    const float4 pj = reinterpret_cast<const float4*>(p)[j];
+    const float4 qj = reinterpret_cast<const float4*>(q)[j];
+    const float4 dj = reinterpret_cast<const float4*>(pinv)[j];
+    float4 ej = reinterpret_cast<const float4*>(e)[j];
+    float4 rj = reinterpret_cast<const float4*>(r)[j];
+    float4 zj;
+    ej.x += alpha * pj.x;
+    ej.y += alpha * pj.y;
+    ej.z += alpha * pj.z;
+    ej.w += alpha * pj.w;
+    rj.x = rj.x - alpha * qj.x;
+    rj.y = rj.y - alpha * qj.y;
+    rj.z = rj.z - alpha * qj.z;
+    rj.w = rj.w - alpha * qj.w;
+    zj.x = rj.x * dj.x;
+    zj.y = rj.y * dj.y;
+    zj.z = rj.z * dj.z;
+    zj.w = rj.w * dj.w;
+    reinterpret_cast<float4*>(e)[j] = ej;
+    reinterpret_cast<float4*>(r)[j] = rj;
+    reinterpret_cast<float4*>(zv)[j] = zj;
+    v[0] += (double)rj.x * (double)zj.x;
+    v[0] += (double)rj.y * (double)zj.y;
+    v[0] += (double)rj.z * (double)zj.z;
+    v[0] += (double)rj.w * (double)zj.w;
+    v[1] += (double)rj.x * (double)rj.x;
+    v[1] += (double)rj.y * (double)rj.y;
+    v[1] += (double)rj.z * (double)rj.z;
+    v[1] += (double)rj.w * (double)rj.w;
   }
+  if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4) node(4 * n4 + threadIdx.x);
   nnpp_block_partial<2>(v, part_u);
 }
 
@@ -702,9 +737,22 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_m_dir(int32_t n, const float* __
   double b;
   if (cg_beta(part_u, nblk_u, scal, par, reliable, b)) return;
   const float beta = (float)b;
-  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * NNPP_BLOCK)
+  // four nodes per thread, 16-byte accesses; the last n % 4 nodes in block 0
+  const int64_t n4 = n >> 2;
+  for (int64_t j = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; j < n4;
+       j += (int64_t)gridDim.x * NNPP_BLOCK) {
+    const float4 zj = reinterpret_cast<const float4*>(zv)[j];
+    float4 pj = reinterpret_cast<const float4*>(p)[j];
+    pj.x = zj.x + beta * pj.x;
+    pj.y = zj.y + beta * pj.y;
+    pj.z = zj.z + beta * pj.z;
+    pj.w = zj.w + beta * pj.w;
+    reinterpret_cast<float4*>(p)[j] = pj;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4) {
+    const int64_t i = 4 * n4 + threadIdx.x;
     p[i] = zv[i] + beta * p[i];
+  }
 }
 
 // p = z + BETA p with the BETA of the iteration that stopped (before a residual replacement).
