@@ -558,8 +558,33 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_update(int32_t n, const doubl
   if (cg_done(scal)) return;
   const double alpha = cg_alpha(part_s, nblk_s, scal, par);
   double v[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * NNPP_BLOCK) {
+  // two consecutive nodes per thread and 16-byte accesses (every vector is its own 256-byte
+  // aligned allocation); an odd last node goes to lane 0 of block 0
+  const int64_t n2 = n >> 1;
+  for (int64_t j = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; j < n2;
+       j += (int64_t)gridDim.x * NNPP_BLOCK) {
+    const double2 pj = reinterpret_cast<const double2*>(p)[j];
+    const double2 qj = reinterpret_cast<const double2*>(q)[j];
+    const double2 dj = reinterpret_cast<const double2*>(pinv)[j];
+    double2 xj = reinterpret_cast<const double2*>(x)[j];
+    double2 rj = reinterpret_cast<const double2*>(r)[j];
+    double2 zj;
+    xj.x += alpha * pj.x;
+    xj.y += alpha * pj.y;
+    rj.x = rj.x - alpha * qj.x;
+    rj.y = rj.y - alpha * qj.y;
+    zj.x = rj.x * dj.x;
+    zj.y = rj.y * dj.y;
+    reinterpret_cast<double2*>(x)[j] = xj;
+    reinterpret_cast<double2*>(r)[j] = rj;
+    reinterpret_cast<double2*>(zv)[j] = zj;
+    v[0] += rj.x * zj.x;
+    v[0] += rj.y * zj.y;
+    v[1] += rj.x * rj.x;
+    v[1] += rj.y * rj.y;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+    const int64_t i = n - 1;
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -579,9 +604,17 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_cg_dir(int32_t n, const double* 
   if (cg_done(scal)) return;
   double beta;
   if (cg_beta(part_u, nblk_u, scal, par, 0.0, beta)) return;
-  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * NNPP_BLOCK)
-    p[i] = zv[i] + beta * p[i];
+  // two nodes per thread, 16-byte accesses; an odd last node in lane 0 of block 0
+  const int64_t n2 = n >> 1;
+  for (int64_t j = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; j < n2;
+       j += (int64_t)gridDim.x * NNPP_BLOCK) {
+    const double2 zj = reinterpret_cast<const double2*>(zv)[j];
+    double2 pj = reinterpret_cast<const double2*>(p)[j];
+    pj.x = zj.x + beta * pj.x;
+    pj.y = zj.y + beta * pj.y;
+    reinterpret_cast<double2*>(p)[j] = pj;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) p[n - 1] = zv[n - 1] + beta * p[n - 1];
 }
 
 // ---- mixed precision: CG with reliable updates ----
