@@ -147,7 +147,7 @@ int main(int argc, char** argv) {
   }
   if ((st = nnpp_set_bc(c, bct.data(), bcv.data()))) die("nnpp_set_bc", st, c);
   if ((st = nnpp_init_state(c, ic.data()))) die("nnpp_init_state", st, c);
-  if ((st = nnpp_set_solver(c, rtol, 20000, 1e-4))) die("nnpp_set_solver", st, c);
+  if ((st = nnpp_set_solver(c, rtol, 20000, 1e-3))) die("nnpp_set_solver", st, c);
   if (implicit && (st = nnpp_set_state(c, NNPP_SLOT_N, ic.data())))  // raw U^n, main.cpp:400-411
     die("nnpp_set_state", st, c);
   if ((st = nnpp_get_state(c, NNPP_SLOT_N, U.data()))) die("nnpp_get_state", st, c);
