@@ -7,6 +7,11 @@ flux, potential and explicit-update kernels, recomputing each segment of `checkp
 steps from its checkpoint, and returns dL/dD_liquid, dL/dD_solid and dL/dr_diss (generic
 chemistry). Only the parameters are differentiable; the start state is a constant.
 The state is exchanged with the library through host arrays (the C ABI takes host pointers).
+
+A Problem holds one tape, and the backward of a graph reverses that tape. The graph remembers
+the tape's generation, the number of steps it recorded and the parameters; backward raises
+RuntimeError instead of returning the gradient of another trajectory when a later simulate()
+replaced the tape, when further steps were appended to it, or when the tape was ended.
 """
 import numpy as np
 import torch
@@ -36,12 +41,24 @@ class NnppSteps(torch.autograd.Function):
             raise RuntimeError(f"forward steps ended with status {st['status']}")
         ctx.problem = problem
         ctx.device = D_liquid.device
+        ctx.tape = problem.tape_info()
+        ctx.params = (dl.copy(), ds.copy(), p.r_diss)
         return torch.from_numpy(problem.get_state()).to(D_liquid.device)
 
     @staticmethod
     def backward(ctx, gU):
         pb = ctx.problem
         S = pb.S
+        if pb.tape_info() != ctx.tape:
+            raise RuntimeError(
+                "the differentiation tape of this graph was replaced, extended or ended "
+                f"(recorded tape {ctx.tape}, problem now holds {pb.tape_info()}); run simulate() "
+                "again before backward")
+        cur = pb.params
+        dl, ds, rd = ctx.params
+        if (any(cur.D_liquid[k] != dl[k] or cur.D_solid[k] != ds[k] for k in range(S))
+                or cur.r_diss != rd):
+            raise RuntimeError("the problem's parameters changed since this graph's forward")
         grad, _, stats = pb.tape_adjoint(gU.detach().to("cpu", torch.float64).numpy())
         ctx.problem.last_adjoint_stats = stats
         g = torch.from_numpy(grad).to(ctx.device)

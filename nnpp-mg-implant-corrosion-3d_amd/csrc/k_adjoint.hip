@@ -328,6 +328,25 @@ int nnpp_adjoint_run(nnpp_ctx* c, const double* lam_h, double* grad_h, double* l
   const int64_t save_step = c->step, save_solid = c->solid_count;
   const double save_vr = c->voln1rel;
 
+  // Every exit after this point restores the state saved above: the reversal overwrites the
+  // context's state, phase and step scalars with checkpoint and recomputed states.
+  auto restore = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(c->U[NNPP_SLOT_N], Usave, fb, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->phase, Psave, m, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && save_hist > 0)
+      e = hipMemcpyAsync(c->phi_prev, PhiSave, sizeof(double) * 2 * m, hipMemcpyDeviceToDevice,
+                         s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    c->phi_hist = save_hist;
+    c->step = save_step;
+    c->solid_count = save_solid;
+    c->voln1rel = save_vr;
+    c->pot_dirty = true;
+    return e != hipSuccess ? e : e2;
+  };
+  std::vector<double> hacc(NG + 1);
+  const int run_status = [&]() -> int {
+
   AdjArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n;
@@ -455,23 +474,16 @@ int nnpp_adjoint_run(nnpp_ctx* c, const double* lam_h, double* grad_h, double* l
     }
   }
 
-  std::vector<double> hacc(NG + 1);
   NNPP_HIP(c, hipMemcpyAsync(hacc.data(), acc, sizeof(double) * (NG + 1), hipMemcpyDeviceToHost, s));
   if (lam0_h)
     NNPP_HIP(c, hipMemcpyAsync(lam0_h, lamA, sizeof(double) * n * F, hipMemcpyDeviceToHost, s));
-  // leave the state as it was before the call
-  NNPP_HIP(c, hipMemcpyAsync(c->U[NNPP_SLOT_N], Usave, fb, hipMemcpyDeviceToDevice, s));
-  NNPP_HIP(c, hipMemcpyAsync(c->phase, Psave, m, hipMemcpyDeviceToDevice, s));
-  if (save_hist > 0)
-    NNPP_HIP(c, hipMemcpyAsync(c->phi_prev, PhiSave, sizeof(double) * 2 * m,
-                               hipMemcpyDeviceToDevice, s));
-  c->phi_hist = save_hist;
-  NNPP_HIP(c, hipStreamSynchronize(s));
-  c->step = save_step;
-  c->solid_count = save_solid;
-  c->voln1rel = save_vr;
-  c->pot_dirty = true;
+  return status;
+  }();
+  // leave the state as it was before the call, on success and on every error
+  const hipError_t re = restore();
+  if (run_status != NNPP_OK && run_status != NNPP_ERR_NOCONV) return run_status;
+  if (re != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(re));
   for (int q = 0; q <= 2 * S; q++) grad_h[q] = hacc[q];
   if (stats) *stats = st_;
-  return status;
+  return run_status;
 }

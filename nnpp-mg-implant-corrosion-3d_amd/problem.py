@@ -34,6 +34,11 @@ class Problem:
         self._check(self.lib.nnpp_set_nodes(self.ctx, self.n, dptr(self.x), dptr(self.y),
                                             dptr(self.z), dptr(self.vol) if self.vol is not None else None))
         self.nnz = None
+        # differentiation tape bookkeeping: a generation number per tape_begin and the steps
+        # recorded since, so a graph built on one tape can tell that it was replaced or extended
+        self._tape_gen = 0
+        self._tape_active = False
+        self._tape_steps = 0
 
     # -- plumbing ------------------------------------------------------------------------
     def _check(self, st):
@@ -205,6 +210,8 @@ class Problem:
         st = self.lib.nnpp_step(self.ctx, nsteps, precision, int(check_finite), C.byref(s))
         d = s.as_dict()
         d["status"] = st
+        if self._tape_active:
+            self._tape_steps += int(d["steps"])
         if st not in (_abi.OK, _abi.ERR_NOCONV):
             self._check(st)
         return d
@@ -260,6 +267,13 @@ class Problem:
     def tape_begin(self, checkpoint_every=1):
         """Record the following FP64 steps for tape_adjoint (state checkpoint every k steps)."""
         self._check(self.lib.nnpp_tape_begin(self.ctx, int(checkpoint_every)))
+        self._tape_gen += 1
+        self._tape_active = True
+        self._tape_steps = 0
+
+    def tape_info(self):
+        """(generation of the current tape, recording, steps recorded on it)."""
+        return self._tape_gen, self._tape_active, self._tape_steps
 
     def tape_adjoint(self, lam, want_lam0=False):
         """Reverse the recorded steps for dL/dU = lam (F, n) at the current state. Returns
@@ -276,6 +290,7 @@ class Problem:
 
     def tape_end(self):
         self._check(self.lib.nnpp_tape_end(self.ctx))
+        self._tape_active = False
 
     def time_kernel(self, kernel, precision=FP64, reps=20):
         """Mean duration (ms) of one launch of a hot kernel, HIP events on the context stream."""

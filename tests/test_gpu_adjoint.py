@@ -175,3 +175,63 @@ def test_tape_guards():
         with pytest.raises(nn.NnppError) as e:
             pb.tape_adjoint(np.ones(case.icvalues.shape))
         assert e.value.code == nn._abi.ERR_STATE
+
+
+def test_autograd_stale_graph_raises():
+    """A graph whose tape was replaced by a later simulate(), extended by further steps or
+    ended gives an error in backward, never the gradient of another trajectory; a graph whose
+    tape is intact still differentiates."""
+    say("case autograd stale graph")
+    case, rp, col = small_case("regular", seed=5, thr=None)
+    S = case.params.S
+    with gpu_problem(case, rp, col) as pb:
+        start = pb.snapshot()
+        Dl0 = torch.tensor([case.params.D_liquid[k] for k in range(S)], dtype=torch.float64)
+        Ds0 = torch.tensor([case.params.D_solid[k] for k in range(S)], dtype=torch.float64)
+        rd0 = torch.tensor(case.params.r_diss, dtype=torch.float64)
+
+        def graph():
+            Dl = Dl0.clone().requires_grad_(True)
+            U = nn.autograd.simulate(pb, Dl, Ds0, rd0, 3, start, 1)
+            return Dl, (U ** 2).sum()
+
+        # (1) a second simulate() before backward replaces the first graph's tape
+        _, l1 = graph()
+        Dl2, l2 = graph()
+        with pytest.raises(RuntimeError, match="replaced, extended or ended"):
+            l1.backward()
+        l2.backward()  # the current graph is intact
+        assert torch.isfinite(Dl2.grad).all() and Dl2.grad.abs().max() > 0
+        # (2) FP64 steps appended to the tape after forward
+        _, l3 = graph()
+        pb.step(1, nn.FP64)
+        with pytest.raises(RuntimeError, match="replaced, extended or ended"):
+            l3.backward()
+        # (3) the tape ended
+        _, l4 = graph()
+        pb.tape_end()
+        with pytest.raises(RuntimeError, match="replaced, extended or ended"):
+            l4.backward()
+
+
+def test_adjoint_leaves_state_on_noconv():
+    """The adjoint restores the caller's state, phase and step scalars on every exit, also when
+    its potential solves stop at the iteration limit (status NOCONV)."""
+    say("case adjoint state restore")
+    case, rp, col = small_case("regular", seed=4)
+    for maxit in (100000, 2):
+        with gpu_problem(case, rp, col) as pb:
+            pb.set_solver(rtol=1e-13, maxit=maxit)  # frozen into the tape
+            pb.tape_begin(2)
+            st = pb.step(5, nn.FP64)
+            assert st["status"] == (nn._abi.OK if maxit > 2 else nn._abi.ERR_NOCONV)
+            U0, ph0, t0 = pb.get_state(), pb.get_phase(), pb.get_time()
+            lam = np.ones(case.icvalues.shape)
+            grad = np.zeros(2 * case.params.S + 1)
+            code = pb.lib.nnpp_tape_adjoint(pb.ctx, nn._abi.dptr(lam), nn._abi.dptr(grad), None,
+                                            None)
+            assert code == (nn._abi.OK if maxit > 2 else nn._abi.ERR_NOCONV)
+            assert np.array_equal(pb.get_state(), U0)
+            assert np.array_equal(pb.get_phase(), ph0)
+            assert pb.get_time() == t0
+            pb.tape_end()
