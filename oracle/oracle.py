@@ -268,6 +268,9 @@ class RefHarness:
                                             _ip, _ip]
         L.ref_residuals.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _ip,
                                     _dp, _dp, _dp, _dp, _ip, _ip, _d, _dp]
+        L.ref_residuals_timed.restype = _d
+        L.ref_residuals_timed.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                          _ip, _ip, _dp, _dp, _dp, _ip, _ip, _d, C.c_int, _dp]
         L.ref_build_clouds_implant.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
                                                _d, C.c_int, _ip]
         L.ref_newton_matrix.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip,
@@ -305,6 +308,22 @@ class RefHarness:
                                _i_(np.ascontiguousarray(phase, dtype=np.int32)),
                                _i_(case.bctypes), voln1rel, _d_(F))
         return F
+
+    def residuals_timed(self, case, ell, lengths, phase, U1, Un, reps=1, voln1rel=0.0):
+        """(F, seconds of the `reps` utils::Residuals calls alone)."""
+        p = case.params
+        assert p.S == 10
+        d1, d2, dc, dphi = self._param_blocks(p)
+        n = case.n
+        F = np.zeros((11, n))
+        t = self.lib.ref_residuals_timed(n, _d_(d1), _d_(d2), _d_(dc), _d_(dphi), _d_(case.x),
+                                         _d_(case.y), _d_(case.z), ell.shape[1],
+                                         _i_(np.ascontiguousarray(ell)), _i_(lengths),
+                                         _d_(case.bcvalues), _d_(np.ascontiguousarray(U1)),
+                                         _d_(np.ascontiguousarray(Un)),
+                                         _i_(np.ascontiguousarray(phase, dtype=np.int32)),
+                                         _i_(case.bctypes), voln1rel, int(reps), _d_(F))
+        return F, float(t)
 
     def build_clouds_implant(self, x, y, z, bx, by, bz, r, max_len=1024):
         lengths = np.zeros(x.size, dtype=np.int32)

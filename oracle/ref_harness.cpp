@@ -10,6 +10,8 @@
 #include <Eigen/Dense>
 #include <Eigen/Sparse>
 
+#include <omp.h>
+
 #include <cstdio>
 #include <stdexcept>
 #include <vector>
@@ -132,6 +134,48 @@ int ref_residuals(int n, const double* data1, const double* data2, const double*
   for (int f = 0; f < 11; f++)
     for (int i = 0; i < n; i++) Fout[(size_t)f * n + i] = F[f][i];
   return 0;
+}
+
+// utils::Residuals as above, called `reps` times on the same inputs; returns the wall seconds
+// of those calls alone (the conversions to Eigen containers are not timed). For the CPU
+// baseline of the benchmark: the reference's own bond loops on the host cores.
+double ref_residuals_timed(int n, const double* data1, const double* data2, const double* datac,
+                           const double* datac11, const double* x, const double* y,
+                           const double* z, int max_len, const int* ell, const int* lengths,
+                           const double* bcv, const double* U1, const double* Un,
+                           const int* phase, const int* bctypes, double voln1rel, int reps,
+                           double* Fout) {
+  utils::parameter p;
+  make_params(data1, data2, datac, datac11, p);
+  p.nodesNo = n;
+  Eigen::MatrixXd nodes;
+  make_nodes(n, x, y, z, nodes);
+  Eigen::MatrixXi h;
+  Eigen::VectorXi hl;
+  make_horizons(n, max_len, ell, lengths, h, hl);
+  Eigen::VectorXd B[11], A[11], N[11], O[11], F[11];
+  vecs(n, bcv, B);
+  vecs(n, U1, A);
+  vecs(n, Un, N);
+  vecs(n, Un, O);
+  for (int f = 0; f < 11; f++) F[f].resize(n);
+  Eigen::VectorXi ph(n), bt(n);
+  for (int i = 0; i < n; i++) {
+    ph[i] = phase[i];
+    bt[i] = bctypes[i];
+  }
+  const double t0 = omp_get_wtime();
+  for (int r = 0; r < reps; r++)
+    utils::Residuals(nodes, h, B[0], B[1], B[2], B[3], B[4], B[5], B[6], B[7], B[8], B[9],
+                     B[10], A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], A[8], A[9], A[10],
+                     N[0], N[1], N[2], N[3], N[4], N[5], N[6], N[7], N[8], N[9], N[10], O[0],
+                     O[1], O[2], O[3], O[4], O[5], O[6], O[7], O[8], O[9], O[10], F[0], F[1],
+                     F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], F[10], ph, hl, bt, voln1rel,
+                     p);
+  const double t = omp_get_wtime() - t0;
+  for (int f = 0; f < 11; f++)
+    for (int i = 0; i < n; i++) Fout[(size_t)f * n + i] = F[f][i];
+  return t;
 }
 
 // utils::NewtonMatrices; returns matrix K<which> (1..11) as coordinate triplets in Eigen's

@@ -75,7 +75,7 @@ def test_isolated_nodes_and_ragged_rows():
         ref, U, ph, st = run_both(case, 5, prec)
         assert st["status"] == 0
         assert np.array_equal(ph, ref["phase"])
-        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        tol = 1e-12 if prec == nn.FP64 else 1e-6
         for f in range(F):
             assert rel_err(U[f], ref["U"][f]) < tol, (prec, f)
 
@@ -87,14 +87,14 @@ def test_all_solid_and_all_dirichlet_potential():
     ref, U, ph, st = run_both(case, 3)
     assert st["status"] == 0 and np.all(ph == ref["phase"])
     for f in range(4):
-        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+        assert rel_err(U[f], ref["U"][f]) < 1e-12, f
     case = nn.synth.regular(6, S=3, seed=2)
     case.bctypes[:] = 0  # every node Dirichlet
     case.bcvalues[:, :] = np.where(case.bcvalues == -1, 1.0, case.bcvalues)
     ref, U, ph, st = run_both(case, 2)
     assert st["status"] == 0
     for f in range(4):
-        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+        assert rel_err(U[f], ref["U"][f]) < 1e-12, f
 
 
 @pytest.mark.parametrize("ng", [5, 7, 9, 16])
@@ -107,7 +107,7 @@ def test_small_lattices_tile_edges(ng):
         ref, U, ph, st = run_both(case, 4, prec)
         assert st["status"] == 0
         assert np.array_equal(ph, ref["phase"])
-        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        tol = 1e-12 if prec == nn.FP64 else 1e-6
         for f in range(4):
             assert rel_err(U[f], ref["U"][f]) < tol, (ng, prec, f)
 
@@ -154,7 +154,7 @@ def _exact_run(shape, stencil):
         assert st["flux_stencil"] == int(stencil) and st["pot_stencil"] == 3 * int(stencil), \
             (shape, prec)
         assert np.array_equal(ph, ref["phase"])
-        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        tol = 1e-12 if prec == nn.FP64 else 1e-6
         for f in range(case.params.S + 1):
             assert rel_err(U[f], ref["U"][f]) < tol, (shape, prec, f)
         if stencil:  # the table-driven stencil loop gives the same trajectory bit for bit
@@ -184,7 +184,7 @@ def test_stencil_table_loop_when_not_the_radius3_ball():
         assert st["status"] == 0
         assert st["pot_stencil"] == 1 and st["flux_stencil"] == 1, prec
         assert np.array_equal(ph, ref["phase"])
-        tol = 1e-10 if prec == nn.FP64 else 1e-6
+        tol = 1e-12 if prec == nn.FP64 else 1e-6
         for f in range(case.params.S + 1):
             assert rel_err(U[f], ref["U"][f]) < tol, (prec, f)
 

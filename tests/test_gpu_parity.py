@@ -1,7 +1,7 @@
 """GPU parity of libnnpp against the CPU oracle and the reference golden vectors.
 
 Tolerances: neighbour lists bit-exact; FP64 operators (residuals, potential operator) within
-1e-12 relative (max-norm per field); FP64 multi-step trajectories within 1e-10 relative with
+1e-12 relative (max-norm per field); FP64 multi-step trajectories within 1e-12 relative with
 both solves at rtol 1e-13; mixed precision within 1e-6 relative of FP64."""
 import os
 import sys
@@ -192,7 +192,7 @@ def test_step_fp64_matches_oracle():
     assert st["status"] == 0 and st["steps"] == 10
     assert np.array_equal(ph, ref["phase"])
     for f in range(12):
-        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+        assert rel_err(U[f], ref["U"][f]) < 1e-12, f
 
 
 def test_step_reference_chemistry_matches_oracle():
@@ -220,8 +220,31 @@ def test_step_reference_chemistry_matches_oracle():
         st = pb.step(5, nn.FP64)
         U = pb.get_state()
     assert st["status"] == 0
+    # This case is ill-conditioned on purpose (states over 7 decades, dt = 1 s against fast
+    # reactions): the oracle's own 5-step result moves by ~3e-13 (field max) when its initial
+    # state is perturbed by 1e-15 relative. The GPU (whose per-step operators agree to ~1e-15)
+    # must stay within 1e-12 after one step, and after five within 1e-12 or 20x that
+    # measured sensitivity, whichever is larger.
+    U0, ph0, sc0, v0 = O.init_state(case)
+    st0 = {"U": U0, "phase": ph0, "step": 1, "voln1rel": 0.0, "vol0rel": v0, "solid": sc0}
+    Up = U0.copy()
+    Up[:10] *= 1 + 1e-15 * np.random.default_rng(6).standard_normal(Up[:10].shape)
+    pert = O.run(case, rp, col, 5, rtol=1e-13, state=dict(st0, U=Up))["U"]
+    sens = max(rel_err(pert[f], ref["U"][f]) for f in range(11))
+    one = O.run(case, rp, col, 1, rtol=1e-13)["U"]
+    with problem(case) as pb:
+        pb.set_neighbors(rp, col)
+        pb.set_bc(case.bctypes, case.bcvalues)
+        pb.init_state(case.icvalues)
+        pb.set_solver(rtol=1e-13, maxit=100000)
+        pb.step(1, nn.FP64)
+        U1 = pb.get_state()
     for f in range(11):
-        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+        assert rel_err(U1[f], one[f]) < 1e-12, f
+    tol = max(1e-12, 20 * sens)
+    say("reference chemistry: sensitivity", sens, "tolerance", tol)
+    for f in range(11):
+        assert rel_err(U[f], ref["U"][f]) < tol, (f, rel_err(U[f], ref["U"][f]), sens)
 
 
 def test_front_update_and_volume_cadence():
@@ -242,7 +265,7 @@ def test_front_update_and_volume_cadence():
     assert np.array_equal(ph, ref["phase"])
     assert abs(st["voln1rel"] - ref["voln1rel"]) <= 1e-12 * abs(ref["voln1rel"])
     for f in range(4):
-        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+        assert rel_err(U[f], ref["U"][f]) < 1e-12, f
 
 
 def test_mixed_step_close_to_fp64():
@@ -415,6 +438,16 @@ def test_cpp_driver_end_to_end(tmp_path):
                                       nn._abi.dptr(U), nn._abi.iptr(ph)) == 0
     assert np.array_equal(ph, ref["phase"])
     for f in range(12):
-        assert rel_err(U[f], ref["U"][f]) < 1e-10, f
+        assert rel_err(U[f], ref["U"][f]) < 1e-12, f
     vol = open(job + "-01.volume").read().split()
     assert len(vol) == 2 and abs(float(vol[-1].rstrip(",")) - ref["voln1rel"]) <= 1e-12 * abs(ref["voln1rel"])
+    # .energy (main.cpp:962-998): sum of U_Mg^2 dx dy dz over the nodes with |x| or |y| below
+    # the largest coordinate of the node file (energyBound = nodes.maxCoeff(), main.cpp:242),
+    # written with the stream's default 6 significant digits
+    bound = max(case.x.max(), case.y.max(), case.z.max())
+    sel = (np.abs(case.x) < bound) | (np.abs(case.y) < bound)
+    assert not sel.all()  # the nodes at x = y = max are left out
+    p = case.params
+    e_hand = float(np.sum(U[0][sel] ** 2) * p.dx * p.dy * p.dz)
+    ene = open(job + "-01.energy").read().split()
+    assert len(ene) == 2 and abs(float(ene[-1]) - e_hand) <= 5e-6 * e_hand, (ene, e_hand)
