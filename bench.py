@@ -320,13 +320,13 @@ def summarize_rocprof(trace_db, fetch_db=None, write_db=None, copyf_db=None, cop
     print("\n".join(lines))
 
 
-def roofline_entry(t_ms, fmt_bytes, csr_bytes, stencil, cb, wb):
+def roofline_entry(t_ms, fmt_bytes, csr_bytes, stencil, cb, wb, zmarch=False):
     """Roofline of the dominant kernel (the FP32 potential SpMV): `achieved` = the bytes its
     stored operator form must move per launch (compulsory traffic: the form's per-row words and
     the vectors, each gathered value counted once) / the measured launch time. The survey's CSR
     figure (a 4-byte column per liquid bond) is kept beside it as `csr_model_GBps`: the
     stencil form never reads those bytes, so that ratio is not a bandwidth."""
-    kname = "k_m_spmv_st" if stencil else "k_m_spmv"
+    kname = "k_zm_m_spmv" if zmarch else ("k_m_spmv_st" if stencil else "k_m_spmv")
     traffic = measured_traffic(kname)
     ach = fmt_bytes / (t_ms * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -335,7 +335,9 @@ def roofline_entry(t_ms, fmt_bytes, csr_bytes, stencil, cb, wb):
            "bytes_per_launch": fmt_bytes,
            "csr_model_bytes_per_launch": csr_bytes,
            "csr_model_GBps": csr_bytes / (t_ms * 1e-3) / 1e9,
-           "operator_format": ({"stencil": True, "mask_bytes_per_row": 16} if stencil else
+           "operator_format": ({"stencil": True, "zmarch": True, "flag_bytes_per_row": 1}
+                               if zmarch else
+                               {"stencil": True, "mask_bytes_per_row": 16} if stencil else
                                {"stencil": False, "col_bytes": cb, "weight_bytes": wb})}
     if traffic:
         out["hbm_measured_GBps"] = traffic / (t_ms * 1e-3) / 1e9
@@ -520,7 +522,11 @@ def main():
     stencil = bool(st_["pot_stencil"])
     cb, wb32 = st_["pot_col_bytes"], st_["pot_weight_bytes"]
     wb64 = 1 if wb32 == 1 else (0 if stencil else 8)
-    if stencil:
+    zmarch = st_["pot_stencil"] == 4
+    if zmarch:  # no per-row words: a free flag per node
+        bytes32 = n * (1 + 3 * 4)
+        bytes64 = n * (1 + 3 * 8)
+    elif stencil:
         bytes32 = n * (16 + 3 * 4)
         bytes64 = n * (16 + 3 * 8)
     else:
@@ -585,7 +591,7 @@ def main():
                               "within_1e-6": bool(max_err <= 1e-6 and max_l2 <= 1e-6)},
         "neighbor_build_s": t_nb,
         "kernels": kern,
-        "roofline": roofline_entry(t_spmv32, bytes32, alg_bytes, stencil, cb, wb32),
+        "roofline": roofline_entry(t_spmv32, bytes32, alg_bytes, stencil, cb, wb32, zmarch),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -127,7 +127,8 @@ typedef struct nnpp_step_stats {
   int32_t pot_stencil;      /* the operator used the stencil form (16 bytes per row, no
                                per-entry stream; pot_col_bytes = 0): 1 with the table-driven
                                loop, 3 with the loop specialised for the radius-3 lattice
-                               ball; 0 for the column form */
+                               ball, 4 with the z-marching kernels of a full box lattice (no
+                               row masks: a free flag per node); 0 for the column form */
   int32_t flux_stencil;     /* the explicit flux used the lattice stencil form (FP64 in both
                                precisions) */
 } nnpp_step_stats;
@@ -320,6 +321,10 @@ int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol
 #define NNPP_FORMAT_NO_STENCIL 4
 #define NNPP_FORMAT_NO_BALL 8 /* stencil form: the table-driven SpMV loop even where the
                                  offsets are the radius-3 lattice ball (same results) */
+#define NNPP_FORMAT_NO_ZMARCH 16 /* lattice-ball stencil on a full box lattice: the tile SpMV
+                                    (one 256-row tile per LDS window) instead of the z-marching
+                                    one (column tiles walking z); FP64 rows identical, dot
+                                    products summed in another order */
 int nnpp_set_format(nnpp_ctx* ctx, int32_t flags);
 
 /* Solve the potential for the charge of slot N: phi^{n+1} on free nodes (liquid, with a

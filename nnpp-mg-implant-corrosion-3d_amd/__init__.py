@@ -5,14 +5,15 @@ library; importing this package loads it and fails if it is missing.
 """
 from . import _abi
 from ._abi import (CHEM_GENERIC, CHEM_REFERENCE, FORMAT_AUTO, FORMAT_NO_BALL,
-                   FORMAT_NO_DICTIONARY, FORMAT_NO_STENCIL, FORMAT_WIDE_COLUMNS, FP64, MIXED, SLOT_N, SLOT_N1,
+                   FORMAT_NO_ZMARCH, FORMAT_NO_DICTIONARY, FORMAT_NO_STENCIL, FORMAT_WIDE_COLUMNS, FP64, MIXED, SLOT_N, SLOT_N1,
                    NnppError, Params, StepStats)
 from .problem import Problem
 from . import synth, formats
 
 lib = _abi.load()
 
-__all__ = ["CHEM_GENERIC", "CHEM_REFERENCE", "FORMAT_AUTO", "FORMAT_NO_BALL", "FORMAT_NO_DICTIONARY",
+__all__ = ["CHEM_GENERIC", "CHEM_REFERENCE", "FORMAT_AUTO", "FORMAT_NO_BALL", "FORMAT_NO_ZMARCH",
+           "FORMAT_NO_DICTIONARY",
            "FORMAT_NO_STENCIL", "FORMAT_WIDE_COLUMNS", "FP64", "MIXED", "SLOT_N", "SLOT_N1",
            "NnppError", "Params", "StepStats", "Problem", "synth", "formats", "lib", "autograd"]
 

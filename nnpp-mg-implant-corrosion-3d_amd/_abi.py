@@ -20,6 +20,7 @@ SLOT_N, SLOT_N1 = 0, 1
 KERNEL_FLUX, KERNEL_POT_SPMV = 0, 1
 FORMAT_AUTO, FORMAT_WIDE_COLUMNS, FORMAT_NO_DICTIONARY, FORMAT_NO_STENCIL = 0, 1, 2, 4
 FORMAT_NO_BALL = 8
+FORMAT_NO_ZMARCH = 16
 
 # .data2 field names in file order (utils.h:2348-2398); pKH2O is the reference's pKH20
 DATA2_FIELDS = [

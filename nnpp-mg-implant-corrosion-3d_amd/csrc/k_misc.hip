@@ -586,6 +586,26 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_pot_st_fill(
   nnpp_block_partial<3>(v, part);
 }
 
+// Box-lattice check: row i = x + nx (y + ny z) must hold exactly the NNPP_BALL_M ball offsets
+// whose far end lies inside the nx x ny x nz box (bit t of bmask = entry t of nnpp_ball's
+// enumeration, the order of the offsets).
+__global__ void k_st_box(int32_t n, int nx, int ny, int nz, const uint4* __restrict__ bmask,
+                         int* __restrict__ fail) {
+  constexpr nnpp_ball<NNPP_BALL_M> B{};
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+  uint32_t m[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int t = 0; t < B.K; t++) {
+    const int xx = x + B.dx[t], yy = y + B.ldy[B.line[t]], zz = z + B.ldz[B.line[t]];
+    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz)
+      m[t >> 5] |= 1u << (t & 31);
+  }
+  const uint4 b = bmask[i];
+  if (b.x != m[0] || b.y != m[1] || b.z != m[2] || b.w != m[3]) atomicExch(fail, 1);
+}
+
 // Lane order of each 256-row tile of the stencil SpMV: rows whose mask holds every entry first,
 // then the other rows with entries, then rows without any (and rows past n), each group in
 // row order. Waves made of complete rows then run without per-entry mask tests.
@@ -746,7 +766,7 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
 
 void nnpp_free_stencil(nnpp_stencil& t) {
   void* ps[] = {t.toff, t.lc, t.seg_start, t.seg_len, t.seg_base, t.piece, t.tw64, t.tw32, t.tf64,
-                t.bmask, t.fperm, t.pmask, t.perm};
+                t.bmask, t.fperm, t.pmask, t.perm, t.zmfull};
   for (void* q : ps)
     if (q) hipFree(q);
   t = nnpp_stencil();
@@ -936,6 +956,33 @@ static int build_stencil(nnpp_ctx* c) {
   for (int t = 0; t < K; t++)
     if (tf[t] != 0.0) T.fnz[t >> 5] |= 1u << (t & 31);
   detect_ball(T, toff, lc, pw32, pw64);
+  if (T.ball == NNPP_BALL_M) {  // the full box, numbered x fastest?
+    constexpr nnpp_ball<NNPP_BALL_M> B{};
+    int64_t sy = -1, sz = -1;
+    for (int t = 0; t < B.K; t++) {
+      const int l = B.line[t];
+      if (B.ldz[l] == 0 && B.ldy[l] == 1 && B.dx[t] == 0) sy = toff[t];
+      if (B.ldz[l] == 1 && B.ldy[l] == 0 && B.dx[t] == 0) sz = toff[t];
+    }
+    if (sy > 2 * NNPP_BALL_M && sz > 0 && sz % sy == 0 && n % sz == 0) {
+      const int nx = (int)sy, ny = (int)(sz / sy), nz = (int)(n / sz);
+      int* bf = nullptr;
+      NNPP_HIP(c, hipMalloc((void**)&bf, sizeof(int)));
+      NNPP_HIP(c, hipMemsetAsync(bf, 0, sizeof(int), s));
+      k_st_box<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, nx, ny, nz, T.bmask, bf);
+      int hb = 1;
+      NNPP_HIP(c, hipGetLastError());
+      NNPP_HIP(c, hipMemcpyAsync(&hb, bf, sizeof(int), hipMemcpyDeviceToHost, s));
+      NNPP_HIP(c, hipStreamSynchronize(s));
+      hipFree(bf);
+      if (!hb && ny > 2 * NNPP_BALL_M && nz >= 1) {
+        T.box = 1;
+        T.nx = nx;
+        T.ny = ny;
+        T.nz = nz;
+      }
+    }
+  }
   T.state = 1;
   return NNPP_OK;
 }
@@ -956,6 +1003,10 @@ static int build_potential_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipGetLastError());
   k_st_perm<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, T.K, T.pmask, T.perm);
   NNPP_HIP(c, hipGetLastError());
+  if (T.box) {
+    const int zst = nnpp_zm_prepare(c);
+    if (zst) return zst;
+  }
   int st = nnpp_finalize(c, FIN_FRONT, g, 0.0);
   if (st) return st;
   NNPP_HIP(c, hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));

@@ -27,6 +27,7 @@
 #include <cmath>
 
 #include <cstring>
+#include <type_traits>
 
 #include "nnpp_internal.hpp"
 
@@ -408,6 +409,320 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
     *row_out = r0 + threadIdx.x;
     return lds[threadIdx.x];
   }
+}
+
+// ---------------------------------------------------------------- z-marching form -------
+// Box lattices (the radius-3 ball on the full nx x ny x nz box, numbered x fastest): a block
+// owns a ZM_TX x ZM_TY column tile of (x, y) and a chunk of lz output planes along z. It walks
+// the input planes z0-3 .. z0+lz+2; each plane's tile with a 3-node halo is staged into LDS
+// once, as (p_j, m_j) pairs with m_j = 1 on free nodes and 0 elsewhere (outside the box too).
+// A thread reads the 29-point disc dx^2 + dy^2 <= 9 around its column from LDS once per plane
+// and feeds the (up to 7) output rows of its column that are in flight, held in registers:
+// output z_o takes the bonds with dz = z_i - z_o from input plane z_i. Per row this is 29 LDS
+// reads per plane instead of one per bond.
+//
+// The entries of row i are the bonds to free nodes inside the box (the potential operator's
+// free-free entries; its rows are the free rows): an absent entry takes weight w * m_j = +0,
+// exactly the masked weight of the stencil form. FP64: every row is one fused multiply-add
+// chain over its bonds in ascending offset order (planes in z order, then dy, then dx), the
+// additions of the stencil and column forms, bit for bit. FP32: per row and plane one chain,
+// the plane sums Kahan-compensated (chunks of at most 28 entries).
+constexpr int ZM_TX = 32, ZM_TY = 8;  // a half-wave reads one x-row of 32 (no bank conflicts)
+constexpr int ZM_M = NNPP_BALL_M;
+constexpr int ZM_W = ZM_TX + 2 * ZM_M, ZM_H = ZM_TY + 2 * ZM_M;
+constexpr int ZM_PTS = ZM_W * ZM_H;
+constexpr int ZM_STAGE = (ZM_PTS + NNPP_BLOCK - 1) / NNPP_BLOCK;  // staged points per thread
+constexpr int ZM_SL = 2 * ZM_M + 1;                               // rows in flight
+
+struct ZmGeom {
+  int nx, ny, nz;
+  int tiles_x, tiles_y;
+  int lz, nzc;       // output planes per block, chunks along z
+  int64_t sz;        // nx * ny
+  int64_t items;     // tiles_x * tiles_y * nzc (the grid)
+};
+
+template <typename T>
+struct ZmPair {
+  T v, m;
+};
+
+template <typename T>
+struct ZmRow;  // per in-flight row: running sum (FP32: with its Kahan compensation)
+template <>
+struct ZmRow<double> {
+  double s;
+  __device__ __forceinline__ void reset() { s = 0.0; }
+  __device__ __forceinline__ void fold(double part) { s = part; }  // FP64: one chain
+  __device__ __forceinline__ double start() const { return s; }
+  __device__ __forceinline__ double result(double dd, double vi) const {
+    return __fma_rn(dd, vi, s);
+  }
+};
+template <>
+struct ZmRow<float> {
+  float s, c;
+  __device__ __forceinline__ void reset() { s = c = 0.0f; }
+  __device__ __forceinline__ float start() const { return 0.0f; }  // a chain per plane
+  __device__ __forceinline__ void fold(float part) {
+    const float y = part - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ float result(float dd, float vi) const {
+    return __fmaf_rn(dd, vi, s);
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T zm_weight(const nnpp_ball_args& b, int d2);
+template <>
+__device__ __forceinline__ float zm_weight<float>(const nnpp_ball_args& b, int d2) {
+  return b.w32[d2];
+}
+template <>
+__device__ __forceinline__ double zm_weight<double>(const nnpp_ball_args& b, int d2) {
+  return b.w64[d2];
+}
+
+// Walks the block's work item and hands every finished row to emit(i, y, p_i) (rows inside
+// the box only). All threads of the block must call it (barriers inside). full[tile * nz + z]:
+// every staged point of plane z of the tile is a free node (no weight needs its m_j).
+template <typename T, typename Emit>
+__device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& ball,
+                                        const uint8_t* __restrict__ pfree,
+                                        const uint8_t* __restrict__ full_tab,
+                                        const T* __restrict__ dd, const T* __restrict__ v,
+                                        Emit&& emit) {
+  __shared__ ZmPair<T> lds[2][ZM_PTS];
+  const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
+  const int zc = (int)(item % g.nzc);
+  const int64_t tile = item / g.nzc;
+  const int x0 = (int)(tile % g.tiles_x) * ZM_TX, y0 = (int)(tile / g.tiles_x) * ZM_TY;
+  const int z0 = zc * g.lz;
+  const int zend = min(z0 + g.lz, g.nz);
+  const int tx = threadIdx.x % ZM_TX, ty = threadIdx.x / ZM_TX;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool col_ok = x < g.nx && y < g.ny;
+  // this thread's staged points: in-plane offset (-1: outside the box or past the tile)
+  int32_t soff[ZM_STAGE];
+#pragma unroll
+  for (int k = 0; k < ZM_STAGE; k++) {
+    const int e = threadIdx.x + k * NNPP_BLOCK;
+    const int gx = x0 + e % ZM_W - ZM_M, gy = y0 + e / ZM_W - ZM_M;
+    soff[k] = (e < ZM_PTS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx : -1;
+  }
+  const uint8_t* __restrict__ ftab = full_tab + tile * g.nz;
+  const int zs = max(z0 - ZM_M, 0), ze = min(zend + ZM_M, g.nz);  // input planes [zs, ze)
+  // row (x, y, zo) is node zo * sz + col (node indices fit 32 bits: n is an int32_t)
+  const uint32_t col = col_ok ? (uint32_t)(y * g.nx + x) : 0u;
+  const uint32_t sz = (uint32_t)g.sz;
+  // slot q holds output plane zi - M + q while input plane zi is processed; rows outside the
+  // chunk are never summed (uniform tests) nor emitted
+  ZmRow<T> row[ZM_SL];
+  T po[ZM_SL];
+  const auto in_chunk = [&](int zo) { return zo >= z0 && zo < zend; };
+#pragma unroll
+  for (int q = 0; q < ZM_SL; q++) {
+    const int zo = zs - ZM_M + q;
+    row[q].reset();
+    po[q] = (col_ok && in_chunk(zo)) ? v[(uint32_t)zo * sz + col] : T(0);
+  }
+  // diagonal term and free flag of the row in slot 0 (the next to finish), loaded a plane ahead
+  T d0 = T(0);
+  bool f0 = false;
+  const auto load_first = [&](int zo) {
+    if (col_ok && in_chunk(zo)) {
+      d0 = dd[(uint32_t)zo * sz + col];
+      f0 = pfree[(uint32_t)zo * sz + col] != 0;
+    }
+  };
+  load_first(zs - ZM_M);
+  // registers of the next plane's staged points
+  T sv[ZM_STAGE];
+  uint8_t sm[ZM_STAGE];
+  const auto fetch = [&](int zi) {
+    const uint32_t base = (uint32_t)zi * sz;
+#pragma unroll
+    for (int k = 0; k < ZM_STAGE; k++) {
+      sv[k] = soff[k] >= 0 ? v[base + (uint32_t)soff[k]] : T(0);
+      sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
+    }
+  };
+  fetch(zs);
+  // after input plane zi: output plane zi - M is complete; the slots move down by one and the
+  // last one takes output plane zi + 1 + M
+  const auto finish = [&](int zi) {
+    const int zo = zi - ZM_M;
+    if (col_ok && in_chunk(zo)) {
+      const T y = f0 ? row[0].result(d0, po[0]) : ZmRow<T>{}.result(d0, po[0]);
+      emit((int64_t)((uint32_t)zo * sz + col), y, po[0]);
+    }
+#pragma unroll
+    for (int q = 0; q + 1 < ZM_SL; q++) {
+      row[q] = row[q + 1];
+      po[q] = po[q + 1];
+    }
+    const int zn = zi + 1 + ZM_M;
+    row[ZM_SL - 1].reset();
+    po[ZM_SL - 1] = (col_ok && in_chunk(zn)) ? v[(uint32_t)zn * sz + col] : T(0);
+    load_first(zo + 1);
+  };
+  int buf = 0;
+  for (int zi = zs; zi < ze; zi++) {
+#pragma unroll
+    for (int k = 0; k < ZM_STAGE; k++) {
+      const int e = threadIdx.x + k * NNPP_BLOCK;
+      if (e < ZM_PTS) {
+        ZmPair<T> pr;
+        pr.v = sv[k];
+        pr.m = sm[k] ? T(1) : T(0);
+        lds[buf][e] = pr;
+      }
+    }
+    const bool full = nnpp_ld_table(ftab, zi) != 0;
+    // this plane staged (and the buffer written two planes ago read by every thread)
+    __syncthreads();
+    if (zi + 1 < ze) fetch(zi + 1);
+    const ZmPair<T>* __restrict__ me = &lds[buf][(ty + ZM_M) * ZM_W + tx + ZM_M];
+    // slot q (output plane zi - M + q) takes the bonds with dz = M - q
+    bool act[ZM_SL];
+#pragma unroll
+    for (int q = 0; q < ZM_SL; q++) act[q] = in_chunk(zi - ZM_M + q);
+    // MASK: some staged point of the plane is not a free node (weights times m_j). Separate
+    // instances, so the common case (free interior planes) is one subtraction and one fused
+    // multiply-add per bond.
+    const auto body = [&](auto mask_c, auto all_c) {
+      constexpr bool MASK = decltype(mask_c)::value, ALL = decltype(all_c)::value;
+      T part[ZM_SL];
+#pragma unroll
+      for (int q = 0; q < ZM_SL; q++) part[q] = row[q].start();
+#pragma unroll
+      for (int dy = -ZM_M; dy <= ZM_M; dy++) {
+        ZmPair<T> lv[2 * ZM_M + 1];
+#pragma unroll
+        for (int dx = -ZM_M; dx <= ZM_M; dx++)
+          if (dx * dx + dy * dy <= ZM_M * ZM_M) lv[dx + ZM_M] = me[dy * ZM_W + dx];
+#pragma unroll
+        for (int q = 0; q < ZM_SL; q++) {
+          const int dz = ZM_M - q;
+          if (dz * dz + dy * dy > ZM_M * ZM_M) continue;
+          if (!ALL && !act[q]) continue;  // uniform over the block
+#pragma unroll
+          for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+            const int d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > ZM_M * ZM_M || d2 == 0) continue;
+            const T w = zm_weight<T>(ball, d2);
+            const ZmPair<T> pj = lv[dx + ZM_M];
+            if constexpr (sizeof(T) == 8)
+              part[q] = __fma_rn(MASK ? w * pj.m : w, po[q] - pj.v, part[q]);
+            else
+              part[q] = __fmaf_rn(MASK ? w * pj.m : w, po[q] - pj.v, part[q]);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < ZM_SL; q++)
+        if (ALL || act[q]) row[q].fold(part[q]);
+    };
+    using t_ = std::true_type;
+    using f_ = std::false_type;
+    // (an instance without the per-slot tests for the planes where every slot is active
+    // measured slower: its straight-line code spilled at four waves per SIMD)
+    if (full) body(f_{}, f_{});
+    else body(t_{}, f_{});
+    finish(zi);
+    buf ^= 1;
+  }
+  // planes past the top of the box add nothing: the rows still in flight are complete
+  for (int zi = ze; zi < zend + ZM_M; zi++) finish(zi);
+}
+
+// Per column tile and plane: 1 when every point the tile stages from that plane (tile plus
+// halo, all inside the box) is a free node; the z-marching kernels then skip the m_j factors.
+__global__ void k_zm_full(ZmGeom g, const uint8_t* __restrict__ pfree, uint8_t* __restrict__ full) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y;
+  if (k >= tiles * g.nz) return;
+  const int64_t tile = k / g.nz;
+  const int z = (int)(k % g.nz);
+  const int x0 = (int)(tile % g.tiles_x) * ZM_TX - ZM_M, y0 = (int)(tile / g.tiles_x) * ZM_TY - ZM_M;
+  uint8_t all = 1;
+  if (x0 < 0 || y0 < 0 || x0 + ZM_W > g.nx || y0 + ZM_H > g.ny) all = 0;
+  const uint8_t* __restrict__ f = pfree + (int64_t)z * g.sz;
+  for (int yy = 0; yy < ZM_H && all; yy++)
+    for (int xx = 0; xx < ZM_W; xx++)
+      if (!f[(int64_t)(y0 + yy) * g.nx + x0 + xx]) {
+        all = 0;
+        break;
+      }
+  full[k] = all;
+}
+
+// FP32 CG SpMV: q = M p, partial p.q (FP64 over FP32 products).
+__global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_spmv(ZmGeom g, nnpp_ball_args ball,
+                                                          const uint8_t* __restrict__ pfree,
+                                                          const uint8_t* __restrict__ full,
+                                                          const float* __restrict__ dd,
+                                                          const float* __restrict__ p,
+                                                          float* __restrict__ q,
+                                                          double* __restrict__ part,
+                                                          const double* __restrict__ scal) {
+  if (cg_done(scal)) return;
+  double v[1] = {0.0};
+  zm_walk<float>(g, ball, pfree, full, dd, p, [&](int64_t i, float y, float pi) {
+    q[i] = y;
+    v[0] += (double)pi * (double)y;
+  });
+  nnpp_block_partial<1>(v, part);
+}
+
+// FP64 CG SpMV: q = M p, partial p.q.
+__global__ __launch_bounds__(NNPP_BLOCK) void k_zm_cg_spmv(ZmGeom g, nnpp_ball_args ball,
+                                                           const uint8_t* __restrict__ pfree,
+                                                           const uint8_t* __restrict__ full,
+                                                           const double* __restrict__ dd,
+                                                           const double* __restrict__ p,
+                                                           double* __restrict__ q,
+                                                           double* __restrict__ part,
+                                                           const double* __restrict__ scal) {
+  if (cg_done(scal)) return;
+  double v[1] = {0.0};
+  zm_walk<double>(g, ball, pfree, full, dd, p, [&](int64_t i, double y, double pi) {
+    q[i] = y;
+    v[0] += pi * y;
+  });
+  nnpp_block_partial<1>(v, part);
+}
+
+// r = b - M x in FP64 (partials r.z, r.r); START: also z = D^-1 r, p = z.
+template <bool START>
+__global__ __launch_bounds__(NNPP_BLOCK) void k_zm_resid(ZmGeom g, nnpp_ball_args ball,
+                                                         const uint8_t* __restrict__ pfree,
+                                                         const uint8_t* __restrict__ full,
+                                                         const double* __restrict__ dd,
+                                                         const double* __restrict__ pinv,
+                                                         const double* __restrict__ b,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ r,
+                                                         double* __restrict__ zv,
+                                                         double* __restrict__ pv,
+                                                         double* __restrict__ part) {
+  double v[2] = {0.0, 0.0};
+  zm_walk<double>(g, ball, pfree, full, dd, x, [&](int64_t i, double mx, double) {
+    const double pi_ = pinv[i];
+    const double ri = (pi_ != 0.0) ? b[i] - mx : 0.0;
+    r[i] = ri;
+    v[1] += ri * ri;
+    if (START) {
+      const double zi = ri * pi_;
+      zv[i] = zi;
+      pv[i] = zi;
+      v[0] += ri * zi;
+    }
+  });
+  nnpp_block_partial<2>(v, part);
 }
 
 // b (free rows), warm start x = phi, ||b||^2.
@@ -872,7 +1187,30 @@ struct PotOp {
   const double* w64;
   const float* w32;
   int64_t g32, g64;  // grids of the FP32 / FP64 SpMV-type kernels
+  bool zm;           // z-marching kernels (box lattice)
+  ZmGeom zg;
 };
+
+// Work items of the z-marching kernels: column tiles times chunks of lz output planes, lz
+// chosen for about 1024 items (at most NNPP_RED_GRID: the reduction partials), at least 4.
+ZmGeom zm_geom(const nnpp_stencil& T) {
+  ZmGeom g;
+  g.nx = T.nx;
+  g.ny = T.ny;
+  g.nz = T.nz;
+  g.tiles_x = (T.nx + ZM_TX - 1) / ZM_TX;
+  g.tiles_y = (T.ny + ZM_TY - 1) / ZM_TY;
+  const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y;
+  int64_t lz = (T.nz * tiles + 1023) / 1024;
+  lz = std::max<int64_t>(lz, (T.nz * tiles + NNPP_RED_GRID - 1) / NNPP_RED_GRID);
+  lz = std::max<int64_t>(lz, 4);
+  lz = std::min<int64_t>(lz, T.nz);
+  g.lz = (int)lz;
+  g.nzc = (int)((T.nz + lz - 1) / lz);
+  g.sz = (int64_t)T.nx * T.ny;
+  g.items = tiles * g.nzc;
+  return g;
+}
 
 // Blocks of the stencil SpMV kernels that fit on the device at once (LDS bounds them): a grid of
 // exactly that many blocks walks the tiles in one round, where a larger grid would leave a
@@ -898,7 +1236,10 @@ PotOp pot_op(const nnpp_ctx* c) {
   o.sl = Slices{P.off, P.len, P.col, P.d16, P.first, P.cls, P.tab64, P.tab32,
                 T.pmask, T.lc, T.seg_start, T.seg_len, T.seg_base, T.piece, T.tw64, T.tw32,
                 T.perm, T.K, T.R, T.P, T.off_min, T.off_max, T.ball_args};
-  const bool ball = nnpp_pot_form(c) == NNPP_BALL_M;
+  const int form = nnpp_pot_form(c);
+  const bool ball = form == NNPP_BALL_M;
+  o.zm = form == NNPP_POT_ZMARCH;
+  if (o.zm) o.zg = zm_geom(T);
   if (T.active) {
     o.var = ball ? 5 : 4;
     o.lds32 = sizeof(float) * T.lds_elems;
@@ -912,7 +1253,9 @@ PotOp pot_op(const nnpp_ctx* c) {
   }
   const int64_t cap = nnpp_red_grid(nnpp_nblocks(c->n));
   o.g32 = o.g64 = cap;
-  if (T.active) {
+  if (o.zm) {
+    o.g32 = o.g64 = o.zg.items;
+  } else if (T.active) {
     o.g32 = ball ? resident_grid(k_m_spmv_st<NNPP_BALL_M>, o.lds32, c->device, cap)
                  : resident_grid(k_m_spmv_st<0>, o.lds32, c->device, cap);
     o.g64 = ball ? resident_grid(k_cg_spmv<false, false, true, NNPP_BALL_M>, o.lds64, c->device,
@@ -1006,8 +1349,18 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   int status = NNPP_OK;
   double rr = 0.0;
   // r = b - M x in FP64 (START: also z, p, rho, target, converged flag)
+  const nnpp_ball_args& ball = c->pst.ball_args;
   auto resid = [&](bool start) -> int {
-    if (start)
+    if (op.zm) {
+      if (start)
+        k_zm_resid<true><<<op.g64, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd, c->pinv,
+                                                       c->cg_b, c->cg_x, c->cg_r, c->cg_z,
+                                                       c->cg_p, part);
+      else
+        k_zm_resid<false><<<op.g64, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd, c->pinv,
+                                                        c->cg_b, c->cg_x, c->cg_r, nullptr,
+                                                        nullptr, part);
+    } else if (start)
       launch_resid<true>(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->pinv, c->cg_b, c->cg_x,
                          c->cg_r, c->cg_z, c->cg_p, part);
     else
@@ -1041,8 +1394,12 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       }
       const int batch = plan.next(rr_b, c->h_scal[SC_TARGET2], left, total);
       for (int it = 0; it < batch; it++, par ^= 1) {
-        launch_cg_spmv(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q,
-                       part_s, scal);
+        if (op.zm)
+          k_zm_cg_spmv<<<op.g64, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd, c->cg_p,
+                                                     c->cg_q, part_s, scal);
+        else
+          launch_cg_spmv(var, op.lds64, op.g64, s, n, sl, op.w64, c->pdd, c->cg_p, c->cg_q,
+                         part_s, scal);
         k_cg_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv, c->cg_p, c->cg_q, c->cg_x, c->cg_r,
                                              c->cg_z, part, part_s, op.g64, c->scal, par);
         k_cg_dir<<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_z, c->cg_p, part, g, c->scal, par);
@@ -1090,8 +1447,12 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         }
         const int batch = plan.next(rr_b, stop_b, left, total);
         for (int it = 0; it < batch; it++, par ^= 1) {
-          launch_m_spmv(var, op.lds32, op.g32, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q,
-                        part_s, scal);
+          if (op.zm)
+            k_zm_m_spmv<<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd32, c->f_p,
+                                                      c->f_q, part_s, scal);
+          else
+            launch_m_spmv(var, op.lds32, op.g32, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q,
+                          part_s, scal);
           k_m_update<<<g, NNPP_BLOCK, 0, s>>>(n, c->pinv32, c->f_p, c->f_q, c->f_e, c->f_r,
                                               c->f_z, part, part_s, op.g32, c->scal, par);
           k_m_dir<<<g, NNPP_BLOCK, 0, s>>>(n, c->f_z, c->f_p, part, g, c->scal, par,
@@ -1143,6 +1504,18 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
   return status;
 }
 
+int nnpp_zm_prepare(nnpp_ctx* c) {
+  nnpp_stencil& T = c->pst;
+  const ZmGeom g = zm_geom(T);
+  const int64_t m = (int64_t)g.tiles_x * g.tiles_y * g.nz;
+  if (!T.zmfull) NNPP_HIP(c, hipMalloc((void**)&T.zmfull, (size_t)(m ? m : 1)));
+  if (m)
+    k_zm_full<<<(m + NNPP_BLOCK - 1) / NNPP_BLOCK, NNPP_BLOCK, 0, c->stream>>>(g, c->pfree,
+                                                                              T.zmfull);
+  NNPP_HIP(c, hipGetLastError());
+  return NNPP_OK;
+}
+
 int nnpp_phi_guess_dev(nnpp_ctx* c, int hist, const double* phi, const double* prev,
                        const double* prev2, double* out) {
   if (!c->n) return NNPP_OK;
@@ -1160,7 +1533,14 @@ int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const PotOp op = pot_op(c);
   const Slices& sl = op.sl;
   const int var = op.var;
-  if (precision == NNPP_MIXED)
+  const nnpp_ball_args& ball = c->pst.ball_args;
+  if (op.zm && precision == NNPP_MIXED)
+    k_zm_m_spmv<<<op.g32, NNPP_BLOCK, 0, c->stream>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd32, c->f_p,
+                                                      c->f_q, c->red_part, (const double*)c->scal);
+  else if (op.zm)
+    k_zm_cg_spmv<<<op.g64, NNPP_BLOCK, 0, c->stream>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd, c->cg_p,
+                                                       c->cg_q, c->red_part, (const double*)c->scal);
+  else if (precision == NNPP_MIXED)
     launch_m_spmv(var, op.lds32, op.g32, c->stream, n, sl, op.w32, c->pdd32, c->f_p, c->f_q, c->red_part,
                   (const double*)c->scal);
   else

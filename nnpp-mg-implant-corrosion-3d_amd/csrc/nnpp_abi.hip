@@ -546,7 +546,7 @@ int nnpp_set_solver(nnpp_ctx* c, double rtol, int32_t maxit, double inner_rtol) 
 
 int nnpp_set_format(nnpp_ctx* c, int32_t flags) {
   if (!c || (flags & ~(NNPP_FORMAT_WIDE_COLUMNS | NNPP_FORMAT_NO_DICTIONARY |
-                       NNPP_FORMAT_NO_STENCIL | NNPP_FORMAT_NO_BALL)))
+                       NNPP_FORMAT_NO_STENCIL | NNPP_FORMAT_NO_BALL | NNPP_FORMAT_NO_ZMARCH)))
     return NNPP_ERR_ARG;
   if (flags != c->format) c->geom_dirty = c->pot_dirty = true;
   c->format = flags;

@@ -151,6 +151,15 @@ struct nnpp_stencil {
   int active = 0;              // the current potential operator uses the stencil form
   int ball = 0;  // the offsets, LDS positions and weights fit the NNPP_BALL_M ball (bit for bit)
   nnpp_ball_args ball_args = {};
+  // Box lattice (ball == NNPP_BALL_M only): the nodes are the full nx x ny x nz box numbered
+  // x fastest (row strides sy = nx, sz = nx ny, n = nx ny nz) and every row's bonds are exactly
+  // the ball offsets whose far end lies inside the box. The z-marching kernels need this: they
+  // address neighbours by (x, y, z) and treat positions outside the box as absent.
+  int box = 0;
+  int nx = 0, ny = 0, nz = 0;
+  uint8_t* zmfull = nullptr;  // [column tiles x nz] per tile and plane: every point the
+                              // z-marching SpMV stages there is a free node (rebuilt with the
+                              // potential operator)
 };
 
 // Differentiation tape (nnpp_tape_begin .. nnpp_tape_end).
@@ -544,6 +553,8 @@ int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flip
 int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo);
 int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision);
+// Tables of the z-marching potential SpMV for the current free flags (box lattices).
+int nnpp_zm_prepare(nnpp_ctx* c);
 int nnpp_finalize(nnpp_ctx* c, int op, int64_t nblk, double a0);
 int nnpp_build_dictionary(nnpp_ctx* c, nnpp_slices& sl);
 // One explicit step of nnpp_step (flux, potential, front, voln1rel); fills *rec if non-null.
@@ -559,8 +570,11 @@ int nnpp_stencil_ensure(nnpp_ctx* c);
 // current geometry.
 int nnpp_flux_stencil_active(nnpp_ctx* c);
 // The potential SpMV's form: 0 column slices, 1 stencil masks with the table-driven loop,
-// NNPP_BALL_M stencil masks with the lattice-ball loop.
+// NNPP_BALL_M stencil masks with the lattice-ball loop, NNPP_POT_ZMARCH the z-marching kernels
+// of a full box lattice.
+#define NNPP_POT_ZMARCH 4
 inline int nnpp_pot_form(const nnpp_ctx* c) {
   if (!c->pst.active) return 0;
-  return (c->pst.ball == NNPP_BALL_M && !(c->format & NNPP_FORMAT_NO_BALL)) ? NNPP_BALL_M : 1;
+  if (c->pst.ball != NNPP_BALL_M || (c->format & NNPP_FORMAT_NO_BALL)) return 1;
+  return (c->pst.box && !(c->format & NNPP_FORMAT_NO_ZMARCH)) ? NNPP_POT_ZMARCH : NNPP_BALL_M;
 }

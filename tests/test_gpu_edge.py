@@ -151,16 +151,24 @@ def _exact_run(shape, stencil):
         ref, U, ph, st = run_both(case, 4, prec)
         assert st["status"] == 0
         # the lattices at least 7 wide hold the whole radius-3 ball of offsets
-        assert st["flux_stencil"] == int(stencil) and st["pot_stencil"] == 3 * int(stencil), \
+        # (full boxes: the z-marching SpMV)
+        assert st["flux_stencil"] == int(stencil) and st["pot_stencil"] == 4 * int(stencil), \
             (shape, prec)
         assert np.array_equal(ph, ref["phase"])
         tol = 1e-12 if prec == nn.FP64 else 1e-6
         for f in range(case.params.S + 1):
             assert rel_err(U[f], ref["U"][f]) < tol, (shape, prec, f)
-        if stencil:  # the table-driven stencil loop gives the same trajectory bit for bit
+        if stencil:
+            # the tile SpMV of the lattice ball against the oracle, and the table-driven loop
+            # gives its trajectory bit for bit
+            _, U1, ph1, st1 = run_both(case, 4, prec, fmt=nn.FORMAT_NO_ZMARCH, oracle=False)
+            assert st1["pot_stencil"] == 3
+            assert np.array_equal(ph1, ref["phase"])
+            for f in range(case.params.S + 1):
+                assert rel_err(U1[f], ref["U"][f]) < tol, (shape, prec, f)
             _, U2, ph2, st2 = run_both(case, 4, prec, fmt=nn.FORMAT_NO_BALL, oracle=False)
             assert st2["pot_stencil"] == 1
-            assert np.array_equal(U2, U) and np.array_equal(ph2, ph), (shape, prec)
+            assert np.array_equal(U2, U1) and np.array_equal(ph2, ph1), (shape, prec)
 
 
 @pytest.mark.parametrize("shape", [(7, 7, 5), (7, 7, 7), (9, 9, 9), (16, 9, 7), (13, 13, 13)])

@@ -214,7 +214,7 @@ def test_jacobian_and_implicit_steps_stencil_form():
     v = rng.standard_normal(U.shape)
     ko, do = O.jacobian_apply(case, rp, col, ph, U, v, 0.3, want_diag=True)
     res = {}
-    for fmt in (nn.FORMAT_AUTO, nn.FORMAT_NO_BALL, nn.FORMAT_NO_STENCIL):
+    for fmt in (nn.FORMAT_AUTO, nn.FORMAT_NO_ZMARCH, nn.FORMAT_NO_BALL, nn.FORMAT_NO_STENCIL):
         with problem(case) as pb:
             pb.set_format(fmt)
             pb.set_neighbors(rp, col)
@@ -232,12 +232,17 @@ def test_jacobian_and_implicit_steps_stencil_form():
         pb.set_bc(case.bctypes, case.bcvalues)
         pb.init_state(case.icvalues)
         assert pb.step(1)["flux_stencil"] == 1
-    ka, da, Ua, pa = res[nn.FORMAT_AUTO]
+    ka, da, Ua, pa = res[nn.FORMAT_NO_ZMARCH]
     # the potential SpMV's table-driven stencil loop; the column forms
     for other in (nn.FORMAT_NO_BALL, nn.FORMAT_NO_STENCIL):
         kc, dc, Uc, pc = res[other]
         assert np.array_equal(ka, kc) and np.array_equal(da, dc), other
         assert np.array_equal(Ua, Uc) and np.array_equal(pa, pc), other
+    # the z-marching potential SpMV (dot products summed in another order)
+    kz, dz, Uz, pz = res[nn.FORMAT_AUTO]
+    assert np.array_equal(ka, kz) and np.array_equal(da, dz) and np.array_equal(pa, pz)
+    for f in range(case.params.S + 1):
+        assert rel_err(Uz[f], Ua[f]) < 1e-12, f
     for f in range(case.params.S + 1):
         assert rel_err(ka[f], ko[f]) < 1e-12, f
         assert rel_err(da[f], do[f]) < 1e-12, f

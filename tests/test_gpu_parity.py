@@ -299,7 +299,8 @@ def test_storage_formats_bit_identical(kind):
     rp, col, _ = oracle_nb(case)
     W, D, T = nn.FORMAT_WIDE_COLUMNS, nn.FORMAT_NO_DICTIONARY, nn.FORMAT_NO_STENCIL
     B = nn.FORMAT_NO_BALL  # the table-driven stencil loop instead of the lattice-ball one
-    formats = [nn.FORMAT_AUTO, W, D, W | D, T, T | W, T | D, T | W | D, B, B | T]
+    Z = nn.FORMAT_NO_ZMARCH  # the tile SpMV of the lattice ball instead of the z-marching one
+    formats = [nn.FORMAT_AUTO, Z, W | Z, D | Z, T, T | W, T | D, T | W | D, B, B | T]
     res = {}
     for prec in (nn.FP64, nn.MIXED):
         out = res[prec] = {}
@@ -313,13 +314,16 @@ def test_storage_formats_bit_identical(kind):
                 st = pb.step(4, prec)
                 assert st["status"] == 0
                 out[fmt] = (pb.get_state(), st)
-        U0, st0 = out[nn.FORMAT_AUTO]
+        U0, st0 = out[Z]  # tile form: the column forms' additions, dot products included
         Uc = out[T][0]  # column form
         assert st0["flipped"] > 0
         for fmt, (U, st) in out.items():
             stencil = kind == "regular" and not (fmt & T)
-            # 16^3 lattice with delta = 3 spacings: the offsets are the radius-3 ball
-            assert st["pot_stencil"] == (0 if not stencil else (1 if fmt & B else 3)), fmt
+            zm = stencil and not (fmt & (B | Z))
+            # 16^3 lattice with delta = 3 spacings: the offsets are the radius-3 ball on the
+            # full box, so the z-marching kernels run unless a flag turns them off
+            assert st["pot_stencil"] == (0 if not stencil else (1 if fmt & B else (3 if fmt & Z
+                                                                                    else 4))), fmt
             # the explicit flux takes the stencil form (FP64) wherever the operator does
             assert st["flux_stencil"] == int(stencil), fmt
             if stencil:
@@ -327,7 +331,13 @@ def test_storage_formats_bit_identical(kind):
             else:
                 assert st["pot_col_bytes"] == (4 if fmt & W else 2)
                 assert st["pot_weight_bytes"] == (4 if (fmt & D or kind == "screw") else 1)
-            if prec == nn.FP64 or not stencil:
+            if zm:
+                # the z-marching SpMV: FP64 rows as the other forms, dot products summed in
+                # another order; FP32 rows with their own chunks
+                for f in range(case.params.S + 1):
+                    assert rel_err(U[f], res[nn.FP64][Z][0][f]) < (1e-12 if prec == nn.FP64
+                                                                 else 1e-9), (prec, fmt, f)
+            elif prec == nn.FP64 or not stencil:
                 # FP64: every form performs the same additions; FP32 column forms too
                 assert np.array_equal(U, U0 if prec == nn.FP64 else Uc), (prec, fmt)
                 if prec == nn.FP64:
@@ -340,7 +350,7 @@ def test_storage_formats_bit_identical(kind):
                     assert rel_err(U[f], res[nn.FP64][nn.FORMAT_AUTO][0][f]) < 1e-9, (fmt, f)
     with problem(case) as pb:
         with pytest.raises(nn.NnppError):
-            pb.set_format(16)
+            pb.set_format(32)
 
 
 @pytest.mark.parametrize("kind", ["regular", "screw"])
@@ -366,7 +376,8 @@ def test_potential_operator_kernel(kind):
     T = nn.FORMAT_NO_STENCIL
     out = {}
     B = nn.FORMAT_NO_BALL
-    for fmt in (nn.FORMAT_AUTO, B, T, T | nn.FORMAT_NO_DICTIONARY, T | nn.FORMAT_WIDE_COLUMNS):
+    Z = nn.FORMAT_NO_ZMARCH
+    for fmt in (nn.FORMAT_AUTO, Z, B, T, T | nn.FORMAT_NO_DICTIONARY, T | nn.FORMAT_WIDE_COLUMNS):
         with problem(case) as pb:
             pb.set_format(fmt)
             pb.set_neighbors(rp, col)
@@ -374,16 +385,20 @@ def test_potential_operator_kernel(kind):
             pb.init_state(case.icvalues)
             out[fmt] = (pb.potential_operator_apply(x, nn.FP64),
                         pb.potential_operator_apply(x, nn.MIXED))
-    y64, y32 = out[nn.FORMAT_AUTO]
+    y64, y32 = out[Z]
     assert rel_err(y64, yo) < 1e-12
     assert np.abs(y32 - y64).max() <= 1e-6 * np.abs(y64).max()
     yc32 = out[T][1]
+    yz32 = out[nn.FORMAT_AUTO][1]
     for fmt, (a, b) in out.items():
         assert np.array_equal(a, y64), fmt  # FP64: the same additions in every form
-        # FP32: the column forms agree among themselves, the two stencil loops between them
-        assert np.array_equal(b, y32 if fmt in (nn.FORMAT_AUTO, B) else yc32), fmt
+        # FP32: the column forms agree among themselves, the two stencil tile loops between
+        # them; the z-marching rows (plane chunks) are their own
+        exp = {nn.FORMAT_AUTO: yz32 if kind == "regular" else yc32, Z: y32, B: y32}
+        assert np.array_equal(b, exp.get(fmt, yc32)), fmt
     # FP32 stencil rows (chunk sums, compensated across chunks) against FP32 column rows
     assert np.abs(y32 - yc32).max() <= 4e-7 * np.abs(y64).max()
+    assert np.abs(yz32 - yc32).max() <= 4e-7 * np.abs(y64).max()
 
 
 def test_deterministic_and_error_paths():
