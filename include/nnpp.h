@@ -230,6 +230,22 @@ int nnpp_implant_classify(nnpp_ctx* ctx, int64_t nb, const double* bx, const dou
                           const double* bz, double r, int32_t max_count, int32_t* counts,
                           double* ic_out);
 
+/* The implant classification of nnpp_implant_classify without any device allocation. No
+ * counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same arguments, results
+ * and errors; its scratch (the points, their cell grid, the counts) is a static device array
+ * of NNPP_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the process
+ * ends, so nb <= NNPP_NOALLOC_MAX_POINTS, the node count <= NNPP_NOALLOC_MAX_NODES
+ * (NNPP_ERR_ARG otherwise), and the cell grid is coarsened to at most NNPP_NOALLOC_MAX_CELLS
+ * cells (the counts do not change). Every call on one GPU uses the same array: calls that
+ * overlap on one GPU (other threads, other contexts of the same device) race. */
+#define NNPP_NOALLOC_MAX_POINTS (1 << 19)
+#define NNPP_NOALLOC_MAX_NODES (1 << 22)
+#define NNPP_NOALLOC_MAX_CELLS (1 << 18)
+int nnpp_implant_classify_noalloc(nnpp_ctx* ctx, int64_t nb, const double* bx, const double* by,
+                                  const double* bz, double r, int32_t max_count,
+                                  int32_t* counts, double* ic_out);
+
 /* Voronoi-approximate node volumes for irregular clouds (the reference multiplies every bond
  * by dx*dy*dz, utils.h:122-124; pass the result to nnpp_set_nodes as per-node volumes).
  * Probes on a lattice of spacing h fill the nodes' bounding box widened by margin on every

@@ -173,6 +173,7 @@ SIGNATURES = {
     "nnpp_tape_end": (C.c_int, [_p]),
     "nnpp_abi_adjoint_stats_size": (C.c_int64, []),
     "nnpp_implant_classify": (C.c_int, [_p, _i64, _dp, _dp, _dp, _d, _i32, _ip, _dp]),
+    "nnpp_implant_classify_noalloc": (C.c_int, [_p, _i64, _dp, _dp, _dp, _d, _i32, _ip, _dp]),
     "nnpp_voronoi_volumes": (C.c_int, [_p, _d, _d, _d, _dp]),
     "nnpp_newton_defaults": (C.c_int, [C.POINTER(NewtonOpts)]),
     "nnpp_step_implicit": (C.c_int, [_p, _i32, _i32, _i32, C.POINTER(NewtonOpts),

@@ -624,3 +624,180 @@ int nnpp_voronoi_dev(nnpp_ctx* c, double hp, double margin, double radius,
   if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
   return NNPP_OK;
 }
+
+// ---- implant classification without device allocation ----
+// Scratch of nnpp_implant_classify_noalloc: a file-scope device array, one copy per GPU. Host
+// code reaches it only through the address hipGetSymbolAddress returns on each device (kept
+// in the context); the array's name in host code is not a device address.
+namespace {
+constexpr int64_t NA_P = NNPP_NOALLOC_MAX_POINTS, NA_C = NNPP_NOALLOC_MAX_CELLS,
+                  NA_N = NNPP_NOALLOC_MAX_NODES;
+constexpr size_t na_round(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NA_OFF_PTS = 0;                                        // 3 NA_P doubles
+constexpr size_t NA_OFF_SORT = NA_OFF_PTS + na_round(3 * NA_P * 8);     // 3 NA_P doubles
+constexpr size_t NA_OFF_CELL = NA_OFF_SORT + na_round(3 * NA_P * 8);    // NA_P int32
+constexpr size_t NA_OFF_CNT = NA_OFF_CELL + na_round(NA_P * 4);         // NA_C int32
+constexpr size_t NA_OFF_CUR = NA_OFF_CNT + na_round(NA_C * 4);          // NA_C int32
+constexpr size_t NA_OFF_START = NA_OFF_CUR + na_round(NA_C * 4);        // NA_C + 1 int64
+constexpr size_t NA_OFF_COUNTS = NA_OFF_START + na_round((NA_C + 1) * 8);  // NA_N int32
+constexpr size_t NA_OFF_FLAG = NA_OFF_COUNTS + na_round(NA_N * 4);      // 1 int32
+constexpr size_t NA_BYTES = NA_OFF_FLAG + 256;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+
+__global__ void k_na_fill(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Exclusive scan of n <= NA_C counts into start[0..n] by one block of 1024 threads: each
+// thread sums a contiguous chunk, the chunk sums are scanned in LDS, then each thread writes
+// its chunk's prefixes.
+__global__ __launch_bounds__(1024) void k_na_scan(const int32_t* __restrict__ in, int64_t n,
+                                                  int64_t* __restrict__ start) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t a = t * per, b = a + per < n ? a + per : n;
+  int64_t sum = 0;
+  for (int64_t k = a; k < b; k++) sum += in[k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan of the chunk sums
+    const int64_t v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = t ? part[t - 1] : 0;
+  for (int64_t k = a; k < b; k++) {
+    start[k] = run;
+    run += in[k];
+  }
+  if (t == 1023) start[n] = part[1023];
+}
+
+}  // namespace
+
+void nnpp_implant_icvalues_host(const nnpp_ctx* c, const int32_t* counts, double* ic_out) {
+  const int32_t n = c->n;
+  const double CS = c->p.CSolid;
+  for (int32_t i = 0; i < n; i++) {  // main.cpp:422-457
+    double v[11] = {0.0};
+    if (counts[i] == 0) {
+      v[1] = 1e-4 / CS;
+      v[2] = 1e-4 / CS;
+      v[3] = c->p.C0HCO3 / CS;
+      v[5] = c->p.C0HPO4 / CS;
+      v[7] = c->p.C0Ca / CS;
+      v[8] = c->p.C0Na / CS;
+      v[9] = c->p.C0Cl / CS;
+    } else {
+      v[0] = CS / CS;
+    }
+    for (int f = 0; f < 11; f++) ic_out[(int64_t)f * n + i] = v[f];
+  }
+}
+
+int nnpp_noalloc_setup(nnpp_ctx* c) {
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+    return nnpp_set_error(c, NNPP_ERR_HIP, "scratch symbol address");
+  c->noalloc_scratch = p;
+  return NNPP_OK;
+}
+
+int nnpp_implant_classify_noalloc(nnpp_ctx* c, int64_t nb, const double* bx, const double* by,
+                                  const double* bz, double r, int32_t max_count,
+                                  int32_t* counts, double* ic_out) {
+  if (!c || nb < 0 || (nb && (!bx || !by || !bz)) || (!counts && c->n) || !(r >= 0) ||
+      max_count < 0)
+    return NNPP_ERR_ARG;
+  if (!c->have_params) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_params not called");
+  if (!c->x) return nnpp_set_error(c, NNPP_ERR_STATE, "nnpp_set_nodes not called");
+  if (ic_out && c->p.n_species != 10)
+    return nnpp_set_error(c, NNPP_ERR_UNSUPPORTED,
+                          "implant initial values need the 10 reference species");
+  if (nb > NA_P)
+    return nnpp_set_error(c, NNPP_ERR_ARG,
+                          "nnpp_implant_classify_noalloc: more implant points than "
+                          "NNPP_NOALLOC_MAX_POINTS");
+  if ((int64_t)c->n > NA_N)
+    return nnpp_set_error(c, NNPP_ERR_ARG,
+                          "nnpp_implant_classify_noalloc: more nodes than NNPP_NOALLOC_MAX_NODES");
+  if (!c->noalloc_scratch) return nnpp_set_error(c, NNPP_ERR_STATE, "no scratch address");
+  NNPP_HIP(c, hipSetDevice(c->device));
+  const int32_t n = c->n;
+  if (n == 0) return NNPP_OK;
+  hipStream_t s = c->stream;
+  int over_h = 0;
+  if (nb == 0) {
+    for (int32_t i = 0; i < n; i++) counts[i] = 0;
+  } else {
+    double bbox[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t k = 0; k < nb; k++) {
+      const double v[3] = {bx[k], by[k], bz[k]};
+      for (int a = 0; a < 3; a++) {
+        if (k == 0 || v[a] < bbox[2 * a]) bbox[2 * a] = v[a];
+        if (k == 0 || v[a] > bbox[2 * a + 1]) bbox[2 * a + 1] = v[a];
+      }
+    }
+    for (int a = 0; a < 6; a++)
+      if (!std::isfinite(bbox[a]))
+        return nnpp_set_error(c, NNPP_ERR_ARG, "non-finite implant coordinates");
+    // cells at least 1.001 r wide (as nnpp_implant_classify), widened to fit the scratch grid
+    IGrid g;
+    g.x0 = bbox[0];
+    g.y0 = bbox[2];
+    g.z0 = bbox[4];
+    g.h = r * 1.001 > 0 ? r * 1.001 : 1.0;
+    if (g.h < 1e-300) g.h = 1e-300;
+    for (;;) {
+      g.mx = (int64_t)std::floor((bbox[1] - bbox[0]) / g.h) + 1;
+      g.my = (int64_t)std::floor((bbox[3] - bbox[2]) / g.h) + 1;
+      g.mz = (int64_t)std::floor((bbox[5] - bbox[4]) / g.h) + 1;
+      if ((double)g.mx * (double)g.my * (double)g.mz <= (double)NA_C) break;
+      g.h *= 2.0;
+    }
+    const int64_t ncell = g.mx * g.my * g.mz;
+    unsigned char* base = static_cast<unsigned char*>(c->noalloc_scratch);
+    double* pts = reinterpret_cast<double*>(base + NA_OFF_PTS);
+    double* srt = reinterpret_cast<double*>(base + NA_OFF_SORT);
+    int32_t* cell = reinterpret_cast<int32_t*>(base + NA_OFF_CELL);
+    int32_t* cnt = reinterpret_cast<int32_t*>(base + NA_OFF_CNT);
+    int32_t* cur = reinterpret_cast<int32_t*>(base + NA_OFF_CUR);
+    int64_t* start = reinterpret_cast<int64_t*>(base + NA_OFF_START);
+    int32_t* dcounts = reinterpret_cast<int32_t*>(base + NA_OFF_COUNTS);
+    int32_t* over = reinterpret_cast<int32_t*>(base + NA_OFF_FLAG);
+    // the array keeps the last call's contents: clear what must start at zero (fill kernels,
+    // which graph capture records as kernel nodes)
+    k_na_fill<<<(ncell + NNPP_BLOCK - 1) / NNPP_BLOCK, NNPP_BLOCK, 0, s>>>(cnt, ncell, 0);
+    k_na_fill<<<(ncell + NNPP_BLOCK - 1) / NNPP_BLOCK, NNPP_BLOCK, 0, s>>>(cur, ncell, 0);
+    k_na_fill<<<1, 64, 0, s>>>(over, 1, 0);
+    NNPP_HIP(c, hipGetLastError());
+    NNPP_HIP(c, hipMemcpyAsync(pts, bx, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    NNPP_HIP(c, hipMemcpyAsync(pts + NA_P, by, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    NNPP_HIP(c, hipMemcpyAsync(pts + 2 * NA_P, bz, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+    const int64_t gb = (nb + NNPP_BLOCK - 1) / NNPP_BLOCK;
+    k_imp_cell<<<gb, NNPP_BLOCK, 0, s>>>(nb, pts, pts + NA_P, pts + 2 * NA_P, g, cell, cnt);
+    k_na_scan<<<1, 1024, 0, s>>>(cnt, ncell, start);
+    k_imp_scatter<<<gb, NNPP_BLOCK, 0, s>>>(nb, pts, pts + NA_P, pts + 2 * NA_P, cell, start, cur,
+                                            srt, srt + NA_P, srt + 2 * NA_P, nullptr);
+    k_imp_count<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, c->x, c->y, c->z, g, start, srt,
+                                                       srt + NA_P, srt + 2 * NA_P, r, max_count,
+                                                       dcounts, over);
+    NNPP_HIP(c, hipGetLastError());
+    NNPP_HIP(c, hipMemcpyAsync(counts, dcounts, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    NNPP_HIP(c, hipMemcpyAsync(&over_h, over, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NNPP_HIP(c, hipStreamSynchronize(s));
+  }
+  if (ic_out) nnpp_implant_icvalues_host(c, counts, ic_out);
+  if (over_h)
+    return nnpp_set_error(c, NNPP_ERR_OVERFLOW,
+                          "an implant count exceeds max_count (kd_tree_interface.h:214-217)");
+  return NNPP_OK;
+}

@@ -183,7 +183,8 @@ int nnpp_create(nnpp_ctx** out, int32_t device) {
       hipMalloc((void**)&c->scal, sizeof(double) * (SC_COUNT + NNPP_MAX_SPECIES)) != hipSuccess ||
       hipHostMalloc((void**)&c->h_scal, sizeof(double) * (SC_COUNT + NNPP_MAX_SPECIES)) !=
           hipSuccess ||
-      hipMemset(c->scal, 0, sizeof(double) * (SC_COUNT + NNPP_MAX_SPECIES)) != hipSuccess) {
+      hipMemset(c->scal, 0, sizeof(double) * (SC_COUNT + NNPP_MAX_SPECIES)) != hipSuccess ||
+      nnpp_noalloc_setup(c) != NNPP_OK) {
     nnpp_destroy(c);
     return NNPP_ERR_HIP;
   }
@@ -730,26 +731,10 @@ int nnpp_implant_classify(nnpp_ctx* c, int64_t nb, const double* bx, const doubl
   if (st && st != NNPP_ERR_OVERFLOW) return done(st);
   if (hipMemcpy(counts, dc, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
     return done(nnpp_set_error(c, NNPP_ERR_HIP, "count download"));
-  if (ic_out) {  // main.cpp:422-457
-    const double CS = c->p.CSolid;
-    for (int32_t i = 0; i < n; i++) {
-      double v[11] = {0.0};
-      if (counts[i] == 0) {
-        v[1] = 1e-4 / CS;
-        v[2] = 1e-4 / CS;
-        v[3] = c->p.C0HCO3 / CS;
-        v[5] = c->p.C0HPO4 / CS;
-        v[7] = c->p.C0Ca / CS;
-        v[8] = c->p.C0Na / CS;
-        v[9] = c->p.C0Cl / CS;
-      } else {
-        v[0] = CS / CS;
-      }
-      for (int f = 0; f < 11; f++) ic_out[(int64_t)f * n + i] = v[f];
-    }
-  }
+  if (ic_out) nnpp_implant_icvalues_host(c, counts, ic_out);
   return done(st);
 }
+
 
 int nnpp_voronoi_volumes(nnpp_ctx* c, double h, double margin, double radius,
                          double* vol_out) {

@@ -281,6 +281,10 @@ struct nnpp_ctx {
   int64_t solid_count = 0;
 
   nnpp_tape tape;
+
+  // device address of the static scratch array of nnpp_implant_classify_noalloc on this
+  // context's device (hipGetSymbolAddress at nnpp_create)
+  void* noalloc_scratch = nullptr;
 };
 
 // Device scalars used by the CG kernels and reductions.
@@ -555,6 +559,10 @@ int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision);
 // Tables of the z-marching potential SpMV for the current free flags (box lattices).
 int nnpp_zm_prepare(nnpp_ctx* c);
+// Address of the no-allocation scratch array on the context's device (current device).
+int nnpp_noalloc_setup(nnpp_ctx* c);
+// Initial values of main.cpp:422-457 from implant counts (host, 10 reference species + phi).
+void nnpp_implant_icvalues_host(const nnpp_ctx* c, const int32_t* counts, double* ic_out);
 int nnpp_finalize(nnpp_ctx* c, int op, int64_t nblk, double a0);
 int nnpp_build_dictionary(nnpp_ctx* c, nnpp_slices& sl);
 // One explicit step of nnpp_step (flux, potential, front, voln1rel); fills *rec if non-null.
