@@ -533,10 +533,11 @@ def main():
         bytes32 = nll * (cb + wb32) + n * (4 + 3 * 4)
         bytes64 = nll * (cb + wb64) + n * (4 + 3 * 8)
     S = case.S
-    # flux bytes per launch: column form, per bond a column and a weight; stencil form (lattices,
-    # FP64 in both precisions), a 16-byte bond mask per row; both: the fields read once, phase,
-    # Dirichlet mask and the updated fields
-    fstencil = bool(st_["flux_stencil"])
+    # flux bytes per launch: column form, per bond a column and a weight; stencil form (lattices;
+    # FP32 bond sums in the mixed run), a 16-byte bond mask per row; both: the fields read once,
+    # phase, Dirichlet mask and the updated fields
+    fform = st_["flux_stencil"]
+    fstencil = bool(fform)
     fbytes = lambda wb: ((n * 16 if fstencil else nnz * (4 + wb))  # noqa: E731
                          + n * (8 * (S + 1) + 1 + 2 + 8 * S))
     kern = {
@@ -546,7 +547,9 @@ def main():
                           "format_GBps": bytes64 / (t_spmv64 * 1e-3) / 1e9},
         "flux_fp64": {"ms": t_flux64, "form": "stencil" if fstencil else "columns",
                       "GBps": fbytes(8) / (t_flux64 * 1e-3) / 1e9},
-        "flux_mixed": {"ms": t_flux32, "form": "stencil (fp64)" if fstencil else "columns (fp32)",
+        "flux_mixed": {"ms": t_flux32,
+                       "form": {2: "stencil (fp32 bond sums)", 1: "stencil (fp64)"}.get(
+                           fform, "columns (fp32)"),
                        "GBps": fbytes(4) / (t_flux32 * 1e-3) / 1e9},
     }
     for r_ in res.values():
@@ -568,8 +571,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 CG inner iterations (Kahan bond sums); f64 state, dots, residual checks; "
-                  "flux " + ("f64 (lattice stencil form)" if fstencil else "f32 bond sums (Kahan)")),
+        "dtype": ("f32 CG inner iterations; f32 flux bond sums "
+                  + ("(FP32 differences per bond, partial sums per ball plane added in f64)"
+                     if fform == 2 else "(f64 stencil form)" if fform == 1 else "(Kahan)")
+                  + "; f64 state, dots, residual checks, reactions and update"),
         "data": "synthetic (seeded lattice case in the reference's input layout)",
         "config": {"workload": (f"regular {ng}^3 lattice, 11 species + phi, delta=3dx, Mg sphere "
                                 f"r=0.3, Dirichlet faces, CG rtol {a.rtol:g}" if a.case == "regular"

@@ -55,9 +55,12 @@ enum {
 /* Arithmetic of the step. */
 enum {
   NNPP_FP64 = 0,             /* every operation in FP64 */
-  NNPP_MIXED = 1             /* FP32 bond sums with compensated per-node accumulation, FP32
-                                differences taken from FP64 state; FP64 CG dots, residual
-                                checks and state update */
+  NNPP_MIXED = 1             /* FP32 CG iterations (FP32 operator and vectors) with FP64
+                                dots, residual replacement and solution; FP32 flux bond sums
+                                (column form: FP32 differences from the FP64 state, Kahan sums
+                                over groups of 16 bonds; lattice stencil form: FP32 sums of
+                                differences from each tile's first row, per 16 bonds, added in
+                                FP64); FP64 reactions and state update */
 };
 
 /* Lower clamp applied to a species after each update (main.cpp:805-837). */
@@ -129,8 +132,9 @@ typedef struct nnpp_step_stats {
                                loop, 3 with the loop specialised for the radius-3 lattice
                                ball, 4 with the z-marching kernels of a full box lattice (no
                                row masks: a free flag per node); 0 for the column form */
-  int32_t flux_stencil;     /* the explicit flux used the lattice stencil form (FP64 in both
-                               precisions) */
+  int32_t flux_stencil;     /* the explicit flux used the lattice stencil form: 1 with FP64
+                               bond sums, 2 with the FP32 bond sums of NNPP_MIXED; 0 for the
+                               column form */
 } nnpp_step_stats;
 
 typedef struct nnpp_ctx nnpp_ctx;

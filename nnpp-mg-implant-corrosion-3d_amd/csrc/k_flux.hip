@@ -745,6 +745,406 @@ hipError_t launch_st(const FluxArgs& a, const FluxSt& st, size_t lds, hipStream_
   return hipGetLastError();
 }
 
+// ------------------------------------------------------- stencil form, mixed precision -------
+// The explicit update of the stencil form with FP32 bond sums. The window of each species is
+// staged as FP32 pairs (c^_j, phi^_j c^_j), where phi^ = phi - phi_r is the difference from the
+// tile's first row and c^_j = c_j - c_r(p_j) the difference from the window's first node of
+// the same phase p_j (both taken in FP64, then rounded). Per pass (same-phase bonds m0,
+// solid-liquid bonds m1; c_r = c_r(p_i), c_r' = c_r(p_j) of the pass):
+//   sd = sum w (c_j - c_i)          = D + (c_r' - c_r) W
+//   sm = sum w (phi_j - phi_i) c_j  = (S2 - phi^_i (D + c^_i W)) + c_r' P
+// with D = sum w (c^_j - c^_i) and S2 = sum w phi^_j c^_j in FP32 (one 8-byte LDS read, one
+// subtraction and two FMAs per bond; FP32 partial sums per plane of the lattice ball or per 16
+// table entries, accumulated in FP64), and W = sum w and P = sum w (phi_j - phi_i) in FP64 from
+// the FP64 phi window, once per row. w is the flux weight rounded to FP32 in every sum. The
+// diffusion sum adds differences, as the column form's does, so its rounding follows the
+// field's local variation (a sum of staged values would round with the variation over the
+// whole window, which over 1000 steps at 128^3 moved a front flip by a step); the jumps between
+// the phases go into the FP64 terms. Reactions, the update, the Dirichlet rows and the clamp
+// stay FP64.
+
+// W += sum w, P += sum w (phi_j - phi_i) over the bonds of mask m (FP64, phi window in LDS).
+__device__ __forceinline__ void st_wp(const FluxSt& st, const float* tf32, const double* rphi,
+                                      const uint32_t (&m)[4], uint32_t anyc, double phii,
+                                      double& W, double& P) {
+  constexpr int BT = 16;
+  const int K = st.K;
+  for (int c = 0; c * BT < K; c++) {
+    if (!((anyc >> c) & 1u)) continue;
+    const int t0 = c * BT;
+    const int wd = t0 >> 5;
+    const uint32_t bits =
+        (wd == 0 ? m[0] : (wd == 1 ? m[1] : (wd == 2 ? m[2] : m[3]))) >> (t0 & 31);
+    int lv[BT];
+    float wv[BT];
+    double v[BT];
+#pragma unroll
+    for (int k = 0; k < BT; k++) {
+      lv[k] = nnpp_ld_table(st.lc, t0 + k);
+      wv[k] = nnpp_ld_table(tf32, t0 + k);
+    }
+#pragma unroll
+    for (int k = 0; k < BT; k++) v[k] = rphi[lv[k]];
+#pragma unroll
+    for (int k = 0; k < BT; k++) {
+      const double w = (double)nnpp_mask_weight<float>(wv[k], (int)(bits << (31 - k)) >> 31);
+      W += w;
+      P = __fma_rn(w, v[k] - phii, P);
+    }
+  }
+}
+
+// S1 += sum w (c^_j - chi), S2 += sum w q_j over the bonds of mask m from the staged pairs
+// (c^_j, q_j): FP32 sums over each chunk of 16 entries, added in FP64.
+__device__ __forceinline__ void st_sums32(const FluxSt& st, const float* tf32, const float2* rcq,
+                                          const uint32_t (&m)[4], uint32_t anyc, uint32_t fullc,
+                                          float chi, double& S1, double& S2) {
+  constexpr int BT = 16;
+  const int K = st.K;
+  for (int c = 0; c * BT < K; c++) {
+    if (!((anyc >> c) & 1u)) continue;
+    const int t0 = c * BT;
+    int lv[BT];
+    float wv[BT];
+    float2 v[BT];
+#pragma unroll
+    for (int k = 0; k < BT; k++) {
+      lv[k] = nnpp_ld_table(st.lc, t0 + k);
+      wv[k] = nnpp_ld_table(tf32, t0 + k);
+    }
+#pragma unroll
+    for (int k = 0; k < BT; k++) v[k] = rcq[lv[k]];
+    float a1 = 0.0f, a2 = 0.0f;
+    if ((fullc >> c) & 1u) {
+#pragma unroll
+      for (int k = 0; k < BT; k++) {
+        a1 = __builtin_fmaf(wv[k], v[k].x - chi, a1);
+        a2 = __builtin_fmaf(wv[k], v[k].y, a2);
+      }
+    } else {
+      const int wd = t0 >> 5;
+      const uint32_t bits =
+          (wd == 0 ? m[0] : (wd == 1 ? m[1] : (wd == 2 ? m[2] : m[3]))) >> (t0 & 31);
+#pragma unroll
+      for (int k = 0; k < BT; k++) {
+        const float w = nnpp_mask_weight<float>(wv[k], (int)(bits << (31 - k)) >> 31);
+        a1 = __builtin_fmaf(w, v[k].x - chi, a1);
+        a2 = __builtin_fmaf(w, v[k].y, a2);
+      }
+    }
+    S1 += (double)a1;
+    S2 += (double)a2;
+  }
+}
+
+// Radius-3 lattice ball with flux weights by class (the box lattice case): the entries walked
+// line by line in ascending order, one LDS base per line and the entries at constant offsets
+// from it, the weights by class dx^2 + dy^2 + dz^2 in scalar registers.
+struct FluxBall {
+  int32_t lc[NNPP_BALL_LINES];   // LDS position of each line's first entry, relative to the lane
+  float w32[NNPP_BALL_CLASSES];  // FP32 flux weight by class
+  double w64[NNPP_BALL_CLASSES];  // the same values in FP64
+  double wsum;                   // sum of w32 over the ball (FP64)
+};
+
+// S1 += sum w (c^_j - chi), S2 += sum w q_j over the ball's entries (FULL) or those of mask m:
+// FP32 partial sums per plane of the ball (at most 28 entries), added in FP64.
+template <bool FULL>
+__device__ __forceinline__ void ball_sums32(const FluxBall& fb, const float2* lds, int lane,
+                                            const uint32_t (&mi)[4], float chi, double& S1,
+                                            double& S2) {
+  constexpr nnpp_ball<NNPP_BALL_M> B{};
+  // opaque copies of the mask: the per-entry weights are formed here, not speculated into the
+  // caller ahead of its branches and held in registers
+  uint32_t m[4];
+#pragma unroll
+  for (int wd = 0; wd < 4; wd++) {
+    m[wd] = mi[wd];
+    if (!FULL) asm volatile("" : "+v"(m[wd]));
+  }
+  float a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+  for (int t = 0; t < B.K; t++) {
+    const int l = B.line[t];
+    const float2 v = lds[lane + fb.lc[l] + B.pos[t]];
+    float w = fb.w32[B.d2[t]];
+    if (!FULL) w = nnpp_mask_weight<float>(w, (int)(m[t >> 5] << (31 - (t & 31))) >> 31);
+    a1 = __builtin_fmaf(w, v.x - chi, a1);
+    a2 = __builtin_fmaf(w, v.y, a2);
+    if (t == B.K - 1 || B.ldz[B.line[t < B.K - 1 ? t + 1 : t]] != B.ldz[l]) {
+      S1 += (double)a1;
+      S2 += (double)a2;
+      a1 = a2 = 0.0f;
+      // one plane at a time: the next plane's addresses and weights depend on values made
+      // opaque after this plane's sums, so they are not formed ahead and held in registers
+      if (FULL) asm volatile("" : "+v"(lane) : "v"(S1), "v"(S2));
+      else
+        asm volatile("" : "+v"(lane), "+v"(m[0]), "+v"(m[1]), "+v"(m[2]), "+v"(m[3])
+                     : "v"(S1), "v"(S2));
+    }
+  }
+}
+
+// W += sum w, P += sum w (phi_j - phi_i) over the ball's entries (FULL: W += wsum) or those of
+// mask m, in FP64 from the phi window.
+template <bool FULL>
+__device__ __forceinline__ void ball_wp(const FluxBall& fb, const double* lds, int lane,
+                                        const uint32_t (&mi)[4], double phii, double& W, double& P) {
+  constexpr nnpp_ball<NNPP_BALL_M> B{};
+  uint32_t m[4];  // opaque copies, as in ball_sums32
+#pragma unroll
+  for (int wd = 0; wd < 4; wd++) {
+    m[wd] = mi[wd];
+    if (!FULL) asm volatile("" : "+v"(m[wd]));
+  }
+#pragma unroll
+  for (int t = 0; t < B.K; t++) {
+    const double v = lds[lane + fb.lc[B.line[t]] + B.pos[t]];
+    double w = fb.w64[B.d2[t]];
+    if (!FULL) {
+      w = nnpp_mask_weight<double>(w, (int)(m[t >> 5] << (31 - (t & 31))) >> 31);
+      W += w;
+    }
+    P = __fma_rn(w, v - phii, P);
+    if (t < B.K - 1 && B.ldz[B.line[t + 1]] != B.ldz[B.line[t]]) {  // plane by plane
+      if (FULL) asm volatile("" : "+v"(lane) : "v"(P));
+      else
+        asm volatile("" : "+v"(lane), "+v"(m[0]), "+v"(m[1]), "+v"(m[2]), "+v"(m[3])
+                     : "v"(P), "v"(W));
+    }
+  }
+  if (FULL) W += fb.wsum;
+}
+
+// Bits of the ball's entries whose node has phase pi (staged phases in LDS).
+__device__ __forceinline__ void ball_same(const FluxBall& fb, const uint8_t* lds, int lane, int pi,
+                                          uint32_t (&same)[4]) {
+  constexpr nnpp_ball<NNPP_BALL_M> B{};
+#pragma unroll
+  for (int w = 0; w < 4; w++) same[w] = 0u;
+#pragma unroll
+  for (int t = 0; t < B.K; t++) {
+    same[t >> 5] |= (uint32_t)((int)lds[lane + fb.lc[B.line[t]] + B.pos[t]] == pi) << (t & 31);
+    if (t < B.K - 1 && B.ldz[B.line[t + 1]] != B.ldz[B.line[t]])
+      asm volatile("" : "+v"(lane) : "v"(same[0]), "v"(same[1]), "v"(same[2]), "v"(same[3]));
+  }
+}
+
+// BALL: the ball walks above (the stencil is the radius-3 lattice ball with flux weights by
+// class); otherwise the table-driven chunks of 16 entries. The species loop is not unrolled (one
+// copy of the bond walks); acc[k] is written by selects.
+template <int S, bool BALL>
+__global__ __launch_bounds__(NNPP_BLOCK, 3) void k_flux_st32(const FluxArgs a, const FluxSt st,
+                                                             const float* __restrict__ tf32,
+                                                             const FluxBall fb) {
+  extern __shared__ float2 fst32_lds[];
+  float2* lcq = fst32_lds;                                      // [E] (c^_j, phi^_j c^_j)
+  double* lphi = reinterpret_cast<double*>(fst32_lds);          // [E] phi_j (before the species)
+  uint8_t* lph = reinterpret_cast<uint8_t*>(fst32_lds + st.E);  // [E] phases
+  const int64_t n = a.n;
+  const int64_t r0 = nnpp_xcd_block(blockIdx.x, gridDim.x) * NNPP_BLOCK;
+  const int64_t w0 = r0 + st.off_min;
+  const int l = st.perm[r0 + threadIdx.x];
+  const int64_t i = r0 + l;
+  const bool in = i < n;
+  const double* __restrict__ U = a.Un;
+  const double* __restrict__ phi = a.Un + (int64_t)S * n;
+  StSlots sl;
+  sl.init(st);
+  const double phir = phi[r0];  // the tile's phi reference: its first row (r0 < n)
+  // the species references: per phase, the window's first node of that phase (row r0 where the
+  // window has none), so the FP32 values of the same-phase sums stay away from the jumps
+  // between the phases
+  __shared__ uint32_t refpos[2];
+  if (threadIdx.x < 2) refpos[threadIdx.x] = 0xffffffffu;
+  __syncthreads();
+  // the phi window (FP64) and, per staged position, phi^ (FP32) for the species pairs; the
+  // phases, and whether the window holds one phase only (the phase of row r0)
+  float php[ST_SLOTS];
+  uint32_t sbits = 0u;  // staged positions (slots) of a solid node
+  bool uni;
+  {
+    uint32_t add;
+    const __amdgpu_buffer_rsrc_t rs = StSlots::rsrc(phi, n, w0, add);
+    double v[ST_SLOTS];
+#pragma unroll
+    for (int k = 0; k < ST_SLOTS; k++)
+      if (k * NNPP_BLOCK < st.E)
+        v[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                              rs, sl.off[k] * 8u + add, 0, 0));
+    const __amdgpu_buffer_rsrc_t rp = StSlots::rsrc(a.phase, n, w0, add);
+    uint8_t q[ST_SLOTS];
+#pragma unroll
+    for (int k = 0; k < ST_SLOTS; k++)
+      if (k * NNPP_BLOCK < st.E) q[k] = __builtin_amdgcn_raw_buffer_load_b8(rp, sl.off[k] + add, 0, 0);
+    const int p0 = a.phase[r0];
+    bool u = true;
+    uint32_t lo[2] = {0xffffffffu, 0xffffffffu};
+#pragma unroll
+    for (int k = 0; k < ST_SLOTS; k++) {
+      const int e = (int)threadIdx.x + k * NNPP_BLOCK;
+      if (k * NNPP_BLOCK < st.E) {
+        php[k] = (float)(v[k] - phir);
+        sbits |= (q[k] != 0 ? 1u : 0u) << k;
+        if (e < st.E) {
+          lphi[e] = v[k];
+          lph[e] = q[k];
+          u = u && (int)q[k] == p0;
+          const int64_t j = w0 + (int64_t)sl.off[k];
+          if (j >= 0 && j < n) {
+            const int ph = q[k] != 0 ? 1 : 0;
+            lo[ph] = sl.off[k] < lo[ph] ? sl.off[k] : lo[ph];
+          }
+        }
+      }
+    }
+    if (lo[0] != 0xffffffffu) atomicMin(&refpos[0], lo[0]);
+    if (lo[1] != 0xffffffffu) atomicMin(&refpos[1], lo[1]);
+    uni = __syncthreads_and(u) != 0;
+  }
+  const int64_t jr0 = refpos[0] != 0xffffffffu ? w0 + (int64_t)refpos[0] : r0;
+  const int64_t jr1 = refpos[1] != 0xffffffffu ? w0 + (int64_t)refpos[1] : r0;
+  const double phii = in ? phi[i] : 0.0;
+  const float phih = (float)(phii - phir);
+  const int pi = in ? a.phase[i] : 0;
+  const uint4 bq = in ? st.bmask[i] : make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t bw[4] = {bq.x, bq.y, bq.z, bq.w};
+  uint32_t m0[4], m1[4];
+  {
+    const uint8_t* rph = lph + l;
+    const int K = st.K;
+    uint32_t same[4];
+    if (uni) {  // every staged node has the phase of row r0
+#pragma unroll
+      for (int wd = 0; wd < 4; wd++) same[wd] = pi == a.phase[r0] ? ~0u : 0u;
+    } else if (BALL) {
+      ball_same(fb, lph, l, pi, same);
+    } else {
+#pragma unroll
+      for (int wd = 0; wd < 4; wd++) {
+        same[wd] = 0u;
+        if (wd * 32 < K) {
+#pragma unroll 8
+          for (int t = 0; t < 32; t++) {
+            const int tt = wd * 32 + t;
+            if (tt < K) same[wd] |= (uint32_t)((int)rph[nnpp_ld_table(st.lc, tt)] == pi) << t;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int wd = 0; wd < 4; wd++) {
+      m0[wd] = bw[wd] & st.nz[wd] & same[wd];
+      m1[wd] = bw[wd] & st.nz[wd] & ~same[wd];
+    }
+  }
+  const bool mixed = (m1[0] | m1[1] | m1[2] | m1[3]) != 0u;
+  const int intf = (pi == 1 && mixed) ? 1 : 0;  // utils.h:82-89
+  uint32_t any0, full0, any1, full1;
+  bool wfull0 = false;  // BALL: every lane of the wave has every same-phase entry
+  if (BALL) {
+    wfull0 = __all(m0[0] == st.nz[0] && m0[1] == st.nz[1] && m0[2] == st.nz[2] &&
+                   m0[3] == st.nz[3]);
+    any0 = __any((m0[0] | m0[1] | m0[2] | m0[3]) != 0u) ? ~0u : 0u;
+    any1 = __any(mixed) ? ~0u : 0u;
+    full0 = full1 = 0u;
+  } else {
+    st_chunks(st, m0, any0, full0);
+    st_chunks(st, m1, any1, full1);
+  }
+  double W0 = 0.0, P0 = 0.0, W1 = 0.0, P1 = 0.0;
+  if (BALL) {
+    if (wfull0) ball_wp<true>(fb, lphi, l, m0, phii, W0, P0);
+    else if (any0) ball_wp<false>(fb, lphi, l, m0, phii, W0, P0);
+    if (any1) ball_wp<false>(fb, lphi, l, m1, phii, W1, P1);
+  } else {
+    st_wp(st, tf32, lphi + l, m0, any0, phii, W0, P0);
+    if (any1) st_wp(st, tf32, lphi + l, m1, any1, phii, W1, P1);
+  }
+  double acc[S];
+#pragma unroll
+  for (int k = 0; k < S; k++) acc[k] = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < S; k++) {
+    __syncthreads();  // the previous reads of the buffer (phi window, species k - 1) are complete
+    const double* __restrict__ Uk = U + (int64_t)k * n;
+    const double cr0 = Uk[jr0], cr1 = Uk[jr1];
+    {
+      uint32_t add;
+      const __amdgpu_buffer_rsrc_t rs = StSlots::rsrc(Uk, n, w0, add);
+      constexpr int H = ST_SLOTS / 2;  // two rounds of loads (registers)
+#pragma unroll
+      for (int h = 0; h < ST_SLOTS; h += H) {
+        double v[H];
+#pragma unroll
+        for (int s = 0; s < H; s++)
+          if ((h + s) * NNPP_BLOCK < st.E)
+            v[s] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                  rs, sl.off[h + s] * 8u + add, 0, 0));
+#pragma unroll
+        for (int s = 0; s < H; s++) {
+          const int e = (int)threadIdx.x + (h + s) * NNPP_BLOCK;
+          if ((h + s) * NNPP_BLOCK < st.E && e < st.E) {
+            const float ch = (float)(v[s] - (((sbits >> (h + s)) & 1u) ? cr1 : cr0));
+            lcq[e] = make_float2(ch, php[h + s] * ch);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const double cri = pi ? cr1 : cr0, cro = pi ? cr0 : cr1;  // own phase, other phase
+    const float chf = (float)((in ? Uk[i] : 0.0) - cri);
+    const double ch_i = (double)chf;
+    double S1 = 0.0, S2 = 0.0;
+    // the lane's row as an opaque value per species: the per-line LDS addresses are formed
+    // inside the loop instead of being hoisted out of it and held in registers
+    int lk = l;
+    asm volatile("" : "+v"(lk));
+    const float2* rcq = lcq + lk;
+    if (BALL) {
+      if (wfull0) ball_sums32<true>(fb, lcq, lk, m0, chf, S1, S2);
+      else if (any0) ball_sums32<false>(fb, lcq, lk, m0, chf, S1, S2);
+    } else {
+      st_sums32(st, tf32, rcq, m0, any0, full0, chf, S1, S2);
+    }
+    double ak;
+    {
+      const double sd = S1;
+      const double sm = (S2 - (double)phih * (S1 + ch_i * W0)) + cri * P0;
+      ak = pass_sum(pi ? a.A_SS[k] : a.A_LL[k], sd, pi ? a.M_SS[k] : a.M_LL[k], sm);
+    }
+    if (any1) {  // interface nodes: the solid-liquid bonds
+      double T1 = 0.0, T2 = 0.0;
+      if (BALL) ball_sums32<false>(fb, lcq, lk, m1, chf, T1, T2);
+      else st_sums32(st, tf32, rcq, m1, any1, full1, chf, T1, T2);
+      const double sd = T1 + (cro - cri) * W1;
+      const double sm = (T2 - (double)phih * (T1 + ch_i * W1)) + cro * P1;
+      if (mixed) ak += pass_sum(a.A_SL[k], sd, a.M_SL[k], sm);
+    }
+#pragma unroll
+    for (int j = 0; j < S; j++)
+      if (j == k) acc[j] = ak;
+  }
+  if (in) {
+    double ci[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) ci[k] = U[(int64_t)k * n + i];
+    explicit_update<S>(a, i, pi, intf, ci, acc, phii);
+  }
+}
+
+template <int S>
+hipError_t launch_st32(const FluxArgs& a, const FluxSt& st, const float* tf32, const FluxBall* fb,
+                       size_t lds, hipStream_t s) {
+  const void* f = fb ? reinterpret_cast<const void*>(&k_flux_st32<S, true>)
+                     : reinterpret_cast<const void*>(&k_flux_st32<S, false>);
+  hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  if (fb) k_flux_st32<S, true><<<nnpp_nblocks(a.n), NNPP_BLOCK, lds, s>>>(a, st, tf32, *fb);
+  else k_flux_st32<S, false><<<nnpp_nblocks(a.n), NNPP_BLOCK, lds, s>>>(a, st, tf32, FluxBall{});
+  return hipGetLastError();
+}
+
 }  // namespace
 
 static FluxArgs flux_args(nnpp_ctx* c, double BV, const double* U1, const double* Un,
@@ -824,13 +1224,14 @@ static FluxArgs flux_args(nnpp_ctx* c, double BV, const double* U1, const double
   return a;
 }
 
-// LDS bytes of the stencil flux for the current geometry, or 0 where it does not apply.
-static size_t flux_stencil_lds(nnpp_ctx* c) {
+// LDS bytes of the stencil flux (FP64 or, with fp32, the mixed form) for the current
+// geometry, or 0 where it does not apply.
+static size_t flux_stencil_lds(nnpp_ctx* c, bool fp32 = false) {
   if (c->format & NNPP_FORMAT_NO_STENCIL) return 0;
   if (nnpp_stencil_ensure(c) != NNPP_OK) return 0;
   const nnpp_stencil& T = c->pst;
   if (T.state != 1 || !T.flux_ok || T.lds_elems > ST_SLOTS * NNPP_BLOCK) return 0;
-  const size_t bytes = (2 * sizeof(double) + 1) * (size_t)T.lds_elems;
+  const size_t bytes = ((fp32 ? sizeof(float2) : 2 * sizeof(double)) + 1) * (size_t)T.lds_elems;
   int max_lds = 0;
   if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
           hipSuccess ||
@@ -839,9 +1240,10 @@ static size_t flux_stencil_lds(nnpp_ctx* c) {
   return (bytes + 7) & ~(size_t)7;
 }
 
-// The stencil form of the explicit update (JAC = false) or of the Jacobian apply (JAC = true).
+// The stencil form of the explicit update (JAC = false; FP32 bond sums with fp32) or of the
+// Jacobian apply (JAC = true).
 template <bool JAC>
-static int launch_stencil(nnpp_ctx* c, const FluxArgs& a, size_t st_lds) {
+static int launch_stencil(nnpp_ctx* c, const FluxArgs& a, size_t st_lds, bool fp32 = false) {
   const int S = c->p.n_species;
   const nnpp_stencil& T = c->pst;
   FluxSt st;
@@ -857,10 +1259,24 @@ static int launch_stencil(nnpp_ctx* c, const FluxArgs& a, size_t st_lds) {
   st.E = T.lds_elems;
   st.off_min = T.off_min;
   for (int w = 0; w < 4; w++) st.nz[w] = T.fnz[w];
+  FluxBall fball;
+  const FluxBall* fbp = nullptr;
+  if (T.fball && !(c->format & NNPP_FORMAT_NO_BALL)) {  // the ball walks of the mixed form
+    for (int l = 0; l < NNPP_BALL_LINES; l++) fball.lc[l] = T.ball_args.lc[l];
+    for (int d = 0; d < NNPP_BALL_CLASSES; d++) {
+      fball.w32[d] = T.fball_w32[d];
+      fball.w64[d] = (double)T.fball_w32[d];
+    }
+    fball.wsum = T.fball_wsum;
+    fbp = &fball;
+  }
   hipError_t e = hipSuccess;
   switch (S) {
-#define NNPP_CASE(SS) \
-  case SS: e = launch_st<SS, JAC>(a, st, st_lds, c->stream); break;
+#define NNPP_CASE(SS)                                                                  \
+  case SS:                                                                             \
+    e = (!JAC && fp32) ? launch_st32<SS>(a, st, T.tf32, fbp, st_lds, c->stream)        \
+                       : launch_st<SS, JAC>(a, st, st_lds, c->stream);                 \
+    break;
     NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
     NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
     NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)
@@ -882,10 +1298,9 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   const int64_t nb = nnpp_nblocks(c->n);
   const int mixed = (precision == NNPP_MIXED);
   hipError_t e = hipSuccess;
-  // explicit update on a lattice: the stencil form, FP64 in both precisions (on a 128^3 lattice
-  // 2.4 ms against 4.65 ms for the FP64 and 2.45 ms for the FP32 column form on MI355X)
-  const size_t st_lds = resid ? 0 : flux_stencil_lds(c);
-  if (st_lds) return launch_stencil<false>(c, a, st_lds);
+  // explicit update on a lattice: the stencil form, with FP32 bond sums in mixed precision
+  const size_t st_lds = resid ? 0 : flux_stencil_lds(c, mixed);
+  if (st_lds) return launch_stencil<false>(c, a, st_lds, mixed);
   switch (S) {
 #define NNPP_CASE(SS) \
   case SS: e = launch_s<SS>(a, resid, mixed, c->bonds.wide, nb, c->stream); break;

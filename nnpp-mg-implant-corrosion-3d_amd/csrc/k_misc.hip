@@ -766,7 +766,7 @@ int nnpp_build_bond_slices(nnpp_ctx* c) {
 
 void nnpp_free_stencil(nnpp_stencil& t) {
   void* ps[] = {t.toff, t.lc, t.seg_start, t.seg_len, t.seg_base, t.piece, t.tw64, t.tw32, t.tf64,
-                t.bmask, t.fperm, t.pmask, t.perm, t.zmfull};
+                t.tf32, t.bmask, t.fperm, t.pmask, t.perm, t.zmfull};
   for (void* q : ps)
     if (q) hipFree(q);
   t = nnpp_stencil();
@@ -915,6 +915,7 @@ static int build_stencil(nnpp_ctx* c) {
   NNPP_HIP(c, hipMemsetAsync(T.tw32, 0, sizeof(float) * Kp, s));
   NNPP_HIP(c, hipMalloc((void**)&T.tf64, sizeof(double) * Kp));
   NNPP_HIP(c, hipMemsetAsync(T.tf64, 0, sizeof(double) * Kp, s));
+  NNPP_HIP(c, hipMalloc((void**)&T.tf32, sizeof(float) * Kp));
   NNPP_HIP(c, hipMalloc((void**)&T.bmask, sizeof(uint4) * n));
   NNPP_HIP(c, hipMalloc((void**)&T.pmask, sizeof(uint4) * n));
   NNPP_HIP(c, hipMalloc((void**)&T.perm, (size_t)nnpp_nblocks(n) * NNPP_BLOCK));
@@ -951,11 +952,36 @@ static int build_stencil(nnpp_ctx* c) {
     return NNPP_OK;
   }
   T.flux_ok = hf[2] ? 0 : 1;
+  {
+    std::vector<float> tf32(Kp, 0.0f);
+    for (int t = 0; t < K; t++) tf32[t] = (float)tf[t];
+    NNPP_HIP(c, hipMemcpyAsync(T.tf32, tf32.data(), sizeof(float) * Kp, hipMemcpyHostToDevice, s));
+    NNPP_HIP(c, hipStreamSynchronize(s));
+  }
   k_st_perm<<<nnpp_nblocks(n), NNPP_BLOCK, 0, s>>>(n, K, T.bmask, T.fperm);
   NNPP_HIP(c, hipGetLastError());
   for (int t = 0; t < K; t++)
     if (tf[t] != 0.0) T.fnz[t >> 5] |= 1u << (t & 31);
   detect_ball(T, toff, lc, pw32, pw64);
+  if (T.ball == NNPP_BALL_M && T.flux_ok) {  // the flux weights by class as well?
+    constexpr nnpp_ball<NNPP_BALL_M> B{};
+    double wc[NNPP_BALL_CLASSES] = {};
+    bool seen[NNPP_BALL_CLASSES] = {};
+    bool ok = true;
+    T.fball_wsum = 0.0;
+    for (int t = 0; t < B.K && ok; t++) {
+      const int d2 = B.d2[t];
+      if (!seen[d2]) {
+        seen[d2] = true;
+        wc[d2] = tf[t];
+      } else if (std::memcmp(&wc[d2], &tf[t], sizeof(double))) {
+        ok = false;
+      }
+      T.fball_wsum += (double)(float)tf[t];
+    }
+    for (int d = 0; d < NNPP_BALL_CLASSES; d++) T.fball_w32[d] = (float)wc[d];
+    T.fball = ok ? 1 : 0;
+  }
   if (T.ball == NNPP_BALL_M) {  // the full box, numbered x fastest?
     constexpr nnpp_ball<NNPP_BALL_M> B{};
     int64_t sy = -1, sz = -1;

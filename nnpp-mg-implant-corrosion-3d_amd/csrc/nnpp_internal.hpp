@@ -139,7 +139,7 @@ struct nnpp_stencil {
   double* tw64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] potential weight of
   float* tw32 = nullptr;       // offset t (FP64 / FP32; padding: 0)
   double* tf64 = nullptr;      // [K rounded up to NNPP_STENCIL_CHUNK] flux weight of offset t
-                               // (flux exponent; padding: 0)
+  float* tf32 = nullptr;       // (flux exponent; FP64 / FP32; padding: 0)
   int flux_ok = 0;             // every bond's flux weight equals its offset's (bit for bit)
   uint32_t fnz[4] = {0u, 0u, 0u, 0u};  // offsets with a nonzero flux weight
   uint4* bmask = nullptr;      // [n] bonds of each row
@@ -151,6 +151,9 @@ struct nnpp_stencil {
   int active = 0;              // the current potential operator uses the stencil form
   int ball = 0;  // the offsets, LDS positions and weights fit the NNPP_BALL_M ball (bit for bit)
   nnpp_ball_args ball_args = {};
+  int fball = 0;  // ball == NNPP_BALL_M and the flux weights depend on dx^2 + dy^2 + dz^2 only
+  float fball_w32[NNPP_BALL_CLASSES] = {};  // (bit for bit): the FP32 flux weight by class
+  double fball_wsum = 0.0;  // sum of the FP32 flux weights of the ball's offsets (in FP64)
   // Box lattice (ball == NNPP_BALL_M only): the nodes are the full nx x ny x nz box numbered
   // x fastest (row strides sy = nx, sz = nx ny, n = nx ny nz) and every row's bonds are exactly
   // the ball offsets whose far end lies inside the box. The z-marching kernels need this: they

@@ -153,6 +153,8 @@ def test_c3_mixed_vs_fp64_1000_steps(case128):
                 done += st["steps"]
                 say(name, done)
             out[name] = (pb.get_state(), pb.get_phase(), st)
+    # the lattice's stencil flux: FP64 bond sums, FP32 bond sums in mixed precision
+    assert out["fp64"][2]["flux_stencil"] == 1 and out["mixed"][2]["flux_stencil"] == 2
     errs = field_errors(out["mixed"][0], out["fp64"][0])
     say("per-field (max, L2):", errs)
     assert np.array_equal(out["mixed"][1], out["fp64"][1])

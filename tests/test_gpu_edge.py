@@ -151,9 +151,9 @@ def _exact_run(shape, stencil):
         ref, U, ph, st = run_both(case, 4, prec)
         assert st["status"] == 0
         # the lattices at least 7 wide hold the whole radius-3 ball of offsets
-        # (full boxes: the z-marching SpMV)
-        assert st["flux_stencil"] == int(stencil) and st["pot_stencil"] == 4 * int(stencil), \
-            (shape, prec)
+        # (full boxes: the z-marching SpMV); the stencil flux sums in FP32 in mixed precision
+        fs = (1 if prec == nn.FP64 else 2) * int(stencil)
+        assert st["flux_stencil"] == fs and st["pot_stencil"] == 4 * int(stencil), (shape, prec)
         assert np.array_equal(ph, ref["phase"])
         tol = 1e-12 if prec == nn.FP64 else 1e-6
         for f in range(case.params.S + 1):
@@ -168,7 +168,12 @@ def _exact_run(shape, stencil):
                 assert rel_err(U1[f], ref["U"][f]) < tol, (shape, prec, f)
             _, U2, ph2, st2 = run_both(case, 4, prec, fmt=nn.FORMAT_NO_BALL, oracle=False)
             assert st2["pot_stencil"] == 1
-            assert np.array_equal(U2, U1) and np.array_equal(ph2, ph1), (shape, prec)
+            assert np.array_equal(ph2, ph1), (shape, prec)
+            if prec == nn.FP64:
+                assert np.array_equal(U2, U1), shape
+            else:  # the mixed flux's table-driven walk groups its FP32 sums otherwise
+                for f in range(case.params.S + 1):
+                    assert rel_err(U2[f], ref["U"][f]) < tol, (shape, prec, f)
 
 
 @pytest.mark.parametrize("shape", [(7, 7, 5), (7, 7, 7), (9, 9, 9), (16, 9, 7), (13, 13, 13)])
@@ -190,7 +195,8 @@ def test_stencil_table_loop_when_not_the_radius3_ball():
     for prec in (nn.FP64, nn.MIXED):
         ref, U, ph, st = run_both(case, 3, prec)
         assert st["status"] == 0
-        assert st["pot_stencil"] == 1 and st["flux_stencil"] == 1, prec
+        assert st["pot_stencil"] == 1, prec
+        assert st["flux_stencil"] == (1 if prec == nn.FP64 else 2), prec
         assert np.array_equal(ph, ref["phase"])
         tol = 1e-12 if prec == nn.FP64 else 1e-6
         for f in range(case.params.S + 1):

@@ -324,8 +324,9 @@ def test_storage_formats_bit_identical(kind):
             # full box, so the z-marching kernels run unless a flag turns them off
             assert st["pot_stencil"] == (0 if not stencil else (1 if fmt & B else (3 if fmt & Z
                                                                                     else 4))), fmt
-            # the explicit flux takes the stencil form (FP64) wherever the operator does
-            assert st["flux_stencil"] == int(stencil), fmt
+            # the explicit flux takes the stencil form wherever the operator does (FP32 bond
+            # sums in mixed precision)
+            assert st["flux_stencil"] == (0 if not stencil else (1 if prec == nn.FP64 else 2)), fmt
             if stencil:
                 assert st["pot_col_bytes"] == 0 and st["pot_weight_bytes"] == 0
             else:
@@ -333,21 +334,23 @@ def test_storage_formats_bit_identical(kind):
                 assert st["pot_weight_bytes"] == (4 if (fmt & D or kind == "screw") else 1)
             if zm:
                 # the z-marching SpMV: FP64 rows as the other forms, dot products summed in
-                # another order; FP32 rows with their own chunks
+                # another order; mixed: FP32 rows with their own chunks and the FP32 flux
                 for f in range(case.params.S + 1):
                     assert rel_err(U[f], res[nn.FP64][Z][0][f]) < (1e-12 if prec == nn.FP64
-                                                                 else 1e-9), (prec, fmt, f)
+                                                                 else 1e-6), (prec, fmt, f)
             elif prec == nn.FP64 or not stencil:
                 # FP64: every form performs the same additions; FP32 column forms too
                 assert np.array_equal(U, U0 if prec == nn.FP64 else Uc), (prec, fmt)
                 if prec == nn.FP64:
                     assert st["cg_iterations"] == st0["cg_iterations"], fmt
             else:
-                # mixed stencil form: the FP64 stencil flux and the FP32 stencil SpMV, whose FP64
-                # residual replacements bring the solves to the same rtol as the FP64 run
-                assert np.array_equal(U, U0), fmt
+                # mixed stencil forms: FP32 stencil SpMVs with the same additions, FP64 residual
+                # replacements that bring the solves to the FP64 rtol; the FP32 stencil flux
+                # walks the lattice ball plane by plane, or (B) the offset table in chunks of 16
+                if not fmt & B:
+                    assert np.array_equal(U, U0), fmt
                 for f in range(case.params.S + 1):
-                    assert rel_err(U[f], res[nn.FP64][nn.FORMAT_AUTO][0][f]) < 1e-9, (fmt, f)
+                    assert rel_err(U[f], res[nn.FP64][nn.FORMAT_AUTO][0][f]) < 1e-6, (fmt, f)
     with problem(case) as pb:
         with pytest.raises(nn.NnppError):
             pb.set_format(32)
