@@ -120,6 +120,7 @@ def kernels_only(nn, case, reps):
     out = {"nodes": case.n, "bonds": nnz}
     for name, k, prec in (("pot_spmv_fp32", nn._abi.KERNEL_POT_SPMV, nn.MIXED),
                           ("pot_spmv_fp64", nn._abi.KERNEL_POT_SPMV, nn.FP64),
+                          ("cg_iter_fp32", nn._abi.KERNEL_CG_ITER, nn.MIXED),
                           ("flux_fp64", nn._abi.KERNEL_FLUX, nn.FP64),
                           ("flux_mixed", nn._abi.KERNEL_FLUX, nn.MIXED)):
         out[name + "_ms"] = pb.time_kernel(k, prec, reps)
@@ -321,17 +322,21 @@ def summarize_rocprof(trace_db, fetch_db=None, write_db=None, copyf_db=None, cop
 
 
 def roofline_entry(t_ms, fmt_bytes, csr_bytes, stencil, cb, wb, zmarch=False):
-    """Roofline of the dominant kernel (the FP32 potential SpMV): `achieved` = the bytes its
-    stored operator form must move per launch (compulsory traffic: the form's per-row words and
-    the vectors, each gathered value counted once) / the measured launch time. The survey's CSR
-    figure (a 4-byte column per liquid bond) is kept beside it as `csr_model_GBps`: the
-    stencil form never reads those bytes, so that ratio is not a bandwidth."""
-    kname = "k_zm_m_spmv" if zmarch else ("k_m_spmv_st" if stencil else "k_m_spmv")
+    """Roofline of the dominant kernel: `achieved` = the bytes it must move per launch
+    (compulsory traffic: the stored operator form's per-row words and the vectors, each value
+    counted once) / the measured launch time. On a full box lattice that kernel is the fused
+    FP32 CG iteration (SpMV, the previous iteration's vector updates and the dot products in
+    one launch), elsewhere the FP32 potential SpMV. The survey's CSR figure (a 4-byte column
+    per liquid bond) is kept beside it as `csr_model_GBps`: the stencil forms never read those
+    bytes, so that ratio is not a bandwidth."""
+    kname = "k_zm_m_cgs" if zmarch else ("k_m_spmv_st" if stencil else "k_m_spmv")
+    label = ("FP32 single-reduction CG iteration: M u, vector updates, gamma/delta/r.r"
+             if zmarch else "FP32 potential SpMV + p.q")
     traffic = measured_traffic(kname)
     ach = fmt_bytes / (t_ms * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-           "kernel": f"{kname} (FP32 potential SpMV + p.q)", "avg_launch_ms": t_ms,
+           "kernel": f"{kname} ({label})", "avg_launch_ms": t_ms,
            "bytes_per_launch": fmt_bytes,
            "csr_model_bytes_per_launch": csr_bytes,
            "csr_model_GBps": csr_bytes / (t_ms * 1e-3) / 1e9,
@@ -526,6 +531,8 @@ def main():
     if zmarch:  # no per-row words: a free flag per node
         bytes32 = n * (1 + 3 * 4)
         bytes64 = n * (1 + 3 * 8)
+        # the fused FP32 CG iteration: free flag, diagonal, r w s D^-1 in, p x in, s r w p x out
+        bytes_cgit = n * (1 + 4 + 4 * 4 + 2 * 4 + 5 * 4)
     elif stencil:
         bytes32 = n * (16 + 3 * 4)
         bytes64 = n * (16 + 3 * 8)
@@ -540,6 +547,7 @@ def main():
     fstencil = bool(fform)
     fbytes = lambda wb: ((n * 16 if fstencil else nnz * (4 + wb))  # noqa: E731
                          + n * (8 * (S + 1) + 1 + 2 + 8 * S))
+    t_cgit = pbm.time_kernel(nn._abi.KERNEL_CG_ITER, nn.MIXED, reps) if zmarch else None
     kern = {
         "pot_spmv_fp32": {"ms": t_spmv32, "alg_GBps": alg_bytes / (t_spmv32 * 1e-3) / 1e9,
                           "format_GBps": bytes32 / (t_spmv32 * 1e-3) / 1e9},
@@ -552,6 +560,9 @@ def main():
                            fform, "columns (fp32)"),
                        "GBps": fbytes(4) / (t_flux32 * 1e-3) / 1e9},
     }
+    if zmarch:
+        kern["cg_iter_fp32"] = {"ms": t_cgit, "GBps": bytes_cgit / (t_cgit * 1e-3) / 1e9,
+                                "form": "fused single-reduction CG iteration"}
     for r_ in res.values():
         r_["pb"].close()
 
@@ -596,7 +607,9 @@ def main():
                               "within_1e-6": bool(max_err <= 1e-6 and max_l2 <= 1e-6)},
         "neighbor_build_s": t_nb,
         "kernels": kern,
-        "roofline": roofline_entry(t_spmv32, bytes32, alg_bytes, stencil, cb, wb32, zmarch),
+        "roofline": (roofline_entry(t_cgit, bytes_cgit, alg_bytes, stencil, cb, wb32, zmarch)
+                     if zmarch else
+                     roofline_entry(t_spmv32, bytes32, alg_bytes, stencil, cb, wb32, zmarch)),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -445,10 +445,12 @@ int nnpp_stream(nnpp_ctx* ctx, void** stream);
 
 /* Measurement hook, no reference counterpart: launches one hot kernel `reps` times on the
  * context's stream between two HIP events and returns the mean duration per launch in ms.
- * kernel: NNPP_KERNEL_FLUX (explicit flux/update kernel on slot N) or NNPP_KERNEL_POT_SPMV
- * (potential operator application with its dot product); precision selects the FP64 or the
- * mixed variant. Leaves the state of slot N unchanged. */
-enum { NNPP_KERNEL_FLUX = 0, NNPP_KERNEL_POT_SPMV = 1 };
+ * kernel: NNPP_KERNEL_FLUX (explicit flux/update kernel on slot N), NNPP_KERNEL_POT_SPMV
+ * (potential operator application with its dot product) or NNPP_KERNEL_CG_ITER (one fused
+ * inner iteration of the potential solve: NNPP_MIXED on a full box lattice only, where the
+ * solve runs single-reduction CG iterations as one kernel each; NNPP_ERR_ARG otherwise);
+ * precision selects the FP64 or the mixed variant. Leaves the state of slot N unchanged. */
+enum { NNPP_KERNEL_FLUX = 0, NNPP_KERNEL_POT_SPMV = 1, NNPP_KERNEL_CG_ITER = 2 };
 int nnpp_time_kernel(nnpp_ctx* ctx, int32_t kernel, int32_t precision, int32_t reps,
                      double* avg_ms);
 

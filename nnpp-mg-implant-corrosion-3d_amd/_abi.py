@@ -17,7 +17,7 @@ CHEM_GENERIC, CHEM_REFERENCE = 0, 1
 FP64, MIXED = 0, 1
 CLAMP_NONE, CLAMP_ALWAYS, CLAMP_LIQUID = 0, 1, 2
 SLOT_N, SLOT_N1 = 0, 1
-KERNEL_FLUX, KERNEL_POT_SPMV = 0, 1
+KERNEL_FLUX, KERNEL_POT_SPMV, KERNEL_CG_ITER = 0, 1, 2
 FORMAT_AUTO, FORMAT_WIDE_COLUMNS, FORMAT_NO_DICTIONARY, FORMAT_NO_STENCIL = 0, 1, 2, 4
 FORMAT_NO_BALL = 8
 FORMAT_NO_ZMARCH = 16

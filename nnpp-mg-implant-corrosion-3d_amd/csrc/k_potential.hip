@@ -486,15 +486,40 @@ __device__ __forceinline__ double zm_weight<double>(const nnpp_ball_args& b, int
   return b.w64[d2];
 }
 
-// Walks the block's work item and hands every finished row to emit(i, y, p_i) (rows inside
-// the box only). All threads of the block must call it (barriers inside). full[tile * nz + z]:
-// every staged point of plane z of the tile is a free node (no weight needs its m_j).
-template <typename T, typename Emit>
+// Input vector of the z-marching walk: load(j) issues the loads of node j's value into a Raw,
+// value(raw) forms the value (zero from Raw{}); the loads are issued a plane ahead of use.
+template <typename T>
+struct ZmLoad {  // a stored vector
+  using Raw = T;
+  const T* __restrict__ v;
+  __device__ __forceinline__ Raw load(uint32_t j) const { return v[j]; }
+  __device__ __forceinline__ T value(const Raw& r) const { return r; }
+};
+// Row handler without prefetched operands: f(i, y, p_i).
+template <typename F>
+struct ZmEmit {
+  struct Pre {};
+  F f;
+  __device__ __forceinline__ Pre pre(uint32_t) const { return Pre{}; }
+  template <typename T>
+  __device__ __forceinline__ void operator()(int64_t i, T y, T pi, const Pre&) { f(i, y, pi); }
+};
+template <typename F>
+__device__ __forceinline__ ZmEmit<F> zm_emit(F f) {
+  return ZmEmit<F>{f};
+}
+
+// Walks the block's work item and hands every finished row to emit(i, y, p_i, pre) (rows inside
+// the box only), pre = emit.pre(i) loaded a plane ahead. All threads of the block must call it
+// (barriers inside). full[tile * nz + z]: every staged point of plane z of the tile is a free
+// node (no weight needs its m_j).
+template <typename T, typename Src, typename Emit>
 __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& ball,
                                         const uint8_t* __restrict__ pfree,
                                         const uint8_t* __restrict__ full_tab,
-                                        const T* __restrict__ dd, const T* __restrict__ v,
-                                        Emit&& emit) {
+                                        const T* __restrict__ dd, const Src& src, Emit& emit) {
+  using Raw = typename Src::Raw;
+  using Pre = typename Emit::Pre;
   __shared__ ZmPair<T> lds[2][ZM_PTS];
   const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
   const int zc = (int)(item % g.nzc);
@@ -523,30 +548,38 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
   ZmRow<T> row[ZM_SL];
   T po[ZM_SL];
   const auto in_chunk = [&](int zo) { return zo >= z0 && zo < zend; };
+  const auto row_raw = [&](int zo) {
+    return (col_ok && in_chunk(zo)) ? src.load((uint32_t)zo * sz + col) : Raw{};
+  };
 #pragma unroll
   for (int q = 0; q < ZM_SL; q++) {
-    const int zo = zs - ZM_M + q;
     row[q].reset();
-    po[q] = (col_ok && in_chunk(zo)) ? v[(uint32_t)zo * sz + col] : T(0);
+    po[q] = src.value(row_raw(zs - ZM_M + q));
   }
-  // diagonal term and free flag of the row in slot 0 (the next to finish), loaded a plane ahead
+  // the input value of the row that enters the last slot at the next plane, loaded a plane ahead
+  Raw pend = row_raw(zs + 1 + ZM_M);
+  // diagonal term, free flag and handler operands of the row in slot 0 (the next to finish),
+  // loaded a plane ahead
   T d0 = T(0);
   bool f0 = false;
+  Pre pre0{};
   const auto load_first = [&](int zo) {
     if (col_ok && in_chunk(zo)) {
-      d0 = dd[(uint32_t)zo * sz + col];
-      f0 = pfree[(uint32_t)zo * sz + col] != 0;
+      const uint32_t i = (uint32_t)zo * sz + col;
+      d0 = dd[i];
+      f0 = pfree[i] != 0;
+      pre0 = emit.pre(i);
     }
   };
   load_first(zs - ZM_M);
-  // registers of the next plane's staged points
-  T sv[ZM_STAGE];
+  // loads of the next plane's staged points
+  Raw sv[ZM_STAGE];
   uint8_t sm[ZM_STAGE];
   const auto fetch = [&](int zi) {
     const uint32_t base = (uint32_t)zi * sz;
 #pragma unroll
     for (int k = 0; k < ZM_STAGE; k++) {
-      sv[k] = soff[k] >= 0 ? v[base + (uint32_t)soff[k]] : T(0);
+      sv[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : Raw{};
       sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
     }
   };
@@ -557,16 +590,16 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     const int zo = zi - ZM_M;
     if (col_ok && in_chunk(zo)) {
       const T y = f0 ? row[0].result(d0, po[0]) : ZmRow<T>{}.result(d0, po[0]);
-      emit((int64_t)((uint32_t)zo * sz + col), y, po[0]);
+      emit((int64_t)((uint32_t)zo * sz + col), y, po[0], pre0);
     }
 #pragma unroll
     for (int q = 0; q + 1 < ZM_SL; q++) {
       row[q] = row[q + 1];
       po[q] = po[q + 1];
     }
-    const int zn = zi + 1 + ZM_M;
     row[ZM_SL - 1].reset();
-    po[ZM_SL - 1] = (col_ok && in_chunk(zn)) ? v[(uint32_t)zn * sz + col] : T(0);
+    po[ZM_SL - 1] = src.value(pend);
+    pend = row_raw(zi + 2 + ZM_M);
     load_first(zo + 1);
   };
   int buf = 0;
@@ -576,7 +609,7 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
       const int e = threadIdx.x + k * NNPP_BLOCK;
       if (e < ZM_PTS) {
         ZmPair<T> pr;
-        pr.v = sv[k];
+        pr.v = src.value(sv[k]);
         pr.m = sm[k] ? T(1) : T(0);
         lds[buf][e] = pr;
       }
@@ -671,10 +704,11 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_spmv(ZmGeom g, nnpp_ball_ar
                                                           const double* __restrict__ scal) {
   if (cg_done(scal)) return;
   double v[1] = {0.0};
-  zm_walk<float>(g, ball, pfree, full, dd, p, [&](int64_t i, float y, float pi) {
+  auto em = zm_emit([&](int64_t i, float y, float pi) {
     q[i] = y;
     v[0] += (double)pi * (double)y;
   });
+  zm_walk<float>(g, ball, pfree, full, dd, ZmLoad<float>{p}, em);
   nnpp_block_partial<1>(v, part);
 }
 
@@ -689,10 +723,11 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_cg_spmv(ZmGeom g, nnpp_ball_a
                                                            const double* __restrict__ scal) {
   if (cg_done(scal)) return;
   double v[1] = {0.0};
-  zm_walk<double>(g, ball, pfree, full, dd, p, [&](int64_t i, double y, double pi) {
+  auto em = zm_emit([&](int64_t i, double y, double pi) {
     q[i] = y;
     v[0] += pi * y;
   });
+  zm_walk<double>(g, ball, pfree, full, dd, ZmLoad<double>{p}, em);
   nnpp_block_partial<1>(v, part);
 }
 
@@ -710,7 +745,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_resid(ZmGeom g, nnpp_ball_arg
                                                          double* __restrict__ pv,
                                                          double* __restrict__ part) {
   double v[2] = {0.0, 0.0};
-  zm_walk<double>(g, ball, pfree, full, dd, x, [&](int64_t i, double mx, double) {
+  auto em = zm_emit([&](int64_t i, double mx, double) {
     const double pi_ = pinv[i];
     const double ri = (pi_ != 0.0) ? b[i] - mx : 0.0;
     r[i] = ri;
@@ -722,6 +757,184 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_resid(ZmGeom g, nnpp_ball_arg
       v[0] += ri * zi;
     }
   });
+  zm_walk<double>(g, ball, pfree, full, dd, ZmLoad<double>{x}, em);
+  nnpp_block_partial<2>(v, part);
+}
+
+// ------------------------------------------- single-reduction FP32 CG (z-marching) ---------
+// The mixed solve's inner iterations on a box lattice in the single-reduction form of CG
+// (Chronopoulos and Gear, J. Comput. Appl. Math. 25 (1989) 153-168). With u = D^-1 r and
+// w = M u:
+//   gamma = r.u, delta = u.w, beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old),
+//   p = u + beta p, s = w + beta s (= M p), x += alpha p, r -= alpha s,
+// so one SpMV and one grid reduction (gamma, delta and r.r from the same pass) make an
+// iteration. A launch applies the previous iteration's vector updates while it stages its
+// input: at every point it stages, s_{k-1} = w_{k-1} + beta s_{k-2}, r_k = r_{k-1} - alpha s_{k-1}
+// and u_k = D^-1 r_k are formed from the stored vectors (at the halo points of other blocks
+// too), the SpMV gives w_k = M u_k, and the rows of the block store s_{k-1}, r_k, w_k,
+// p_{k-1} = u_{k-1} + beta p_{k-2} and x_k = x_{k-1} + alpha p_{k-1}. r, w and s alternate
+// between two buffers by launch parity (other blocks read the old values at their halos); p and
+// x are read and written by their own rows only. alpha, beta and the stop test come from the
+// previous launch's partials, which every block sums itself: an iteration is one kernel and one
+// reduction instead of an SpMV, an update and a direction kernel with a reduction each.
+//
+// RESTART (at the start, and after a residual replacement): u = D^-1 r from the stored r,
+// w = M u, r and s copied to the other parity, p and x untouched, no stop test. The next launch
+// then continues the recurrence with the replaced residual (beta from the gamma before the
+// replacement, the direction kept), or at the start (gamma_old = inf, p = s = 0) begins it.
+// The rows outside the operator (D^-1 = 0) carry r = u = w = s = p = 0.
+struct ZmCgVec {
+  const float* r_in;  // r_{k-1}, w_{k-1}, s_{k-2} (parity of the launch)
+  const float* w_in;
+  const float* s_in;
+  float* r_out;       // r_k, w_k, s_{k-1} (the other parity)
+  float* w_out;
+  float* s_out;
+  float* p;           // p_{k-2} -> p_{k-1}
+  float* e;           // x_{k-1} -> x_k (FP32 correction, units of ||b||)
+  const float* pinv;  // D^-1
+};
+
+// timing: no stop test and no scalar writes (the kernel timing hook launches it repeatedly)
+template <bool RESTART>
+__global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_args ball,
+                                                         const uint8_t* __restrict__ pfree,
+                                                         const uint8_t* __restrict__ full,
+                                                         const float* __restrict__ dd, ZmCgVec vec,
+                                                         const double* __restrict__ part_in,
+                                                         int64_t nblk,
+                                                         double* __restrict__ part_out,
+                                                         double* scal, int par, double reliable,
+                                                         int timing) {
+  if (cg_done(scal)) return;
+  float alpha = 0.0f, beta = 0.0f;
+  if (!RESTART) {
+    double v[3];
+    nnpp_final_sum_all<3>(v, part_in, nblk);  // gamma, delta, r.r of the previous launch
+    // RRMAX is a running maximum: a block that reads it after block 0 updated it decides alike
+    const double mx = fmax(nnpp_ld_scalar(scal + SC_RRMAX), v[2]);
+    const bool done = !timing && (v[2] <= nnpp_ld_scalar(scal + SC_TARGET2) ||
+                                  (reliable > 0.0 && v[2] <= reliable * mx));
+    const double g_old = nnpp_ld_scalar(scal + SC_CG_GAMMA0 + 2 * par);
+    const double a_old = nnpp_ld_scalar(scal + SC_CG_ALPHA0 + 2 * par);
+    const double b = v[0] / g_old;
+    const double a = v[0] / (v[1] - b * v[0] / a_old);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !timing) {
+      scal[SC_RR] = v[2];
+      scal[SC_RRMAX] = mx;
+      if (done) {
+        scal[SC_DONE] = 1.0;
+        scal[SC_CG_PARDONE] = (double)par;
+      } else {
+        scal[SC_CG_GAMMA0 + 2 * (par ^ 1)] = v[0];
+        scal[SC_CG_ALPHA0 + 2 * (par ^ 1)] = a;
+        scal[SC_ITERS] += 1.0;
+      }
+    }
+    if (done) return;
+    alpha = (float)a;
+    beta = (float)b;
+  } else if (blockIdx.x == 0 && threadIdx.x == 0 && !timing) {
+    scal[SC_CG_GAMMA0 + 2 * (par ^ 1)] = nnpp_ld_scalar(scal + SC_CG_GAMMA0 + 2 * par);
+    scal[SC_CG_ALPHA0 + 2 * (par ^ 1)] = nnpp_ld_scalar(scal + SC_CG_ALPHA0 + 2 * par);
+  }
+  // r_k at node j from the stored r_{k-1}, w_{k-1}, s_{k-2} (the staged points and the rows
+  // form it alike, bit for bit), u_k = D^-1 r_k
+  struct Src {
+    struct Raw {
+      float r, w, s, d;
+    };
+    const float* __restrict__ r_in;
+    const float* __restrict__ w_in;
+    const float* __restrict__ s_in;
+    const float* __restrict__ pinv;
+    float alpha, beta;
+    __device__ __forceinline__ Raw load(uint32_t j) const {
+      return Raw{r_in[j], RESTART ? 0.0f : w_in[j], RESTART ? 0.0f : s_in[j], pinv[j]};
+    }
+    __device__ __forceinline__ float r_new(const Raw& a) const {
+      return RESTART ? a.r : __fmaf_rn(-alpha, __fmaf_rn(beta, a.s, a.w), a.r);
+    }
+    __device__ __forceinline__ float value(const Raw& a) const { return a.d * r_new(a); }
+  } src{vec.r_in, vec.w_in, vec.s_in, vec.pinv, alpha, beta};
+  // rows: the stored vectors' old values, loaded a plane ahead
+  double v[3] = {0.0, 0.0, 0.0};
+  struct Em {
+    struct Pre {
+      typename Src::Raw a;
+      float p, e;
+    };
+    const Src& src;
+    const ZmCgVec& vec;
+    double (&v)[3];
+    __device__ __forceinline__ Pre pre(uint32_t i) const {
+      Pre q;
+      q.a = src.load(i);
+      q.a.s = vec.s_in[i];  // restart: s is copied
+      q.p = RESTART ? 0.0f : vec.p[i];
+      q.e = RESTART ? 0.0f : vec.e[i];
+      return q;
+    }
+    __device__ __forceinline__ void operator()(int64_t i, float y, float ui, const Pre& q) {
+      const float ri = src.r_new(q.a);
+      if (RESTART) {
+        vec.s_out[i] = q.a.s;
+      } else {
+        const float si = __fmaf_rn(src.beta, q.a.s, q.a.w);
+        const float pi_ = __fmaf_rn(src.beta, q.p, q.a.d * q.a.r);
+        vec.p[i] = pi_;
+        vec.e[i] = __fmaf_rn(src.alpha, pi_, q.e);
+        vec.s_out[i] = si;
+      }
+      vec.r_out[i] = ri;
+      vec.w_out[i] = y;
+      v[0] += (double)ri * (double)ui;
+      v[1] += (double)ui * (double)y;
+      v[2] += (double)ri * (double)ri;
+    }
+  } em{src, vec, v};
+  zm_walk<float>(g, ball, pfree, full, dd, src, em);
+  nnpp_block_partial<3>(v, part_out);
+}
+
+// Start (START) or residual replacement of the single-reduction FP32 CG: r = r64 / ||b|| into
+// the parity-0 buffer, x = 0 (the correction is in the FP64 solution); START: p = s = 0 and
+// gamma_old = inf, so the first iteration takes beta = 0; otherwise s and (gamma, alpha) move
+// from the parity of the launch that stopped to parity 0. Partials (0, r.r).
+template <bool START>
+__global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_replace(int32_t n,
+                                                             const double* __restrict__ r64,
+                                                             double sb, float* __restrict__ r0,
+                                                             float* __restrict__ e,
+                                                             float* __restrict__ p,
+                                                             float* __restrict__ s0,
+                                                             const float* __restrict__ s1,
+                                                             double* __restrict__ part,
+                                                             double* scal) {
+  const int q = START ? 0 : (int)nnpp_ld_scalar(scal + SC_CG_PARDONE);
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * NNPP_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * NNPP_BLOCK) {
+    const float ri = (float)(r64[i] * sb);
+    r0[i] = ri;
+    e[i] = 0.0f;
+    if (START) {
+      p[i] = 0.0f;
+      s0[i] = 0.0f;
+    } else if (q) {
+      s0[i] = s1[i];
+    }
+    v[1] += (double)ri * (double)ri;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (START) {
+      scal[SC_CG_GAMMA0] = __builtin_inf();
+      scal[SC_CG_ALPHA0] = 1.0;
+    } else if (q) {
+      scal[SC_CG_GAMMA0] = nnpp_ld_scalar(scal + SC_CG_GAMMA1);
+      scal[SC_CG_ALPHA0] = nnpp_ld_scalar(scal + SC_CG_ALPHA1);
+    }
+  }
   nnpp_block_partial<2>(v, part);
 }
 
@@ -1429,7 +1642,81 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
     const double drop2 = c->inner_rtol * c->inner_rtol;
     if ((st = resid(false))) return st;
     rr = c->h_scal[SC_RR];
-    if (rr > target2) {
+    if (rr > target2 && op.zm) {
+      // box lattice: the single-reduction iterations (one fused kernel each); r, w and s
+      // alternate between two buffers by launch parity, and so do the partials
+      float* const Rv[2] = {c->f_r, c->f_z};
+      float* const Wv[2] = {c->f_q, c->f_w1};
+      float* const Sv[2] = {c->f_s0, c->f_s1};
+      double* const Pv[2] = {part, part + 3 * NNPP_RED_GRID};
+      const auto launch = [&](bool restart, int q) {
+        const ZmCgVec v{Rv[q], Wv[q], Sv[q], Rv[q ^ 1], Wv[q ^ 1], Sv[q ^ 1],
+                        c->f_p, c->f_e, c->pinv32};
+        if (restart)
+          k_zm_m_cgs<true><<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull,
+                                                         c->pdd32, v, Pv[q], op.g32, Pv[q ^ 1],
+                                                         c->scal, q, drop2, 0);
+        else
+          k_zm_m_cgs<false><<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull,
+                                                          c->pdd32, v, Pv[q], op.g32, Pv[q ^ 1],
+                                                          c->scal, q, drop2, 0);
+      };
+      k_zm_m_replace<true><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, Rv[0], c->f_e, c->f_p,
+                                                    Sv[0], Sv[1], part, c->scal);
+      NNPP_HIP(c, hipGetLastError());
+      if ((st = nnpp_finalize(c, FIN_M_REPLACE_START, g, rtol2))) return st;
+      launch(true, 0);
+      n32++;
+      par = 1;
+      BatchPlan plan;
+      if (!rhs) plan.prev = &c->cg_phase_len[1];
+      double rr_b = rr / bnorm2, stop_b = std::max(rtol2, drop2 * rr_b);
+      int it_b = 0;
+      while (true) {
+        const int left = c->maxit - total;
+        if (left <= 0) {
+          status = NNPP_ERR_NOCONV;
+          break;
+        }
+        const int batch = plan.next(rr_b, stop_b, left, total);
+        for (int it = 0; it < batch; it++, par ^= 1) launch(false, par);
+        NNPP_HIP(c, hipGetLastError());
+        if ((st = read_scalars(c))) return st;
+        total = (int)c->h_scal[SC_ITERS];
+        if (!std::isfinite(c->h_scal[SC_RR]))
+          return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
+        plan.observe(rr_b, c->h_scal[SC_RR], total - it_b);
+        rr_b = c->h_scal[SC_RR];
+        stop_b = std::max(c->h_scal[SC_TARGET2], drop2 * c->h_scal[SC_RRMAX]);
+        it_b = total;
+        if (c->h_scal[SC_DONE] == 0.0) continue;
+        plan.end_phase(total);
+        // reliable update: fold the correction into x, recompute the residual in FP64, and
+        // continue the recurrence from it (the direction kept)
+        k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
+        NNPP_HIP(c, hipGetLastError());
+        if ((st = resid(false))) return st;
+        rr = c->h_scal[SC_RR];
+        if (!std::isfinite(rr))
+          return nnpp_set_error(c, NNPP_ERR_NONFINITE, "potential solve diverged (non-finite)");
+        if (rr <= target2) break;
+        k_zm_m_replace<false><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, Rv[0], c->f_e, c->f_p,
+                                                       Sv[0], Sv[1], part, c->scal);
+        NNPP_HIP(c, hipGetLastError());
+        if ((st = nnpp_finalize(c, FIN_M_REPLACE, g, rtol2))) return st;
+        launch(true, 0);
+        n32++;
+        par = 1;
+        rr_b = rr / bnorm2;  // the replaced residual starts the next phase
+        stop_b = std::max(rtol2, drop2 * rr_b);
+      }
+      n32 += total;
+      if (!rhs && status == NNPP_OK) c->cg_phase_len[1] = plan.lens;
+      if (status != NNPP_OK) {  // keep the progress made so far
+        k_m_apply<<<nb, NNPP_BLOCK, 0, s>>>(n, 1.0 / sb, c->f_e, c->cg_x);
+        NNPP_HIP(c, hipGetLastError());
+      }
+    } else if (rr > target2) {
       k_m_replace<true><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, c->pinv32, c->f_e, c->f_r,
                                                  c->f_z, c->f_p, part);
       NNPP_HIP(c, hipGetLastError());
@@ -1521,6 +1808,23 @@ int nnpp_phi_guess_dev(nnpp_ctx* c, int hist, const double* phi, const double* p
   if (!c->n) return NNPP_OK;
   k_phi_guess<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(c->n, hist, c->pfree, phi, prev,
                                                                  prev2, out);
+  NNPP_HIP(c, hipGetLastError());
+  return NNPP_OK;
+}
+
+// One fused inner iteration of the mixed solve's single-reduction CG on a box lattice, without
+// its stop test and scalar updates (the kernel timing hook; the vectors' values are arbitrary).
+// NNPP_ERR_ARG where the solve has no such kernel (FP64, or not a box lattice).
+int nnpp_cg_iter_launch(nnpp_ctx* c, int precision) {
+  if (!c->n) return NNPP_OK;
+  const PotOp op = pot_op(c);
+  if (!op.zm || precision != NNPP_MIXED)
+    return nnpp_set_error(c, NNPP_ERR_ARG,
+                          "the fused CG iteration runs in mixed precision on box lattices only");
+  const ZmCgVec v{c->f_z, c->f_w1, c->f_s1, c->f_r, c->f_q, c->f_s0, c->f_p, c->f_e, c->pinv32};
+  k_zm_m_cgs<false><<<op.g32, NNPP_BLOCK, 0, c->stream>>>(
+      op.zg, c->pst.ball_args, c->pfree, c->pst.zmfull, c->pdd32, v,
+      c->red_part + 3 * NNPP_RED_GRID, op.g32, c->red_part, c->scal, 1, 0.0, 1);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }

@@ -79,6 +79,9 @@ void free_nodes(nnpp_ctx* c) {
   dfree(c->f_z);
   dfree(c->f_p);
   dfree(c->f_q);
+  dfree(c->f_w1);
+  dfree(c->f_s0);
+  dfree(c->f_s1);
   dfree(c->red_part);
 }
 
@@ -276,7 +279,7 @@ int nnpp_set_nodes(nnpp_ctx* c, int32_t n, const double* x, const double* y, con
   NNPP_HIP(c, hipMalloc((void**)&c->psc, sizeof(double) * m));
   double** dv[] = {&c->cg_b, &c->cg_x, &c->cg_r, &c->cg_z, &c->cg_p, &c->cg_q};
   for (double** d : dv) NNPP_HIP(c, hipMalloc((void**)d, sizeof(double) * m));
-  float** fv[] = {&c->f_e, &c->f_r, &c->f_z, &c->f_p, &c->f_q};
+  float** fv[] = {&c->f_e, &c->f_r, &c->f_z, &c->f_p, &c->f_q, &c->f_w1, &c->f_s0, &c->f_s1};
   for (float** f : fv) NNPP_HIP(c, hipMalloc((void**)f, sizeof(float) * m));
   const int64_t nblk = std::max<int64_t>(nnpp_nblocks(n), 4096);
   NNPP_HIP(c, hipMalloc((void**)&c->red_part, sizeof(double) * 4 * nblk));
@@ -944,6 +947,7 @@ int nnpp_time_kernel(nnpp_ctx* c, int32_t kernel, int32_t precision, int32_t rep
     if (r == 1) NNPP_HIP(c, hipEventRecord(e0, s));
     if (kernel == NNPP_KERNEL_FLUX) st = nnpp_launch_flux(c, 0, precision, BV, U, U, c->Unew);
     else if (kernel == NNPP_KERNEL_POT_SPMV) st = nnpp_pot_spmv_launch(c, precision);
+    else if (kernel == NNPP_KERNEL_CG_ITER) st = nnpp_cg_iter_launch(c, precision);
     else st = NNPP_ERR_ARG;
     if (st) break;
   }

@@ -245,6 +245,9 @@ struct nnpp_ctx {
   float* f_z = nullptr;
   float* f_p = nullptr;
   float* f_q = nullptr;
+  float* f_w1 = nullptr;     // single-reduction FP32 CG: second buffers of w and s (A p)
+  float* f_s0 = nullptr;
+  float* f_s1 = nullptr;
   int64_t pot_nfree = 0;
 
   // reductions: per-block partials + scalars; pinned host mirror of the scalars
@@ -295,6 +298,9 @@ enum CgScalar {
   SC_RHO = 0, SC_ALPHA, SC_BETA, SC_PQ, SC_RR, SC_TARGET2, SC_DONE, SC_ITERS, SC_BNORM2,
   SC_R0, SC_R1, SC_R2, SC_R3, SC_RRMAX,
   SC_RHO1,  // second slot of rho: CG iterations alternate between SC_RHO and SC_RHO1
+  // single-reduction FP32 CG (z-marching kernels): gamma = r.u and alpha of the last
+  // iteration, one pair per launch parity, and the parity of the launch that stopped
+  SC_CG_GAMMA0, SC_CG_ALPHA0, SC_CG_GAMMA1, SC_CG_ALPHA1, SC_CG_PARDONE,
   SC_COUNT
 };
 
@@ -560,6 +566,7 @@ int nnpp_front_dev(nnpp_ctx* c, double* U, double thr, int do_vol, int32_t* flip
 int nnpp_bc_override_dev(nnpp_ctx* c, const double* U1, double* Fo);
 int nnpp_check_finite_dev(nnpp_ctx* c, const double* U, int32_t* bad);
 int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision);
+int nnpp_cg_iter_launch(nnpp_ctx* c, int precision);
 // Tables of the z-marching potential SpMV for the current free flags (box lattices).
 int nnpp_zm_prepare(nnpp_ctx* c);
 // Address of the no-allocation scratch array on the context's device (current device).
