@@ -513,14 +513,21 @@ __device__ __forceinline__ ZmEmit<F> zm_emit(F f) {
 // the box only), pre = emit.pre(i) loaded a plane ahead. All threads of the block must call it
 // (barriers inside). full[tile * nz + z]: every staged point of plane z of the tile is a free
 // node (no weight needs its m_j).
-template <typename T, typename Src, typename Emit>
+//
+// NOMASK: the input vector is 0 at every node that is not free (the CG vectors D^-1 r and
+// their combinations), and dd is the diagonal less the weights of the row's bonds to non-free
+// nodes and to in-plane positions outside the box (pdd32nf): every bond then takes its weight
+// unmasked (the bonds to those nodes add w (p_i - 0), which the diagonal takes back), only the
+// values are staged, and every plane runs the unmasked instance.
+template <typename T, bool NOMASK = false, typename Src, typename Emit>
 __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& ball,
                                         const uint8_t* __restrict__ pfree,
                                         const uint8_t* __restrict__ full_tab,
                                         const T* __restrict__ dd, const Src& src, Emit& emit) {
   using Raw = typename Src::Raw;
   using Pre = typename Emit::Pre;
-  __shared__ ZmPair<T> lds[2][ZM_PTS];
+  __shared__ ZmPair<T> lds[NOMASK ? 1 : 2][NOMASK ? 1 : ZM_PTS];
+  __shared__ T ldsv[NOMASK ? 2 : 1][NOMASK ? ZM_PTS : 1];
   const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
   const int zc = (int)(item % g.nzc);
   const int64_t tile = item / g.nzc;
@@ -580,7 +587,7 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
 #pragma unroll
     for (int k = 0; k < ZM_STAGE; k++) {
       sv[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : Raw{};
-      sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
+      if (!NOMASK) sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
     }
   };
   fetch(zs);
@@ -608,17 +615,23 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     for (int k = 0; k < ZM_STAGE; k++) {
       const int e = threadIdx.x + k * NNPP_BLOCK;
       if (e < ZM_PTS) {
-        ZmPair<T> pr;
-        pr.v = src.value(sv[k]);
-        pr.m = sm[k] ? T(1) : T(0);
-        lds[buf][e] = pr;
+        if (NOMASK) {
+          ldsv[buf][e] = src.value(sv[k]);
+        } else {
+          ZmPair<T> pr;
+          pr.v = src.value(sv[k]);
+          pr.m = sm[k] ? T(1) : T(0);
+          lds[buf][e] = pr;
+        }
       }
     }
-    const bool full = nnpp_ld_table(ftab, zi) != 0;
+    const bool full = NOMASK || nnpp_ld_table(ftab, zi) != 0;
     // this plane staged (and the buffer written two planes ago read by every thread)
     __syncthreads();
     if (zi + 1 < ze) fetch(zi + 1);
-    const ZmPair<T>* __restrict__ me = &lds[buf][(ty + ZM_M) * ZM_W + tx + ZM_M];
+    const int ome = (ty + ZM_M) * ZM_W + tx + ZM_M;
+    const ZmPair<T>* __restrict__ me = &lds[NOMASK ? 0 : buf][NOMASK ? 0 : ome];
+    const T* __restrict__ mev = &ldsv[NOMASK ? buf : 0][NOMASK ? ome : 0];
     // slot q (output plane zi - M + q) takes the bonds with dz = M - q
     bool act[ZM_SL];
 #pragma unroll
@@ -636,7 +649,14 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
         ZmPair<T> lv[2 * ZM_M + 1];
 #pragma unroll
         for (int dx = -ZM_M; dx <= ZM_M; dx++)
-          if (dx * dx + dy * dy <= ZM_M * ZM_M) lv[dx + ZM_M] = me[dy * ZM_W + dx];
+          if (dx * dx + dy * dy <= ZM_M * ZM_M) {
+            if (NOMASK) {
+              lv[dx + ZM_M].v = mev[dy * ZM_W + dx];
+              lv[dx + ZM_M].m = T(1);
+            } else {
+              lv[dx + ZM_M] = me[dy * ZM_W + dx];
+            }
+          }
 #pragma unroll
         for (int q = 0; q < ZM_SL; q++) {
           const int dz = ZM_M - q;
@@ -691,6 +711,33 @@ __global__ void k_zm_full(ZmGeom g, const uint8_t* __restrict__ pfree, uint8_t* 
         break;
       }
   full[k] = all;
+}
+
+// The diagonal of the unmasked z-marching walk (NOMASK): per free row, pdd32 less the FP32
+// weights (summed in FP64) of its bonds, within the box's z range, to nodes that are not free
+// or to in-plane positions outside the box; rows that are not free keep pdd32 (their input
+// value is 0).
+__global__ void k_zm_ddnf(ZmGeom g, nnpp_ball_args ball, const uint8_t* __restrict__ pfree,
+                          const float* __restrict__ dd, float* __restrict__ ddnf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.sz * g.nz) return;
+  if (!pfree[i]) {
+    ddnf[i] = dd[i];
+    return;
+  }
+  const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / g.sz);
+  constexpr nnpp_ball<NNPP_BALL_M> B{};
+  double W = 0.0;
+#pragma unroll
+  for (int t = 0; t < B.K; t++) {
+    const int l = B.line[t];
+    const int zj = z + B.ldz[l], yj = y + B.ldy[l], xj = x + B.dx[t];
+    if (zj < 0 || zj >= g.nz) continue;  // planes outside the box are never walked
+    const bool out = xj < 0 || xj >= g.nx || yj < 0 || yj >= g.ny;
+    if (out || !pfree[(int64_t)zj * g.sz + (int64_t)yj * g.nx + xj])
+      W += (double)ball.w32[B.d2[t]];
+  }
+  ddnf[i] = (float)((double)dd[i] - W);
 }
 
 // FP32 CG SpMV: q = M p, partial p.q (FP64 over FP32 products).
@@ -893,7 +940,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_arg
       v[2] += (double)ri * (double)ri;
     }
   } em{src, vec, v};
-  zm_walk<float>(g, ball, pfree, full, dd, src, em);
+  zm_walk<float, true>(g, ball, pfree, full, dd, src, em);
   nnpp_block_partial<3>(v, part_out);
 }
 
@@ -1654,12 +1701,12 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
                         c->f_p, c->f_e, c->pinv32};
         if (restart)
           k_zm_m_cgs<true><<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull,
-                                                         c->pdd32, v, Pv[q], op.g32, Pv[q ^ 1],
+                                                         c->pdd32nf, v, Pv[q], op.g32, Pv[q ^ 1],
                                                          c->scal, q, drop2, 0);
         else
           k_zm_m_cgs<false><<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull,
-                                                          c->pdd32, v, Pv[q], op.g32, Pv[q ^ 1],
-                                                          c->scal, q, drop2, 0);
+                                                          c->pdd32nf, v, Pv[q], op.g32,
+                                                          Pv[q ^ 1], c->scal, q, drop2, 0);
       };
       k_zm_m_replace<true><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, Rv[0], c->f_e, c->f_p,
                                                     Sv[0], Sv[1], part, c->scal);
@@ -1799,6 +1846,9 @@ int nnpp_zm_prepare(nnpp_ctx* c) {
   if (m)
     k_zm_full<<<(m + NNPP_BLOCK - 1) / NNPP_BLOCK, NNPP_BLOCK, 0, c->stream>>>(g, c->pfree,
                                                                               T.zmfull);
+  if (c->n)
+    k_zm_ddnf<<<nnpp_nblocks(c->n), NNPP_BLOCK, 0, c->stream>>>(g, T.ball_args, c->pfree,
+                                                                c->pdd32, c->pdd32nf);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;
 }
@@ -1823,7 +1873,7 @@ int nnpp_cg_iter_launch(nnpp_ctx* c, int precision) {
                           "the fused CG iteration runs in mixed precision on box lattices only");
   const ZmCgVec v{c->f_z, c->f_w1, c->f_s1, c->f_r, c->f_q, c->f_s0, c->f_p, c->f_e, c->pinv32};
   k_zm_m_cgs<false><<<op.g32, NNPP_BLOCK, 0, c->stream>>>(
-      op.zg, c->pst.ball_args, c->pfree, c->pst.zmfull, c->pdd32, v,
+      op.zg, c->pst.ball_args, c->pfree, c->pst.zmfull, c->pdd32nf, v,
       c->red_part + 3 * NNPP_RED_GRID, op.g32, c->red_part, c->scal, 1, 0.0, 1);
   NNPP_HIP(c, hipGetLastError());
   return NNPP_OK;

@@ -64,6 +64,7 @@ void free_nodes(nnpp_ctx* c) {
   dfree(c->pfree);
   dfree(c->pdd);
   dfree(c->pdd32);
+  dfree(c->pdd32nf);
   dfree(c->pinv);
   dfree(c->pinv32);
   dfree(c->pbdir);
@@ -273,6 +274,7 @@ int nnpp_set_nodes(nnpp_ctx* c, int32_t n, const double* x, const double* y, con
   NNPP_HIP(c, hipMalloc((void**)&c->pfree, m));
   NNPP_HIP(c, hipMalloc((void**)&c->pdd, sizeof(double) * m));
   NNPP_HIP(c, hipMalloc((void**)&c->pdd32, sizeof(float) * m));
+  NNPP_HIP(c, hipMalloc((void**)&c->pdd32nf, sizeof(float) * m));
   NNPP_HIP(c, hipMalloc((void**)&c->pinv, sizeof(double) * m));
   NNPP_HIP(c, hipMalloc((void**)&c->pinv32, sizeof(float) * m));
   NNPP_HIP(c, hipMalloc((void**)&c->pbdir, sizeof(double) * m));

@@ -230,6 +230,8 @@ struct nnpp_ctx {
   uint8_t* pfree = nullptr;
   double* pdd = nullptr;     // scaled weight sum of Dirichlet liquid neighbours (FP64)
   float* pdd32 = nullptr;
+  float* pdd32nf = nullptr;  // box lattice: pdd32 less the weights of the row's bonds to
+                             // non-free or out-of-box nodes in the box's z range (FP32 CG)
   double* pinv = nullptr;    // inverse of the scaled diagonal, 0 on non-free rows
   float* pinv32 = nullptr;
   double* pbdir = nullptr;   // Dirichlet part of the right-hand side (unscaled)
