@@ -1001,7 +1001,8 @@ static int build_stencil(nnpp_ctx* c) {
       NNPP_HIP(c, hipMemcpyAsync(&hb, bf, sizeof(int), hipMemcpyDeviceToHost, s));
       NNPP_HIP(c, hipStreamSynchronize(s));
       hipFree(bf);
-      if (!hb && ny > 2 * NNPP_BALL_M && nz >= 1) {
+      // (the z-marching kernels address nodes by 32-bit byte offsets: n * 8 < 2^32)
+      if (!hb && ny > 2 * NNPP_BALL_M && nz >= 1 && (int64_t)n * 8 < ((int64_t)1 << 32)) {
         T.box = 1;
         T.nx = nx;
         T.ny = ny;

@@ -433,6 +433,24 @@ constexpr int ZM_W = ZM_TX + 2 * ZM_M, ZM_H = ZM_TY + 2 * ZM_M;
 constexpr int ZM_PTS = ZM_W * ZM_H;
 constexpr int ZM_STAGE = (ZM_PTS + NNPP_BLOCK - 1) / NNPP_BLOCK;  // staged points per thread
 constexpr int ZM_SL = 2 * ZM_M + 1;                               // rows in flight
+// 16-byte staging (VEC, nx a multiple of 4): each staged row of the plane starts at x0 - 4 and
+// is ZM_W4 wide, whole quads of 4 nodes, one quad per thread
+constexpr int ZM_X4 = 4;  // the staged row's first column relative to x0 (a whole quad)
+constexpr int ZM_W4 = ZM_TX + 2 * ZM_X4;
+constexpr int ZM_Q4 = ZM_W4 / 4 * ZM_H;  // quads per plane
+static_assert(ZM_X4 >= ZM_M && ZM_Q4 <= NNPP_BLOCK, "one staged quad per thread");
+
+// Loads and stores at a 32-bit byte offset from the array's start (node indices times the
+// element size stay below 2^32 on the z-marching geometries), so the address is a scalar base
+// plus a vector offset.
+template <typename T>
+__device__ __forceinline__ T zm_ld(const T* __restrict__ p, uint32_t j) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + j * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void zm_st(T* __restrict__ p, uint32_t j, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + j * (uint32_t)sizeof(T)) = v;
+}
 
 struct ZmGeom {
   int nx, ny, nz;
@@ -519,15 +537,58 @@ __device__ __forceinline__ ZmEmit<F> zm_emit(F f) {
 // nodes and to in-plane positions outside the box (pdd32nf): every bond then takes its weight
 // unmasked (the bonds to those nodes add w (p_i - 0), which the diagonal takes back), only the
 // values are staged, and every plane runs the unmasked instance.
-template <typename T, bool NOMASK = false, typename Src, typename Emit>
+//
+// NOMASK in FP32 takes the plane's bonds by rings. Around the thread's column (value p_c in
+// input plane z_i) the disc dx^2 + dy^2 <= M^2 splits into rings of equal k = dx^2 + dy^2, and
+// the weight of a bond depends on k + dz^2 only, so with R_k = sum over ring k of (p_j - p_c)
+// and W_dz the weight sum of the bonds of plane distance dz, the row of output plane z_o takes
+//   sum over the plane's bonds of w (p_o - p_j) = W_dz (p_o - p_c) - sum_k w(k + dz^2) R_k,
+// dz = z_i - z_o. The ring sums are formed once per plane and shared by the rows in flight:
+// per plane and thread 28 subtractions and 22 additions for the rings and at most 7
+// multiply-adds per row, against a subtraction and a multiply-add per bond (122). Every
+// difference is a local one (points within the horizon), as in the bond-by-bond sum.
+template <int M>
+__device__ __forceinline__ constexpr bool zm_ring_first(int dx, int dy) {
+  // (dx, dy) is the first point of its ring in the walk order (dy outer, dx inner, ascending)
+  for (int y = -M; y <= M; y++)
+    for (int x = -M; x <= M; x++) {
+      if (x * x + y * y != dx * dx + dy * dy) continue;
+      return x == dx && y == dy;
+    }
+  return false;
+}
+template <int M>
+__device__ __forceinline__ constexpr bool zm_ring_exists(int k) {
+  for (int y = -M; y <= M; y++)
+    for (int x = -M; x <= M; x++)
+      if (x * x + y * y == k) return true;
+  return false;
+}
+
+template <typename Src, bool VEC>
+struct ZmRaw4 {
+  using type = int;
+};
+template <typename Src>
+struct ZmRaw4<Src, true> {
+  using type = typename Src::Raw4;
+};
+
+// VEC (with the ring form; nx a multiple of 4): the plane is staged in 16-byte quads through
+// Src::load4 / value4.
+template <typename T, bool NOMASK = false, bool VEC = false, typename Src, typename Emit>
 __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& ball,
                                         const uint8_t* __restrict__ pfree,
                                         const uint8_t* __restrict__ full_tab,
                                         const T* __restrict__ dd, const Src& src, Emit& emit) {
   using Raw = typename Src::Raw;
   using Pre = typename Emit::Pre;
+  constexpr bool RING = NOMASK && std::is_same<T, float>::value;
+  static_assert(!VEC || RING, "16-byte staging is for the FP32 ring form");
+  constexpr int LW = VEC ? ZM_W4 : ZM_W;  // staged row width in LDS
+  constexpr int LX = VEC ? ZM_X4 : ZM_M;  // the thread's column within the staged row
   __shared__ ZmPair<T> lds[NOMASK ? 1 : 2][NOMASK ? 1 : ZM_PTS];
-  __shared__ T ldsv[NOMASK ? 2 : 1][NOMASK ? ZM_PTS : 1];
+  __shared__ __align__(16) T ldsv[NOMASK ? 2 : 1][NOMASK ? (VEC ? LW * ZM_H : ZM_PTS) : 1];
   const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
   const int zc = (int)(item % g.nzc);
   const int64_t tile = item / g.nzc;
@@ -537,13 +598,22 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
   const int tx = threadIdx.x % ZM_TX, ty = threadIdx.x / ZM_TX;
   const int x = x0 + tx, y = y0 + ty;
   const bool col_ok = x < g.nx && y < g.ny;
-  // this thread's staged points: in-plane offset (-1: outside the box or past the tile)
+  // this thread's staged points: in-plane offset (-1: outside the box or past the tile); VEC:
+  // the first node of its quad (a quad lies wholly inside or outside the box: nx % 4 == 0)
   int32_t soff[ZM_STAGE];
 #pragma unroll
   for (int k = 0; k < ZM_STAGE; k++) {
-    const int e = threadIdx.x + k * NNPP_BLOCK;
-    const int gx = x0 + e % ZM_W - ZM_M, gy = y0 + e / ZM_W - ZM_M;
-    soff[k] = (e < ZM_PTS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx : -1;
+    if (VEC) {
+      const int e = threadIdx.x;
+      const int gx = x0 + (e % (ZM_W4 / 4)) * 4 - ZM_X4, gy = y0 + e / (ZM_W4 / 4) - ZM_M;
+      soff[k] = (k == 0 && e < ZM_Q4 && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny)
+                    ? gy * g.nx + gx : -1;
+    } else {
+      const int e = threadIdx.x + k * NNPP_BLOCK;
+      const int gx = x0 + e % ZM_W - ZM_M, gy = y0 + e / ZM_W - ZM_M;
+      soff[k] = (e < ZM_PTS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx
+                                                                             : -1;
+    }
   }
   const uint8_t* __restrict__ ftab = full_tab + tile * g.nz;
   const int zs = max(z0 - ZM_M, 0), ze = min(zend + ZM_M, g.nz);  // input planes [zs, ze)
@@ -573,24 +643,46 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
   const auto load_first = [&](int zo) {
     if (col_ok && in_chunk(zo)) {
       const uint32_t i = (uint32_t)zo * sz + col;
-      d0 = dd[i];
-      f0 = pfree[i] != 0;
+      d0 = zm_ld(dd, i);
+      f0 = zm_ld(pfree, i) != 0;
       pre0 = emit.pre(i);
     }
   };
   load_first(zs - ZM_M);
   // loads of the next plane's staged points
-  Raw sv[ZM_STAGE];
+  constexpr int NSV = VEC ? 1 : ZM_STAGE;
+  using SvRaw = typename std::conditional<VEC, typename ZmRaw4<Src, VEC>::type, Raw>::type;
+  SvRaw sv[NSV];
   uint8_t sm[ZM_STAGE];
   const auto fetch = [&](int zi) {
     const uint32_t base = (uint32_t)zi * sz;
 #pragma unroll
-    for (int k = 0; k < ZM_STAGE; k++) {
-      sv[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : Raw{};
+    for (int k = 0; k < NSV; k++) {
+      if constexpr (VEC)
+        sv[k] = soff[k] >= 0 ? src.load4(base + (uint32_t)soff[k]) : SvRaw{};
+      else
+        sv[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : Raw{};
       if (!NOMASK) sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
     }
   };
   fetch(zs);
+  // ring form: weight sums by plane distance |dz| (FP32, fixed order)
+  T wz[ZM_M + 1];
+  if constexpr (RING) {
+#pragma unroll
+    for (int a = 0; a <= ZM_M; a++) {
+      T s = T(0);
+#pragma unroll
+      for (int dy = -ZM_M; dy <= ZM_M; dy++)
+#pragma unroll
+        for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+          const int d2 = dx * dx + dy * dy + a * a;
+          if (d2 > ZM_M * ZM_M || d2 == 0) continue;
+          s += zm_weight<T>(ball, d2);
+        }
+      wz[a] = s;
+    }
+  }
   // after input plane zi: output plane zi - M is complete; the slots move down by one and the
   // last one takes output plane zi + 1 + M
   const auto finish = [&](int zi) {
@@ -611,6 +703,10 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
   };
   int buf = 0;
   for (int zi = zs; zi < ze; zi++) {
+    if constexpr (VEC) {
+      if (threadIdx.x < ZM_Q4)
+        reinterpret_cast<float4*>(&ldsv[buf][0])[threadIdx.x] = src.value4(sv[0]);
+    } else {
 #pragma unroll
     for (int k = 0; k < ZM_STAGE; k++) {
       const int e = threadIdx.x + k * NNPP_BLOCK;
@@ -625,11 +721,12 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
         }
       }
     }
+    }
     const bool full = NOMASK || nnpp_ld_table(ftab, zi) != 0;
     // this plane staged (and the buffer written two planes ago read by every thread)
     __syncthreads();
     if (zi + 1 < ze) fetch(zi + 1);
-    const int ome = (ty + ZM_M) * ZM_W + tx + ZM_M;
+    const int ome = (ty + ZM_M) * LW + tx + LX;
     const ZmPair<T>* __restrict__ me = &lds[NOMASK ? 0 : buf][NOMASK ? 0 : ome];
     const T* __restrict__ mev = &ldsv[NOMASK ? buf : 0][NOMASK ? ome : 0];
     // slot q (output plane zi - M + q) takes the bonds with dz = M - q
@@ -681,10 +778,40 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     };
     using t_ = std::true_type;
     using f_ = std::false_type;
-    // (an instance without the per-slot tests for the planes where every slot is active
-    // measured slower: its straight-line code spilled at four waves per SIMD)
-    if (full) body(f_{}, f_{});
-    else body(t_{}, f_{});
+    if constexpr (RING) {
+      const T uc = mev[0];
+      T R[ZM_M * ZM_M + 1];
+#pragma unroll
+      for (int dy = -ZM_M; dy <= ZM_M; dy++)
+#pragma unroll
+        for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+          const int k = dx * dx + dy * dy;
+          if (k == 0 || k > ZM_M * ZM_M) continue;
+          const T d = mev[dy * LW + dx] - uc;
+          R[k] = zm_ring_first<ZM_M>(dx, dy) ? d : R[k] + d;
+        }
+#pragma unroll
+      for (int q = 0; q < ZM_SL; q++) {
+        if (!act[q]) continue;  // uniform over the block
+        const int dz = ZM_M - q, a = dz < 0 ? -dz : dz;
+        // dz = 0: the row is the column itself (p_o - p_c = 0)
+        T part = a ? wz[a] * (po[q] - uc) : T(0);
+        bool first = a == 0;
+#pragma unroll
+        for (int k = 1; k <= ZM_M * ZM_M; k++) {
+          if (!zm_ring_exists<ZM_M>(k) || k + a * a > ZM_M * ZM_M) continue;
+          const T w = zm_weight<T>(ball, k + a * a);
+          part = first ? -w * R[k] : __fmaf_rn(-w, R[k], part);
+          first = false;
+        }
+        row[q].fold(part);
+      }
+    } else {
+      // (an instance without the per-slot tests for the planes where every slot is active
+      // measured slower: its straight-line code spilled at four waves per SIMD)
+      if (full) body(f_{}, f_{});
+      else body(t_{}, f_{});
+    }
     finish(zi);
     buf ^= 1;
   }
@@ -740,11 +867,37 @@ __global__ void k_zm_ddnf(ZmGeom g, nnpp_ball_args ball, const uint8_t* __restri
   ddnf[i] = (float)((double)dd[i] - W);
 }
 
-// FP32 CG SpMV: q = M p, partial p.q (FP64 over FP32 products).
+// A stored vector taken as 0 at the nodes that are not free (the input of the unmasked walk).
+struct ZmLoadFree {
+  struct Raw {
+    float v;
+    uint32_t f;
+  };
+  struct Raw4 {
+    float4 v;
+    uint32_t f;  // the 4 free flags, one per byte
+  };
+  const float* __restrict__ v;
+  const uint8_t* __restrict__ pfree;
+  __device__ __forceinline__ Raw load(uint32_t j) const { return Raw{zm_ld(v, j), zm_ld(pfree, j)}; }
+  __device__ __forceinline__ float value(const Raw& r) const { return r.f ? r.v : 0.0f; }
+  __device__ __forceinline__ Raw4 load4(uint32_t j) const {
+    return Raw4{zm_ld(reinterpret_cast<const float4*>(v), j >> 2),
+                zm_ld(reinterpret_cast<const uint32_t*>(pfree), j >> 2)};
+  }
+  __device__ __forceinline__ float4 value4(const Raw4& r) const {
+    return make_float4((r.f & 0xffu) ? r.v.x : 0.0f, (r.f & 0xff00u) ? r.v.y : 0.0f,
+                       (r.f & 0xff0000u) ? r.v.z : 0.0f, (r.f & 0xff000000u) ? r.v.w : 0.0f);
+  }
+};
+
+// FP32 SpMV: q = M p on the free rows (0 elsewhere), partial p.q (FP64 over FP32 products).
+// The unmasked walk (rings, dd = pdd32nf) on p with its values at non-free nodes taken as 0.
+template <bool VEC>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_spmv(ZmGeom g, nnpp_ball_args ball,
                                                           const uint8_t* __restrict__ pfree,
                                                           const uint8_t* __restrict__ full,
-                                                          const float* __restrict__ dd,
+                                                          const float* __restrict__ ddnf,
                                                           const float* __restrict__ p,
                                                           float* __restrict__ q,
                                                           double* __restrict__ part,
@@ -752,10 +905,10 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_spmv(ZmGeom g, nnpp_ball_ar
   if (cg_done(scal)) return;
   double v[1] = {0.0};
   auto em = zm_emit([&](int64_t i, float y, float pi) {
-    q[i] = y;
+    zm_st(q, (uint32_t)i, y);
     v[0] += (double)pi * (double)y;
   });
-  zm_walk<float>(g, ball, pfree, full, dd, ZmLoad<float>{p}, em);
+  zm_walk<float, true, VEC>(g, ball, pfree, full, ddnf, ZmLoadFree{p, pfree}, em);
   nnpp_block_partial<1>(v, part);
 }
 
@@ -843,7 +996,7 @@ struct ZmCgVec {
 };
 
 // timing: no stop test and no scalar writes (the kernel timing hook launches it repeatedly)
-template <bool RESTART>
+template <bool RESTART, bool VEC>
 __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_args ball,
                                                          const uint8_t* __restrict__ pfree,
                                                          const uint8_t* __restrict__ full,
@@ -891,18 +1044,35 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_arg
     struct Raw {
       float r, w, s, d;
     };
+    struct Raw4 {
+      float4 r, w, s, d;
+    };
     const float* __restrict__ r_in;
     const float* __restrict__ w_in;
     const float* __restrict__ s_in;
     const float* __restrict__ pinv;
     float alpha, beta;
     __device__ __forceinline__ Raw load(uint32_t j) const {
-      return Raw{r_in[j], RESTART ? 0.0f : w_in[j], RESTART ? 0.0f : s_in[j], pinv[j]};
+      return Raw{zm_ld(r_in, j), RESTART ? 0.0f : zm_ld(w_in, j),
+                 RESTART ? 0.0f : zm_ld(s_in, j), zm_ld(pinv, j)};
     }
-    __device__ __forceinline__ float r_new(const Raw& a) const {
-      return RESTART ? a.r : __fmaf_rn(-alpha, __fmaf_rn(beta, a.s, a.w), a.r);
+    __device__ __forceinline__ float r_new(float r, float w, float s) const {
+      return RESTART ? r : __fmaf_rn(-alpha, __fmaf_rn(beta, s, w), r);
     }
+    __device__ __forceinline__ float r_new(const Raw& a) const { return r_new(a.r, a.w, a.s); }
     __device__ __forceinline__ float value(const Raw& a) const { return a.d * r_new(a); }
+    __device__ __forceinline__ Raw4 load4(uint32_t j) const {
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const uint32_t q = j >> 2;
+      return Raw4{zm_ld(reinterpret_cast<const float4*>(r_in), q),
+                  RESTART ? z : zm_ld(reinterpret_cast<const float4*>(w_in), q),
+                  RESTART ? z : zm_ld(reinterpret_cast<const float4*>(s_in), q),
+                  zm_ld(reinterpret_cast<const float4*>(pinv), q)};
+    }
+    __device__ __forceinline__ float4 value4(const Raw4& a) const {
+      return make_float4(a.d.x * r_new(a.r.x, a.w.x, a.s.x), a.d.y * r_new(a.r.y, a.w.y, a.s.y),
+                         a.d.z * r_new(a.r.z, a.w.z, a.s.z), a.d.w * r_new(a.r.w, a.w.w, a.s.w));
+    }
   } src{vec.r_in, vec.w_in, vec.s_in, vec.pinv, alpha, beta};
   // rows: the stored vectors' old values, loaded a plane ahead
   double v[3] = {0.0, 0.0, 0.0};
@@ -917,30 +1087,31 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_arg
     __device__ __forceinline__ Pre pre(uint32_t i) const {
       Pre q;
       q.a = src.load(i);
-      q.a.s = vec.s_in[i];  // restart: s is copied
-      q.p = RESTART ? 0.0f : vec.p[i];
-      q.e = RESTART ? 0.0f : vec.e[i];
+      q.a.s = zm_ld(vec.s_in, i);  // restart: s is copied
+      q.p = RESTART ? 0.0f : zm_ld((const float*)vec.p, i);
+      q.e = RESTART ? 0.0f : zm_ld((const float*)vec.e, i);
       return q;
     }
-    __device__ __forceinline__ void operator()(int64_t i, float y, float ui, const Pre& q) {
+    __device__ __forceinline__ void operator()(int64_t i64, float y, float ui, const Pre& q) {
+      const uint32_t i = (uint32_t)i64;
       const float ri = src.r_new(q.a);
       if (RESTART) {
-        vec.s_out[i] = q.a.s;
+        zm_st(vec.s_out, i, q.a.s);
       } else {
         const float si = __fmaf_rn(src.beta, q.a.s, q.a.w);
         const float pi_ = __fmaf_rn(src.beta, q.p, q.a.d * q.a.r);
-        vec.p[i] = pi_;
-        vec.e[i] = __fmaf_rn(src.alpha, pi_, q.e);
-        vec.s_out[i] = si;
+        zm_st(vec.p, i, pi_);
+        zm_st(vec.e, i, __fmaf_rn(src.alpha, pi_, q.e));
+        zm_st(vec.s_out, i, si);
       }
-      vec.r_out[i] = ri;
-      vec.w_out[i] = y;
+      zm_st(vec.r_out, i, ri);
+      zm_st(vec.w_out, i, y);
       v[0] += (double)ri * (double)ui;
       v[1] += (double)ui * (double)y;
       v[2] += (double)ri * (double)ri;
     }
   } em{src, vec, v};
-  zm_walk<float, true>(g, ball, pfree, full, dd, src, em);
+  zm_walk<float, true, VEC>(g, ball, pfree, full, dd, src, em);
   nnpp_block_partial<3>(v, part_out);
 }
 
@@ -1472,6 +1643,10 @@ ZmGeom zm_geom(const nnpp_stencil& T) {
   return g;
 }
 
+// 16-byte staging of the FP32 ring kernels: every staged quad lies wholly inside or outside the
+// box and starts 16-byte aligned.
+bool zm_vec4(const ZmGeom& g) { return g.nx % 4 == 0; }
+
 // Blocks of the stencil SpMV kernels that fit on the device at once (LDS bounds them): a grid of
 // exactly that many blocks walks the tiles in one round, where a larger grid would leave a
 // partly filled second round. A multiple of 8 (one share per XCD), at most nnpp_red_grid.
@@ -1699,14 +1874,11 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
       const auto launch = [&](bool restart, int q) {
         const ZmCgVec v{Rv[q], Wv[q], Sv[q], Rv[q ^ 1], Wv[q ^ 1], Sv[q ^ 1],
                         c->f_p, c->f_e, c->pinv32};
-        if (restart)
-          k_zm_m_cgs<true><<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull,
-                                                         c->pdd32nf, v, Pv[q], op.g32, Pv[q ^ 1],
-                                                         c->scal, q, drop2, 0);
-        else
-          k_zm_m_cgs<false><<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull,
-                                                          c->pdd32nf, v, Pv[q], op.g32,
-                                                          Pv[q ^ 1], c->scal, q, drop2, 0);
+        const bool v4 = zm_vec4(op.zg);
+        auto kern = restart ? (v4 ? k_zm_m_cgs<true, true> : k_zm_m_cgs<true, false>)
+                            : (v4 ? k_zm_m_cgs<false, true> : k_zm_m_cgs<false, false>);
+        kern<<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd32nf, v,
+                                          Pv[q], op.g32, Pv[q ^ 1], c->scal, q, drop2, 0);
       };
       k_zm_m_replace<true><<<g, NNPP_BLOCK, 0, s>>>(n, c->cg_r, sb, Rv[0], c->f_e, c->f_p,
                                                     Sv[0], Sv[1], part, c->scal);
@@ -1782,7 +1954,7 @@ int nnpp_potential_solve_dev(nnpp_ctx* c, int precision, const double* Uq, doubl
         const int batch = plan.next(rr_b, stop_b, left, total);
         for (int it = 0; it < batch; it++, par ^= 1) {
           if (op.zm)
-            k_zm_m_spmv<<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd32, c->f_p,
+            (zm_vec4(op.zg) ? k_zm_m_spmv<true> : k_zm_m_spmv<false>)<<<op.g32, NNPP_BLOCK, 0, s>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd32nf, c->f_p,
                                                       c->f_q, part_s, scal);
           else
             launch_m_spmv(var, op.lds32, op.g32, s, n, sl, op.w32, c->pdd32, c->f_p, c->f_q,
@@ -1872,7 +2044,7 @@ int nnpp_cg_iter_launch(nnpp_ctx* c, int precision) {
     return nnpp_set_error(c, NNPP_ERR_ARG,
                           "the fused CG iteration runs in mixed precision on box lattices only");
   const ZmCgVec v{c->f_z, c->f_w1, c->f_s1, c->f_r, c->f_q, c->f_s0, c->f_p, c->f_e, c->pinv32};
-  k_zm_m_cgs<false><<<op.g32, NNPP_BLOCK, 0, c->stream>>>(
+  (zm_vec4(op.zg) ? k_zm_m_cgs<false, true> : k_zm_m_cgs<false, false>)<<<op.g32, NNPP_BLOCK, 0, c->stream>>>(
       op.zg, c->pst.ball_args, c->pfree, c->pst.zmfull, c->pdd32nf, v,
       c->red_part + 3 * NNPP_RED_GRID, op.g32, c->red_part, c->scal, 1, 0.0, 1);
   NNPP_HIP(c, hipGetLastError());
@@ -1889,7 +2061,7 @@ int nnpp_pot_spmv_launch(nnpp_ctx* c, int precision) {
   const int var = op.var;
   const nnpp_ball_args& ball = c->pst.ball_args;
   if (op.zm && precision == NNPP_MIXED)
-    k_zm_m_spmv<<<op.g32, NNPP_BLOCK, 0, c->stream>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd32, c->f_p,
+    (zm_vec4(op.zg) ? k_zm_m_spmv<true> : k_zm_m_spmv<false>)<<<op.g32, NNPP_BLOCK, 0, c->stream>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd32nf, c->f_p,
                                                       c->f_q, c->red_part, (const double*)c->scal);
   else if (op.zm)
     k_zm_cg_spmv<<<op.g64, NNPP_BLOCK, 0, c->stream>>>(op.zg, ball, c->pfree, c->pst.zmfull, c->pdd, c->cg_p,
