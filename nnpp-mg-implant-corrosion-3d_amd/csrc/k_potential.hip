@@ -649,23 +649,32 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     }
   };
   load_first(zs - ZM_M);
-  // loads of the next plane's staged points
+  // loads of the staged points: the next plane's (one value per staged point), or VEC the
+  // quads of the next two planes (svq[0] for the planes zs, zs + 2, ..., svq[1] for the others)
   constexpr int NSV = VEC ? 1 : ZM_STAGE;
   using SvRaw = typename std::conditional<VEC, typename ZmRaw4<Src, VEC>::type, Raw>::type;
   SvRaw sv[NSV];
+  SvRaw svq[VEC ? 2 : 1];
   uint8_t sm[ZM_STAGE];
   const auto fetch = [&](int zi) {
     const uint32_t base = (uint32_t)zi * sz;
 #pragma unroll
     for (int k = 0; k < NSV; k++) {
-      if constexpr (VEC)
-        sv[k] = soff[k] >= 0 ? src.load4(base + (uint32_t)soff[k]) : SvRaw{};
-      else
+      if constexpr (!VEC)
         sv[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : Raw{};
       if (!NOMASK) sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
     }
   };
-  fetch(zs);
+  const auto fetch4 = [&](int zi, SvRaw& dst) {
+    if constexpr (VEC)
+      dst = soff[0] >= 0 ? src.load4((uint32_t)zi * sz + (uint32_t)soff[0]) : SvRaw{};
+  };
+  if constexpr (VEC) {
+    fetch4(zs, svq[0]);
+    if (zs + 1 < ze) fetch4(zs + 1, svq[1]);
+  } else {
+    fetch(zs);
+  }
   // ring form: weight sums by plane distance |dz| (FP32, fixed order)
   T wz[ZM_M + 1];
   if constexpr (RING) {
@@ -702,10 +711,11 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     load_first(zo + 1);
   };
   int buf = 0;
-  for (int zi = zs; zi < ze; zi++) {
+  // input plane zi; VEC: its quads are in cur, which then takes plane zi + 2
+  const auto plane = [&](int zi, SvRaw& cur) {
     if constexpr (VEC) {
       if (threadIdx.x < ZM_Q4)
-        reinterpret_cast<float4*>(&ldsv[buf][0])[threadIdx.x] = src.value4(sv[0]);
+        reinterpret_cast<float4*>(&ldsv[buf][0])[threadIdx.x] = src.value4(cur);
     } else {
 #pragma unroll
     for (int k = 0; k < ZM_STAGE; k++) {
@@ -725,7 +735,11 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     const bool full = NOMASK || nnpp_ld_table(ftab, zi) != 0;
     // this plane staged (and the buffer written two planes ago read by every thread)
     __syncthreads();
-    if (zi + 1 < ze) fetch(zi + 1);
+    if constexpr (VEC) {
+      if (zi + 2 < ze) fetch4(zi + 2, cur);
+    } else {
+      if (zi + 1 < ze) fetch(zi + 1);
+    }
     const int ome = (ty + ZM_M) * LW + tx + LX;
     const ZmPair<T>* __restrict__ me = &lds[NOMASK ? 0 : buf][NOMASK ? 0 : ome];
     const T* __restrict__ mev = &ldsv[NOMASK ? buf : 0][NOMASK ? ome : 0];
@@ -814,6 +828,14 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     }
     finish(zi);
     buf ^= 1;
+  };
+  if constexpr (VEC) {
+    for (int zi = zs; zi < ze; zi += 2) {
+      plane(zi, svq[0]);
+      if (zi + 1 < ze) plane(zi + 1, svq[1]);
+    }
+  } else {
+    for (int zi = zs; zi < ze; zi++) plane(zi, svq[0]);
   }
   // planes past the top of the box add nothing: the rows still in flight are complete
   for (int zi = ze; zi < zend + ZM_M; zi++) finish(zi);
