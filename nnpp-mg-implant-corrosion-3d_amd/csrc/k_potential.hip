@@ -15,7 +15,9 @@
 // one-block reduction launch in between), and a converged flag makes the remaining launches of
 // a batch exit at once, so the host synchronises once per batch.
 // MIXED: CG with reliable updates. The iteration runs in FP32 (FP32 weights and vectors,
-// Kahan-compensated bond sums, FP64 dot products) on the residual normalised by ||b||; the
+// Kahan-compensated bond sums, or on box lattices the ring sums of the z-marching walk; FP64
+// dot products) on the residual normalised by ||b||; on box lattices in the single-reduction
+// form, one fused kernel per iteration. The
 // solution is kept in FP64. Whenever the FP32 recursive residual has dropped by inner_rtol (1e-3)
 // (or reached the target), the FP32 correction is added to the FP64 solution, the residual is
 // recomputed in FP64 (FP64 weights and vectors) and replaces the FP32 one, and the iteration
@@ -415,18 +417,18 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
 // Box lattices (the radius-3 ball on the full nx x ny x nz box, numbered x fastest): a block
 // owns a ZM_TX x ZM_TY column tile of (x, y) and a chunk of lz output planes along z. It walks
 // the input planes z0-3 .. z0+lz+2; each plane's tile with a 3-node halo is staged into LDS
-// once, as (p_j, m_j) pairs with m_j = 1 on free nodes and 0 elsewhere (outside the box too).
-// A thread reads the 29-point disc dx^2 + dy^2 <= 9 around its column from LDS once per plane
-// and feeds the (up to 7) output rows of its column that are in flight, held in registers:
-// output z_o takes the bonds with dz = z_i - z_o from input plane z_i. Per row this is 29 LDS
-// reads per plane instead of one per bond.
+// once (FP64: as (p_j, m_j) pairs with m_j = 1 on free nodes and 0 elsewhere, outside the box
+// too). A thread reads the 29-point disc dx^2 + dy^2 <= 9 around its column from LDS once per
+// plane and feeds the (up to 7) output rows of its column that are in flight, held in
+// registers: output z_o takes the bonds with dz = z_i - z_o from input plane z_i. Per row this
+// is 29 LDS reads per plane instead of one per bond.
 //
 // The entries of row i are the bonds to free nodes inside the box (the potential operator's
 // free-free entries; its rows are the free rows): an absent entry takes weight w * m_j = +0,
-// exactly the masked weight of the stencil form. FP64: every row is one fused multiply-add
-// chain over its bonds in ascending offset order (planes in z order, then dy, then dx), the
-// additions of the stencil and column forms, bit for bit. FP32: per row and plane one chain,
-// the plane sums Kahan-compensated (chunks of at most 28 entries).
+// exactly the masked weight of the stencil form. FP64 (zm_walk): every row is one fused
+// multiply-add chain over its bonds in ascending offset order (planes in z order, then dy,
+// then dx), the additions of the stencil and column forms, bit for bit. FP32 (zm_walk_ring):
+// the bonds by rings of equal in-plane distance, one FP32 sum per row.
 constexpr int ZM_TX = 32, ZM_TY = 8;  // a half-wave reads one x-row of 32 (no bank conflicts)
 constexpr int ZM_M = NNPP_BALL_M;
 constexpr int ZM_W = ZM_TX + 2 * ZM_M, ZM_H = ZM_TY + 2 * ZM_M;
@@ -466,7 +468,7 @@ struct ZmPair {
 };
 
 template <typename T>
-struct ZmRow;  // per in-flight row: running sum (FP32: with its Kahan compensation)
+struct ZmRow;  // per in-flight row of the FP64 walk: its running sum
 template <>
 struct ZmRow<double> {
   double s;
@@ -475,21 +477,6 @@ struct ZmRow<double> {
   __device__ __forceinline__ double start() const { return s; }
   __device__ __forceinline__ double result(double dd, double vi) const {
     return __fma_rn(dd, vi, s);
-  }
-};
-template <>
-struct ZmRow<float> {
-  float s, c;
-  __device__ __forceinline__ void reset() { s = c = 0.0f; }
-  __device__ __forceinline__ float start() const { return 0.0f; }  // a chain per plane
-  __device__ __forceinline__ void fold(float part) {
-    const float y = part - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ float result(float dd, float vi) const {
-    return __fmaf_rn(dd, vi, s);
   }
 };
 
@@ -530,65 +517,16 @@ __device__ __forceinline__ ZmEmit<F> zm_emit(F f) {
 // Walks the block's work item and hands every finished row to emit(i, y, p_i, pre) (rows inside
 // the box only), pre = emit.pre(i) loaded a plane ahead. All threads of the block must call it
 // (barriers inside). full[tile * nz + z]: every staged point of plane z of the tile is a free
-// node (no weight needs its m_j).
-//
-// NOMASK: the input vector is 0 at every node that is not free (the CG vectors D^-1 r and
-// their combinations), and dd is the diagonal less the weights of the row's bonds to non-free
-// nodes and to in-plane positions outside the box (pdd32nf): every bond then takes its weight
-// unmasked (the bonds to those nodes add w (p_i - 0), which the diagonal takes back), only the
-// values are staged, and every plane runs the unmasked instance.
-//
-// NOMASK in FP32 takes the plane's bonds by rings. Around the thread's column (value p_c in
-// input plane z_i) the disc dx^2 + dy^2 <= M^2 splits into rings of equal k = dx^2 + dy^2, and
-// the weight of a bond depends on k + dz^2 only, so with R_k = sum over ring k of (p_j - p_c)
-// and W_dz the weight sum of the bonds of plane distance dz, the row of output plane z_o takes
-//   sum over the plane's bonds of w (p_o - p_j) = W_dz (p_o - p_c) - sum_k w(k + dz^2) R_k,
-// dz = z_i - z_o. The ring sums are formed once per plane and shared by the rows in flight:
-// per plane and thread 28 subtractions and 22 additions for the rings and at most 7
-// multiply-adds per row, against a subtraction and a multiply-add per bond (122). Every
-// difference is a local one (points within the horizon), as in the bond-by-bond sum.
-template <int M>
-__device__ __forceinline__ constexpr bool zm_ring_first(int dx, int dy) {
-  // (dx, dy) is the first point of its ring in the walk order (dy outer, dx inner, ascending)
-  for (int y = -M; y <= M; y++)
-    for (int x = -M; x <= M; x++) {
-      if (x * x + y * y != dx * dx + dy * dy) continue;
-      return x == dx && y == dy;
-    }
-  return false;
-}
-template <int M>
-__device__ __forceinline__ constexpr bool zm_ring_exists(int k) {
-  for (int y = -M; y <= M; y++)
-    for (int x = -M; x <= M; x++)
-      if (x * x + y * y == k) return true;
-  return false;
-}
-
-template <typename Src, bool VEC>
-struct ZmRaw4 {
-  using type = int;
-};
-template <typename Src>
-struct ZmRaw4<Src, true> {
-  using type = typename Src::Raw4;
-};
-
-// VEC (with the ring form; nx a multiple of 4): the plane is staged in 16-byte quads through
-// Src::load4 / value4.
-template <typename T, bool NOMASK = false, bool VEC = false, typename Src, typename Emit>
+// node (no weight needs its m_j). The FP64 operator: each row one fused multiply-add chain over
+// its bonds in ascending offset order, the additions of the other forms.
+template <typename T, typename Src, typename Emit>
 __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& ball,
                                         const uint8_t* __restrict__ pfree,
                                         const uint8_t* __restrict__ full_tab,
                                         const T* __restrict__ dd, const Src& src, Emit& emit) {
   using Raw = typename Src::Raw;
   using Pre = typename Emit::Pre;
-  constexpr bool RING = NOMASK && std::is_same<T, float>::value;
-  static_assert(!VEC || RING, "16-byte staging is for the FP32 ring form");
-  constexpr int LW = VEC ? ZM_W4 : ZM_W;  // staged row width in LDS
-  constexpr int LX = VEC ? ZM_X4 : ZM_M;  // the thread's column within the staged row
-  __shared__ ZmPair<T> lds[NOMASK ? 1 : 2][NOMASK ? 1 : ZM_PTS];
-  __shared__ __align__(16) T ldsv[NOMASK ? 2 : 1][NOMASK ? (VEC ? LW * ZM_H : ZM_PTS) : 1];
+  __shared__ ZmPair<T> lds[2][ZM_PTS];
   const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
   const int zc = (int)(item % g.nzc);
   const int64_t tile = item / g.nzc;
@@ -598,22 +536,13 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
   const int tx = threadIdx.x % ZM_TX, ty = threadIdx.x / ZM_TX;
   const int x = x0 + tx, y = y0 + ty;
   const bool col_ok = x < g.nx && y < g.ny;
-  // this thread's staged points: in-plane offset (-1: outside the box or past the tile); VEC:
-  // the first node of its quad (a quad lies wholly inside or outside the box: nx % 4 == 0)
+  // this thread's staged points: in-plane offset (-1: outside the box or past the tile)
   int32_t soff[ZM_STAGE];
 #pragma unroll
   for (int k = 0; k < ZM_STAGE; k++) {
-    if (VEC) {
-      const int e = threadIdx.x;
-      const int gx = x0 + (e % (ZM_W4 / 4)) * 4 - ZM_X4, gy = y0 + e / (ZM_W4 / 4) - ZM_M;
-      soff[k] = (k == 0 && e < ZM_Q4 && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny)
-                    ? gy * g.nx + gx : -1;
-    } else {
-      const int e = threadIdx.x + k * NNPP_BLOCK;
-      const int gx = x0 + e % ZM_W - ZM_M, gy = y0 + e / ZM_W - ZM_M;
-      soff[k] = (e < ZM_PTS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx
-                                                                             : -1;
-    }
+    const int e = threadIdx.x + k * NNPP_BLOCK;
+    const int gx = x0 + e % ZM_W - ZM_M, gy = y0 + e / ZM_W - ZM_M;
+    soff[k] = (e < ZM_PTS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx : -1;
   }
   const uint8_t* __restrict__ ftab = full_tab + tile * g.nz;
   const int zs = max(z0 - ZM_M, 0), ze = min(zend + ZM_M, g.nz);  // input planes [zs, ze)
@@ -649,49 +578,18 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     }
   };
   load_first(zs - ZM_M);
-  // loads of the staged points: the next plane's (one value per staged point), or VEC the
-  // quads of the next two planes (svq[0] for the planes zs, zs + 2, ..., svq[1] for the others)
-  constexpr int NSV = VEC ? 1 : ZM_STAGE;
-  using SvRaw = typename std::conditional<VEC, typename ZmRaw4<Src, VEC>::type, Raw>::type;
-  SvRaw sv[NSV];
-  SvRaw svq[VEC ? 2 : 1];
+  // loads of the next plane's staged points
+  Raw sv[ZM_STAGE];
   uint8_t sm[ZM_STAGE];
   const auto fetch = [&](int zi) {
     const uint32_t base = (uint32_t)zi * sz;
 #pragma unroll
-    for (int k = 0; k < NSV; k++) {
-      if constexpr (!VEC)
-        sv[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : Raw{};
-      if (!NOMASK) sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
+    for (int k = 0; k < ZM_STAGE; k++) {
+      sv[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : Raw{};
+      sm[k] = soff[k] >= 0 ? pfree[base + (uint32_t)soff[k]] : (uint8_t)0;
     }
   };
-  const auto fetch4 = [&](int zi, SvRaw& dst) {
-    if constexpr (VEC)
-      dst = soff[0] >= 0 ? src.load4((uint32_t)zi * sz + (uint32_t)soff[0]) : SvRaw{};
-  };
-  if constexpr (VEC) {
-    fetch4(zs, svq[0]);
-    if (zs + 1 < ze) fetch4(zs + 1, svq[1]);
-  } else {
-    fetch(zs);
-  }
-  // ring form: weight sums by plane distance |dz| (FP32, fixed order)
-  T wz[ZM_M + 1];
-  if constexpr (RING) {
-#pragma unroll
-    for (int a = 0; a <= ZM_M; a++) {
-      T s = T(0);
-#pragma unroll
-      for (int dy = -ZM_M; dy <= ZM_M; dy++)
-#pragma unroll
-        for (int dx = -ZM_M; dx <= ZM_M; dx++) {
-          const int d2 = dx * dx + dy * dy + a * a;
-          if (d2 > ZM_M * ZM_M || d2 == 0) continue;
-          s += zm_weight<T>(ball, d2);
-        }
-      wz[a] = s;
-    }
-  }
+  fetch(zs);
   // after input plane zi: output plane zi - M is complete; the slots move down by one and the
   // last one takes output plane zi + 1 + M
   const auto finish = [&](int zi) {
@@ -711,38 +609,22 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     load_first(zo + 1);
   };
   int buf = 0;
-  // input plane zi; VEC: its quads are in cur, which then takes plane zi + 2
-  const auto plane = [&](int zi, SvRaw& cur) {
-    if constexpr (VEC) {
-      if (threadIdx.x < ZM_Q4)
-        reinterpret_cast<float4*>(&ldsv[buf][0])[threadIdx.x] = src.value4(cur);
-    } else {
+  for (int zi = zs; zi < ze; zi++) {
 #pragma unroll
     for (int k = 0; k < ZM_STAGE; k++) {
       const int e = threadIdx.x + k * NNPP_BLOCK;
       if (e < ZM_PTS) {
-        if (NOMASK) {
-          ldsv[buf][e] = src.value(sv[k]);
-        } else {
-          ZmPair<T> pr;
-          pr.v = src.value(sv[k]);
-          pr.m = sm[k] ? T(1) : T(0);
-          lds[buf][e] = pr;
-        }
+        ZmPair<T> pr;
+        pr.v = src.value(sv[k]);
+        pr.m = sm[k] ? T(1) : T(0);
+        lds[buf][e] = pr;
       }
     }
-    }
-    const bool full = NOMASK || nnpp_ld_table(ftab, zi) != 0;
+    const bool full = nnpp_ld_table(ftab, zi) != 0;
     // this plane staged (and the buffer written two planes ago read by every thread)
     __syncthreads();
-    if constexpr (VEC) {
-      if (zi + 2 < ze) fetch4(zi + 2, cur);
-    } else {
-      if (zi + 1 < ze) fetch(zi + 1);
-    }
-    const int ome = (ty + ZM_M) * LW + tx + LX;
-    const ZmPair<T>* __restrict__ me = &lds[NOMASK ? 0 : buf][NOMASK ? 0 : ome];
-    const T* __restrict__ mev = &ldsv[NOMASK ? buf : 0][NOMASK ? ome : 0];
+    if (zi + 1 < ze) fetch(zi + 1);
+    const ZmPair<T>* __restrict__ me = &lds[buf][(ty + ZM_M) * ZM_W + tx + ZM_M];
     // slot q (output plane zi - M + q) takes the bonds with dz = M - q
     bool act[ZM_SL];
 #pragma unroll
@@ -750,8 +632,8 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
     // MASK: some staged point of the plane is not a free node (weights times m_j). Separate
     // instances, so the common case (free interior planes) is one subtraction and one fused
     // multiply-add per bond.
-    const auto body = [&](auto mask_c, auto all_c) {
-      constexpr bool MASK = decltype(mask_c)::value, ALL = decltype(all_c)::value;
+    const auto body = [&](auto mask_c) {
+      constexpr bool MASK = decltype(mask_c)::value;
       T part[ZM_SL];
 #pragma unroll
       for (int q = 0; q < ZM_SL; q++) part[q] = row[q].start();
@@ -760,19 +642,12 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
         ZmPair<T> lv[2 * ZM_M + 1];
 #pragma unroll
         for (int dx = -ZM_M; dx <= ZM_M; dx++)
-          if (dx * dx + dy * dy <= ZM_M * ZM_M) {
-            if (NOMASK) {
-              lv[dx + ZM_M].v = mev[dy * ZM_W + dx];
-              lv[dx + ZM_M].m = T(1);
-            } else {
-              lv[dx + ZM_M] = me[dy * ZM_W + dx];
-            }
-          }
+          if (dx * dx + dy * dy <= ZM_M * ZM_M) lv[dx + ZM_M] = me[dy * ZM_W + dx];
 #pragma unroll
         for (int q = 0; q < ZM_SL; q++) {
           const int dz = ZM_M - q;
           if (dz * dz + dy * dy > ZM_M * ZM_M) continue;
-          if (!ALL && !act[q]) continue;  // uniform over the block
+          if (!act[q]) continue;  // uniform over the block
 #pragma unroll
           for (int dx = -ZM_M; dx <= ZM_M; dx++) {
             const int d2 = dx * dx + dy * dy + dz * dz;
@@ -788,57 +663,247 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
       }
 #pragma unroll
       for (int q = 0; q < ZM_SL; q++)
-        if (ALL || act[q]) row[q].fold(part[q]);
+        if (act[q]) row[q].fold(part[q]);
     };
-    using t_ = std::true_type;
-    using f_ = std::false_type;
-    if constexpr (RING) {
-      const T uc = mev[0];
-      T R[ZM_M * ZM_M + 1];
-#pragma unroll
-      for (int dy = -ZM_M; dy <= ZM_M; dy++)
-#pragma unroll
-        for (int dx = -ZM_M; dx <= ZM_M; dx++) {
-          const int k = dx * dx + dy * dy;
-          if (k == 0 || k > ZM_M * ZM_M) continue;
-          const T d = mev[dy * LW + dx] - uc;
-          R[k] = zm_ring_first<ZM_M>(dx, dy) ? d : R[k] + d;
-        }
-#pragma unroll
-      for (int q = 0; q < ZM_SL; q++) {
-        if (!act[q]) continue;  // uniform over the block
-        const int dz = ZM_M - q, a = dz < 0 ? -dz : dz;
-        // dz = 0: the row is the column itself (p_o - p_c = 0)
-        T part = a ? wz[a] * (po[q] - uc) : T(0);
-        bool first = a == 0;
-#pragma unroll
-        for (int k = 1; k <= ZM_M * ZM_M; k++) {
-          if (!zm_ring_exists<ZM_M>(k) || k + a * a > ZM_M * ZM_M) continue;
-          const T w = zm_weight<T>(ball, k + a * a);
-          part = first ? -w * R[k] : __fmaf_rn(-w, R[k], part);
-          first = false;
-        }
-        row[q].fold(part);
-      }
-    } else {
-      // (an instance without the per-slot tests for the planes where every slot is active
-      // measured slower: its straight-line code spilled at four waves per SIMD)
-      if (full) body(f_{}, f_{});
-      else body(t_{}, f_{});
+    // (an instance without the per-slot tests for the planes where every slot is active
+    // measured slower: its straight-line code spilled at four waves per SIMD)
+    if (full) body(std::false_type{});
+    else body(std::true_type{});
+    finish(zi);
+    buf ^= 1;
+  }
+  // planes past the top of the box add nothing: the rows still in flight are complete
+  for (int zi = ze; zi < zend + ZM_M; zi++) finish(zi);
+}
+
+template <int M>
+__device__ __forceinline__ constexpr bool zm_ring_first(int dx, int dy) {
+  // (dx, dy) is the first point of its ring in the walk order (dy outer, dx inner, ascending)
+  for (int y = -M; y <= M; y++)
+    for (int x = -M; x <= M; x++) {
+      if (x * x + y * y != dx * dx + dy * dy) continue;
+      return x == dx && y == dy;
     }
+  return false;
+}
+template <int M>
+__device__ __forceinline__ constexpr bool zm_ring_exists(int k) {
+  for (int y = -M; y <= M; y++)
+    for (int x = -M; x <= M; x++)
+      if (x * x + y * y == k) return true;
+  return false;
+}
+
+template <typename Src, bool VEC>
+struct ZmRaw4 {
+  using type = typename Src::Raw;
+};
+template <typename Src>
+struct ZmRaw4<Src, true> {
+  using type = typename Src::Raw4;
+};
+
+// The FP32 walk of the CG vectors (the unmasked operator, rings). The input vector is 0 at
+// every node that is not free (the CG vectors D^-1 r and their combinations, or a vector masked
+// as it is loaded), and dd is the diagonal less the weights of the row's bonds to non-free nodes
+// and to in-plane positions outside the box (pdd32nf): every bond then takes its weight unmasked
+// (the bonds to those nodes add w (p_i - 0), which the diagonal takes back) and only the values
+// are staged. Rows that are not free give dd p_i = 0.
+//
+// Bonds by rings: around the thread's column (value p_c in input plane z_i) the disc
+// dx^2 + dy^2 <= M^2 splits into rings of equal k = dx^2 + dy^2, and the weight of a bond
+// depends on k + dz^2 only, so with R_k = sum over ring k of (p_j - p_c) and W_dz the weight sum
+// of the disc at plane distance dz, the row of output plane z_o takes from input plane z_i
+//   sum over the plane's bonds of w (p_o - p_j) = W_dz (p_o - p_c) - sum_k w(k + dz^2) R_k,
+// dz = z_i - z_o. The ring sums are formed once per plane and shared by the rows in flight. The
+// ring terms are added to the row's sum as the planes pass; the first terms, over the planes,
+// are the bonds along the row's own z-line, sum over a = 1..M of
+//   W_a ((p_o - p_{o+a}) + (p_o - p_{o-a}))   (planes inside the box),
+// taken when the row completes from the column values of the walked planes. The ring sums are
+// symmetric sums of local differences and each z-line pair a symmetric difference, so every
+// term is of the size of the row's result (second differences of p), and the row is one FP32
+// sum without compensation. Per plane and thread: 28 subtractions and 22 additions for the
+// rings, at most 6 multiply-adds per row in flight; against a subtraction and a multiply-add
+// per bond (122).
+//
+// VEC (nx a multiple of 4): the plane is staged in 16-byte quads (Src::load4 / value4), the
+// quads of the next two planes in flight; otherwise single values (Src::load / value), the
+// next plane's in flight.
+template <bool VEC, typename Src, typename Emit>
+__device__ __forceinline__ void zm_walk_ring(const ZmGeom& g, const nnpp_ball_args& ball,
+                                             const uint8_t* __restrict__ pfree,
+                                             const float* __restrict__ dd, const Src& src,
+                                             Emit& emit) {
+  using Raw = typename Src::Raw;
+  using Pre = typename Emit::Pre;
+  constexpr int LW = VEC ? ZM_W4 : ZM_W;  // staged row width in LDS
+  constexpr int LX = VEC ? ZM_X4 : ZM_M;  // the thread's column within the staged row
+  __shared__ __align__(16) float ldsv[2][VEC ? LW * ZM_H : ZM_PTS];
+  const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
+  const int zc = (int)(item % g.nzc);
+  const int64_t tile = item / g.nzc;
+  const int x0 = (int)(tile % g.tiles_x) * ZM_TX, y0 = (int)(tile / g.tiles_x) * ZM_TY;
+  const int z0 = zc * g.lz;
+  const int zend = min(z0 + g.lz, g.nz);
+  const int tx = threadIdx.x % ZM_TX, ty = threadIdx.x / ZM_TX;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool col_ok = x < g.nx && y < g.ny;
+  // this thread's staged points: in-plane offset (-1: outside the box or past the tile); VEC:
+  // the first node of its quad (a quad lies wholly inside or outside the box: nx % 4 == 0)
+  constexpr int NST = VEC ? 1 : ZM_STAGE;
+  int32_t soff[NST];
+#pragma unroll
+  for (int k = 0; k < NST; k++) {
+    if (VEC) {
+      const int e = threadIdx.x;
+      const int gx = x0 + (e % (ZM_W4 / 4)) * 4 - ZM_X4, gy = y0 + e / (ZM_W4 / 4) - ZM_M;
+      soff[k] = (e < ZM_Q4 && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx
+                                                                             : -1;
+    } else {
+      const int e = threadIdx.x + k * NNPP_BLOCK;
+      const int gx = x0 + e % ZM_W - ZM_M, gy = y0 + e / ZM_W - ZM_M;
+      soff[k] = (e < ZM_PTS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx
+                                                                             : -1;
+    }
+  }
+  const int zs = max(z0 - ZM_M, 0), ze = min(zend + ZM_M, g.nz);  // input planes [zs, ze)
+  const uint32_t col = col_ok ? (uint32_t)(y * g.nx + x) : 0u;
+  const uint32_t sz = (uint32_t)g.sz;
+  const auto in_chunk = [&](int zo) { return zo >= z0 && zo < zend; };
+  // rs[q]: the ring terms of output plane zi - M + q so far; cv[k]: the column value of plane
+  // zi - k (walked planes; 0 past the top of the box, never used there)
+  float rs[ZM_SL], cv[ZM_SL];
+#pragma unroll
+  for (int q = 0; q < ZM_SL; q++) rs[q] = cv[q] = 0.0f;
+  // weight sums of the disc by plane distance |dz| (FP32, fixed order)
+  float wz[ZM_M + 1];
+#pragma unroll
+  for (int a = 0; a <= ZM_M; a++) {
+    float s = 0.0f;
+#pragma unroll
+    for (int dy = -ZM_M; dy <= ZM_M; dy++)
+#pragma unroll
+      for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+        const int d2 = dx * dx + dy * dy + a * a;
+        if (d2 > ZM_M * ZM_M || d2 == 0) continue;
+        s += zm_weight<float>(ball, d2);
+      }
+    wz[a] = s;
+  }
+  // diagonal term, free flag and handler operands of the next row to complete, a plane ahead
+  float d0 = 0.0f;
+  bool f0 = false;
+  Pre pre0{};
+  const auto load_first = [&](int zo) {
+    if (col_ok && in_chunk(zo)) {
+      const uint32_t i = (uint32_t)zo * sz + col;
+      d0 = zm_ld(dd, i);
+      f0 = zm_ld(pfree, i) != 0;
+      pre0 = emit.pre(i);
+    }
+  };
+  load_first(zs - ZM_M);
+  // staged loads: VEC the quads of the next two planes (svq[0] for the planes zs, zs + 2, ...,
+  // svq[1] for the others), else the next plane's values
+  using SvRaw = typename std::conditional<VEC, typename ZmRaw4<Src, VEC>::type, Raw>::type;
+  SvRaw svq[VEC ? 2 : 1][NST];
+  const auto fetch = [&](int zi, SvRaw (&dst)[NST]) {
+    const uint32_t base = (uint32_t)zi * sz;
+#pragma unroll
+    for (int k = 0; k < NST; k++) {
+      if constexpr (VEC)
+        dst[k] = soff[k] >= 0 ? src.load4(base + (uint32_t)soff[k]) : SvRaw{};
+      else
+        dst[k] = soff[k] >= 0 ? src.load(base + (uint32_t)soff[k]) : SvRaw{};
+    }
+  };
+  fetch(zs, svq[0]);
+  if (VEC && zs + 1 < ze) fetch(zs + 1, svq[VEC ? 1 : 0]);
+  // after input plane zi (cv[0] its column value): output plane zo = zi - M is complete
+  const auto finish = [&](int zi) {
+    const int zo = zi - ZM_M;
+    if (col_ok && in_chunk(zo)) {
+      const float uo = cv[ZM_M];
+      float line = 0.0f;
+#pragma unroll
+      for (int a = 1; a <= ZM_M; a++) {
+        const bool up = zo + a < g.nz, dn = zo - a >= 0;  // uniform over the block
+        float t;
+        if (up && dn) t = (uo - cv[ZM_M - a]) + (uo - cv[ZM_M + a]);
+        else if (up) t = uo - cv[ZM_M - a];
+        else if (dn) t = uo - cv[ZM_M + a];
+        else continue;
+        line = __fmaf_rn(wz[a], t, line);
+      }
+      const float yv = f0 ? __fmaf_rn(d0, uo, rs[0] + line) : d0 * uo;
+      emit((int64_t)((uint32_t)zo * sz + col), yv, uo, pre0);
+    }
+#pragma unroll
+    for (int q = 0; q + 1 < ZM_SL; q++) rs[q] = rs[q + 1];
+    rs[ZM_SL - 1] = 0.0f;
+    load_first(zo + 1);
+  };
+  const auto shift_cv = [&](float v) {
+#pragma unroll
+    for (int k = ZM_SL - 1; k > 0; k--) cv[k] = cv[k - 1];
+    cv[0] = v;
+  };
+  int buf = 0;
+  // input plane zi; its staged loads are in cur, which then takes plane zi + 2 (VEC) or zi + 1
+  const auto plane = [&](int zi, SvRaw (&cur)[NST]) {
+    if constexpr (VEC) {
+      if (threadIdx.x < ZM_Q4)
+        reinterpret_cast<float4*>(&ldsv[buf][0])[threadIdx.x] = src.value4(cur[0]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NST; k++) {
+        const int e = threadIdx.x + k * NNPP_BLOCK;
+        if (e < ZM_PTS) ldsv[buf][e] = src.value(cur[k]);
+      }
+    }
+    // this plane staged (and the buffer written two planes ago read by every thread)
+    __syncthreads();
+    if (zi + (VEC ? 2 : 1) < ze) fetch(zi + (VEC ? 2 : 1), cur);
+    const float* __restrict__ mev = &ldsv[buf][(ty + ZM_M) * LW + tx + LX];
+    const float uc = mev[0];
+    float R[ZM_M * ZM_M + 1];
+#pragma unroll
+    for (int dy = -ZM_M; dy <= ZM_M; dy++)
+#pragma unroll
+      for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+        const int k = dx * dx + dy * dy;
+        if (k == 0 || k > ZM_M * ZM_M) continue;
+        const float d = mev[dy * LW + dx] - uc;
+        R[k] = zm_ring_first<ZM_M>(dx, dy) ? d : R[k] + d;
+      }
+    // slot q (output plane zi - M + q) takes the rings at dz = M - q
+#pragma unroll
+    for (int q = 0; q < ZM_SL; q++) {
+      if (!in_chunk(zi - ZM_M + q)) continue;  // uniform over the block
+      const int dz = ZM_M - q, a = dz < 0 ? -dz : dz;
+#pragma unroll
+      for (int k = 1; k <= ZM_M * ZM_M; k++) {
+        if (!zm_ring_exists<ZM_M>(k) || k + a * a > ZM_M * ZM_M) continue;
+        rs[q] = __fmaf_rn(-zm_weight<float>(ball, k + a * a), R[k], rs[q]);
+      }
+    }
+    shift_cv(uc);
     finish(zi);
     buf ^= 1;
   };
   if constexpr (VEC) {
     for (int zi = zs; zi < ze; zi += 2) {
       plane(zi, svq[0]);
-      if (zi + 1 < ze) plane(zi + 1, svq[1]);
+      if (zi + 1 < ze) plane(zi + 1, svq[VEC ? 1 : 0]);
     }
   } else {
     for (int zi = zs; zi < ze; zi++) plane(zi, svq[0]);
   }
   // planes past the top of the box add nothing: the rows still in flight are complete
-  for (int zi = ze; zi < zend + ZM_M; zi++) finish(zi);
+  for (int zi = ze; zi < zend + ZM_M; zi++) {
+    shift_cv(0.0f);
+    finish(zi);
+  }
 }
 
 // Per column tile and plane: 1 when every point the tile stages from that plane (tile plus
@@ -930,7 +995,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_spmv(ZmGeom g, nnpp_ball_ar
     zm_st(q, (uint32_t)i, y);
     v[0] += (double)pi * (double)y;
   });
-  zm_walk<float, true, VEC>(g, ball, pfree, full, ddnf, ZmLoadFree{p, pfree}, em);
+  zm_walk_ring<VEC>(g, ball, pfree, ddnf, ZmLoadFree{p, pfree}, em);
   nnpp_block_partial<1>(v, part);
 }
 
@@ -1133,7 +1198,7 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_arg
       v[2] += (double)ri * (double)ri;
     }
   } em{src, vec, v};
-  zm_walk<float, true, VEC>(g, ball, pfree, full, dd, src, em);
+  zm_walk_ring<VEC>(g, ball, pfree, dd, src, em);
   nnpp_block_partial<3>(v, part_out);
 }
 
