@@ -133,7 +133,8 @@ typedef struct nnpp_step_stats {
                                ball, 4 with the z-marching kernels of a full box lattice (no
                                row masks: a free flag per node); 0 for the column form */
   int32_t flux_stencil;     /* the explicit flux used the lattice stencil form: 1 with FP64
-                               bond sums, 2 with the FP32 bond sums of NNPP_MIXED; 0 for the
+                               bond sums, 2 with the FP32 bond sums of NNPP_MIXED, 3 the
+                               z-marching form of a full box lattice (FP64 sums); 0 for the
                                column form */
 } nnpp_step_stats;
 

@@ -1145,6 +1145,333 @@ hipError_t launch_st32(const FluxArgs& a, const FluxSt& st, const float* tf32, c
   return hipGetLastError();
 }
 
+// ------------------------------------------------- z-marching form (full box lattices) -------
+// The explicit update on a full box lattice with the radius-3 ball of offsets and flux weights by
+// class dx^2 + dy^2 + dz^2 (FP64 sums). Blocks take the potential operator's z-marching work
+// items (a 32 x 8 column tile and a chunk of lz output planes, ZmGeom) and the species one at a
+// time: per species the block walks its input planes, staging each plane's tile with a 3-node
+// halo as (c_j, phi^n_j) pairs and classes (the phase, 2 outside the box) into LDS, and each row
+// of the thread's column in flight (up to 7) takes the plane's bonds,
+//   acc += A(p_o, p_j) sum w (c_j - c_o) + M(p_o, p_j) sum w (phi_j - phi_o) c_j,
+// with the phase pair's coefficients (utils.h:100-159: D by phase pair, BV on solid-liquid
+// bonds); A and M are applied per plane. A row is updated for the species when it completes
+// (U_k^{n+1}, Dirichlet row, clamp: explicit_update for one field).
+//
+// Per plane and wave: when the wave's disc region holds nodes of one phase only and every row
+// of the slot has that phase, the slot takes every bond with the weight of its class, no tests
+// (positions outside the box are staged as c = phi = 0: their bonds add -w c_o to the diffusion
+// sum and 0 to the migration sum, and w_out c_o, with w_out the weight of the row's bonds to
+// outside the box in that plane, takes it back). Otherwise the slot's bonds take their class's
+// coefficient bond by bond (outside the box: none).
+struct ZmFluxW {
+  double w[NNPP_BALL_CLASSES];  // FP64 flux weight by class dx^2 + dy^2 + dz^2
+  double wz[NNPP_BALL_M + 1];   // weight sum of the disc at plane distance |dz| (the center
+                                // included for dz != 0)
+};
+
+template <int S>
+__device__ __forceinline__ void zm_update_one(const FluxArgs& a, int k, uint32_t i, int pi,
+                                              int intf, double co, double acc) {
+  const int64_t n = a.n;
+  double Rk = 0.0;
+  if constexpr (S == 10) {
+    if (a.chemistry == NNPP_CHEM_REFERENCE) {
+      double ci[S];
+#pragma unroll
+      for (int f = 0; f < S; f++) ci[f] = a.Un[(int64_t)f * n + i];
+      Rk = ref_reaction(k, a.rx, pi, intf, ci, ci, a.Un[(int64_t)S * n + i]);
+    }
+  }
+  if (a.chemistry == NNPP_CHEM_GENERIC && k == 0 && intf) Rk = -a.r_diss * co;
+  // explicit_update's operations for field k
+  const double flux = -acc;
+  double v = co - a.dt * (flux - Rk);
+  const unsigned bm = a.bcmask[i];
+  bool keep = !((bm >> k) & 1u);
+  if (!keep) v = a.bcval[(int64_t)k * n + i];
+  const bool cl = ((a.clamp_always >> k) & 1u) || (((a.clamp_liquid >> k) & 1u) && pi == 0);
+  if (cl && v < a.clamp_floor) {
+    v = a.clamp_floor;
+    keep = false;
+  }
+  a.out[(int64_t)k * n + i] = v;
+  if (a.passmask) {
+    const uint16_t bit = keep ? (uint16_t)(1u << k) : (uint16_t)0;
+    a.passmask[i] = k == 0 ? bit : (uint16_t)(a.passmask[i] | bit);
+  }
+}
+
+// (held at 2 waves per SIMD: the register demand of the rows in flight, FP64)
+template <int S>
+__global__ __launch_bounds__(NNPP_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
+  __shared__ double2 lcp[2][ZM_PTS];   // (c_j, phi_j); 0 outside the box
+  __shared__ uint8_t lcl[2][ZM_PTS];   // phase, 2 outside the box
+  const int64_t n = a.n;
+  const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
+  const int zc = (int)(item % g.nzc);
+  const int64_t tile = item / g.nzc;
+  const int x0 = (int)(tile % g.tiles_x) * ZM_TX, y0 = (int)(tile / g.tiles_x) * ZM_TY;
+  const int z0 = zc * g.lz;
+  const int zend = min(z0 + g.lz, g.nz);
+  const int tx = threadIdx.x % ZM_TX, ty = threadIdx.x / ZM_TX;
+  const int x = x0 + tx, y = y0 + ty;
+  const bool col_ok = x < g.nx && y < g.ny;
+  const int wv = threadIdx.x >> 6;  // the wave's rows: ty = 2 wv, 2 wv + 1
+  int32_t soff[ZM_STAGE];
+#pragma unroll
+  for (int k = 0; k < ZM_STAGE; k++) {
+    const int e = threadIdx.x + k * NNPP_BLOCK;
+    const int gx = x0 + e % ZM_W - ZM_M, gy = y0 + e / ZM_W - ZM_M;
+    soff[k] = (e < ZM_PTS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) ? gy * g.nx + gx : -1;
+  }
+  const int zs = max(z0 - ZM_M, 0), ze = min(zend + ZM_M, g.nz);  // input planes [zs, ze)
+  const uint32_t col = col_ok ? (uint32_t)(y * g.nx + x) : 0u;
+  const uint32_t sz = (uint32_t)g.sz;
+  const double* __restrict__ phi = a.Un + (int64_t)S * n;
+  const auto in_chunk = [&](int zo) { return zo >= z0 && zo < zend; };
+  // weight of the row's bonds to in-plane positions outside the box, by plane distance |dz|
+  double wout[ZM_M + 1];
+#pragma unroll
+  for (int az = 0; az <= ZM_M; az++) {
+    double s = 0.0;
+#pragma unroll
+    for (int dy = -ZM_M; dy <= ZM_M; dy++)
+#pragma unroll
+      for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+        const int d2 = dx * dx + dy * dy + az * az;
+        if (d2 > ZM_M * ZM_M || d2 == 0) continue;
+        const bool out = x + dx < 0 || x + dx >= g.nx || y + dy < 0 || y + dy >= g.ny;
+        if (out) s += fw.w[d2];
+      }
+    wout[az] = s;
+  }
+
+  int buf = 0;
+#pragma unroll 1
+  for (int k = 0; k < S; k++) {
+    const double* __restrict__ Uk = a.Un + (int64_t)k * n;
+    const double A_same[2] = {a.A_LL[k], a.A_SS[k]}, M_same[2] = {a.M_LL[k], a.M_SS[k]};
+    const double A_x = a.A_SL[k], M_x = a.M_SL[k];
+    // rows in flight: slot q holds output plane zi - M + q while input plane zi is processed
+    double acc[ZM_SL], co[ZM_SL], pho[ZM_SL];
+    uint32_t pos = 0u, cross = 0u;  // bit q: the row of slot q is solid / has a solid-liquid bond
+    const auto row_load = [&](int zo, double& c, double& p, uint32_t& solid) {
+      if (col_ok && zo >= max(z0, 0) && zo < zend) {
+        const uint32_t i = (uint32_t)zo * sz + col;
+        c = Uk[i];
+        p = phi[i];
+        solid = a.phase[i] != 0 ? 1u : 0u;
+      } else {
+        c = p = 0.0;
+        solid = 0u;
+      }
+    };
+#pragma unroll
+    for (int q = 0; q < ZM_SL; q++) {
+      acc[q] = 0.0;
+      uint32_t sb;
+      row_load(zs - ZM_M + q, co[q], pho[q], sb);
+      pos |= sb << q;
+    }
+    // the row that enters the last slot at the next plane, loaded a plane ahead
+    double nc, np;
+    uint32_t nsb;
+    row_load(zs + 1 + ZM_M, nc, np, nsb);
+    // staged loads of the next plane
+    double sc[ZM_STAGE], sp[ZM_STAGE];
+    uint8_t sq[ZM_STAGE];
+    const auto fetch = [&](int zi) {
+      const uint32_t base = (uint32_t)zi * sz;
+#pragma unroll
+      for (int e = 0; e < ZM_STAGE; e++) {
+        const bool ok = soff[e] >= 0;
+        const uint32_t j = base + (uint32_t)(ok ? soff[e] : 0);
+        sc[e] = ok ? Uk[j] : 0.0;
+        sp[e] = ok ? phi[j] : 0.0;
+        sq[e] = ok ? a.phase[j] : (uint8_t)2;
+      }
+    };
+    fetch(zs);
+    const auto finish = [&](int zi) {
+      const int zo = zi - ZM_M;
+      if (col_ok && in_chunk(zo)) {
+        const int pi = (int)(pos & 1u);
+        const int intf = (pi == 1 && (cross & 1u)) ? 1 : 0;  // utils.h:82-89
+        zm_update_one<S>(a, k, (uint32_t)zo * sz + col, pi, intf, co[0], acc[0]);
+      }
+#pragma unroll
+      for (int q = 0; q + 1 < ZM_SL; q++) {
+        acc[q] = acc[q + 1];
+        co[q] = co[q + 1];
+        pho[q] = pho[q + 1];
+      }
+      pos >>= 1;
+      cross >>= 1;
+      acc[ZM_SL - 1] = 0.0;
+      co[ZM_SL - 1] = nc;
+      pho[ZM_SL - 1] = np;
+      pos |= nsb << (ZM_SL - 1);
+      row_load(zi + 2 + ZM_M, nc, np, nsb);
+    };
+#pragma unroll 1
+    for (int zi = zs; zi < ze; zi++) {
+#pragma unroll
+      for (int e = 0; e < ZM_STAGE; e++) {
+        const int pt = threadIdx.x + e * NNPP_BLOCK;
+        if (pt < ZM_PTS) {
+          lcp[buf][pt] = make_double2(sc[e], sp[e]);
+          lcl[buf][pt] = sq[e];
+        }
+      }
+      __syncthreads();
+      if (zi + 1 < ze) fetch(zi + 1);
+      // the phases in the wave's disc region (staged lines 2 wv .. 2 wv + 7, 304 points): five
+      // class bytes per lane, then a vote
+      bool hl = false, hs = false;
+      {
+        const uint8_t* __restrict__ rg = &lcl[buf][2 * wv * ZM_W];
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int r = 0; r < (8 * ZM_W + 63) / 64; r++) {
+          const int pt = lane + 64 * r;
+          if (pt < 8 * ZM_W) {
+            const uint32_t cq = rg[pt];
+            hl = hl || cq == 0u;
+            hs = hs || cq == 1u;
+          }
+        }
+      }
+      const bool hasL = __any(hl), hasS = __any(hs);
+      const uint32_t cstar = hasS ? 1u : 0u;
+      const bool single = !(hasL && hasS);
+      // the thread's first disc line (dy = -M) in the staged plane
+      const int lb0 = ty * ZM_W + tx + ZM_M;
+      // per slot: active (row in the chunk), fast (one phase: every bond, no tests)
+      bool act[ZM_SL], fast[ZM_SL];
+#pragma unroll
+      for (int q = 0; q < ZM_SL; q++) {
+        act[q] = in_chunk(zi - ZM_M + q);
+        const uint32_t sb = (pos >> q) & 1u;
+        fast[q] = act[q] && single && __all(!col_ok || sb == cstar);
+      }
+      {
+        // one phase: the disc by rings of equal k = dx^2 + dy^2 around the column (c_c, phi_c):
+        // RD_k = sum (c_j - c_c), RQ_k = sum (phi_j - phi_c) c_j, and a slot at plane distance
+        // dz takes
+        //   sum w (c_j - c_o)          = G + W_dz (c_c - c_o),          G = sum_k w RD_k
+        //   sum w (phi_j - phi_o) c_j  = H + (phi_c - phi_o)(G + W_dz c_c),  H = sum_k w RQ_k
+        // (positions outside the box are staged as 0: RD takes -c_c there, RQ 0)
+        bool anyfast = false;
+#pragma unroll
+        for (int q = 0; q < ZM_SL; q++) anyfast = anyfast || fast[q];
+        if (anyfast) {
+          const double2 cen = lcp[buf][lb0 + ZM_M * ZM_W];
+          double RD[ZM_M * ZM_M + 1], RQ[ZM_M * ZM_M + 1];
+#pragma unroll
+          for (int dy = -ZM_M; dy <= ZM_M; dy++) {
+            // one disc line at a time: the scheduler does not batch the whole disc's reads
+            // ahead of the sums and hold them in registers
+            __builtin_amdgcn_sched_barrier(0);
+            const double2* __restrict__ line = &lcp[buf][lb0 + (dy + ZM_M) * ZM_W];
+#pragma unroll
+            for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+              const int kk = dx * dx + dy * dy;
+              if (kk == 0 || kk > ZM_M * ZM_M) continue;
+              const double2 v = line[dx];
+              const double d = v.x - cen.x;
+              const double qv = (v.y - cen.y) * v.x;
+              const bool first = zm_ring_first<ZM_M>(dx, dy);
+              RD[kk] = first ? d : RD[kk] + d;
+              RQ[kk] = first ? qv : RQ[kk] + qv;
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int q = 0; q < ZM_SL; q++) {
+            if (!fast[q]) continue;  // uniform over the wave
+            const int dz = ZM_M - q, az = dz < 0 ? -dz : dz;
+            double G = 0.0, H = 0.0;
+            bool first = true;
+#pragma unroll
+            for (int kk = 1; kk <= ZM_M * ZM_M; kk++) {
+              if (!zm_ring_exists<ZM_M>(kk) || kk + az * az > ZM_M * ZM_M) continue;
+              const double w = fw.w[kk + az * az];
+              G = first ? w * RD[kk] : __fma_rn(w, RD[kk], G);
+              H = first ? w * RQ[kk] : __fma_rn(w, RQ[kk], H);
+              first = false;
+            }
+            double sd, sm;
+            if (az == 0) {  // the column itself: c_c = c_o, phi_c = phi_o
+              sd = G;
+              sm = H;
+            } else {
+              sd = __fma_rn(fw.wz[az], cen.x - co[q], G);
+              sm = __fma_rn(cen.y - pho[q], __fma_rn(fw.wz[az], cen.x, G), H);
+            }
+            sd = __fma_rn(co[q], wout[az], sd);  // the bonds outside the box back out
+            acc[q] = __fma_rn(A_same[cstar], sd, acc[q]);
+            acc[q] = __fma_rn(M_same[cstar], sm, acc[q]);
+          }
+        }
+      }
+      // the other active slots: bond by bond, the coefficient of each bond's phase pair
+#pragma unroll
+      for (int q = 0; q < ZM_SL; q++) {
+        if (!act[q] || fast[q]) continue;  // uniform over the wave
+        const int dz = ZM_M - q;
+        const uint32_t po = (pos >> q) & 1u;
+        double ds = 0.0, ms = 0.0, dxs = 0.0, mxs = 0.0;  // same-phase and solid-liquid sums
+        bool anyx = false;
+        int lb = lb0;
+        __builtin_amdgcn_sched_barrier(0);  // one slot at a time
+#pragma unroll
+        for (int dy = -ZM_M; dy <= ZM_M; dy++) {
+          if (dz * dz + dy * dy > ZM_M * ZM_M) {
+            lb += ZM_W;
+            continue;
+          }
+          __builtin_amdgcn_sched_barrier(0);  // one disc line at a time
+          const double2* __restrict__ line = &lcp[buf][lb];
+          const uint8_t* __restrict__ linec = &lcl[buf][lb];
+          lb += ZM_W;
+#pragma unroll
+          for (int dx = -ZM_M; dx <= ZM_M; dx++) {
+            const int d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > ZM_M * ZM_M || d2 == 0) continue;
+            const double2 v = line[dx];
+            const uint32_t cj = linec[dx];
+            const double w = fw.w[d2];
+            const double ws = cj == po ? w : 0.0, wx = cj == (po ^ 1u) ? w : 0.0;
+            anyx = anyx || cj == (po ^ 1u);
+            const double dc = v.x - co[q], dp = v.y - pho[q];
+            ds = __fma_rn(ws, dc, ds);
+            ms = __fma_rn(ws * dp, v.x, ms);
+            dxs = __fma_rn(wx, dc, dxs);
+            mxs = __fma_rn(wx * dp, v.x, mxs);
+          }
+        }
+        acc[q] = __fma_rn(A_same[po], ds, acc[q]);
+        acc[q] = __fma_rn(M_same[po], ms, acc[q]);
+        acc[q] = __fma_rn(A_x, dxs, acc[q]);
+        acc[q] = __fma_rn(M_x, mxs, acc[q]);
+        cross |= (anyx ? 1u : 0u) << q;
+      }
+      finish(zi);
+      buf ^= 1;
+    }
+    // planes past the top of the box add nothing: the rows still in flight are complete
+#pragma unroll 1
+    for (int zi = ze; zi < zend + ZM_M; zi++) finish(zi);
+  }
+}
+
+template <int S>
+hipError_t launch_zm(const FluxArgs& a, const ZmGeom& g, const ZmFluxW& fw, hipStream_t s) {
+  k_zm_flux<S><<<g.items, NNPP_BLOCK, 0, s>>>(a, g, fw);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 static FluxArgs flux_args(nnpp_ctx* c, double BV, const double* U1, const double* Un,
@@ -1290,6 +1617,45 @@ static int launch_stencil(nnpp_ctx* c, const FluxArgs& a, size_t st_lds, bool fp
 
 int nnpp_flux_stencil_active(nnpp_ctx* c) { return flux_stencil_lds(c) != 0; }
 
+// The z-marching form of the explicit update (FP64 sums): a full box lattice with the radius-3
+// ball of offsets and flux weights by class, unless a format flag selects the tile forms.
+int nnpp_flux_zmarch_active(nnpp_ctx* c) {
+  if (c->format & (NNPP_FORMAT_NO_STENCIL | NNPP_FORMAT_NO_BALL | NNPP_FORMAT_NO_ZMARCH)) return 0;
+  if (flux_stencil_lds(c) == 0) return 0;
+  const nnpp_stencil& T = c->pst;
+  return (T.box && T.fball && T.ball == NNPP_BALL_M) ? 1 : 0;
+}
+
+static int launch_zmarch(nnpp_ctx* c, const FluxArgs& a) {
+  const int S = c->p.n_species;
+  const ZmGeom g = nnpp_zm_geom(c->pst, 1024);
+  ZmFluxW fw;
+  for (int d = 0; d < NNPP_BALL_CLASSES; d++) fw.w[d] = c->pst.fball_w64[d];
+  constexpr int M = NNPP_BALL_M;
+  for (int az = 0; az <= M; az++) {
+    double sw = 0.0;
+    for (int dy = -M; dy <= M; dy++)
+      for (int dx = -M; dx <= M; dx++) {
+        const int d2 = dx * dx + dy * dy + az * az;
+        if (d2 <= M * M && d2 != 0) sw += fw.w[d2];
+      }
+    fw.wz[az] = sw;
+  }
+  hipError_t e = hipSuccess;
+  switch (S) {
+#define NNPP_CASE(SS) \
+  case SS: e = launch_zm<SS>(a, g, fw, c->stream); break;
+    NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
+    NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
+    NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)
+#undef NNPP_CASE
+    default:
+      return nnpp_set_error(c, NNPP_ERR_ARG, "n_species out of range");
+  }
+  if (e != hipSuccess) return nnpp_set_error(c, NNPP_ERR_HIP, hipGetErrorString(e));
+  return NNPP_OK;
+}
+
 int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const double* U1,
                      const double* Un, double* out, uint16_t* passmask) {
   const int S = c->p.n_species;
@@ -1298,6 +1664,8 @@ int nnpp_launch_flux(nnpp_ctx* c, int resid, int precision, double BV, const dou
   const int64_t nb = nnpp_nblocks(c->n);
   const int mixed = (precision == NNPP_MIXED);
   hipError_t e = hipSuccess;
+  // explicit update on a full box lattice in FP64: the z-marching form
+  if (!resid && !mixed && nnpp_flux_zmarch_active(c)) return launch_zmarch(c, a);
   // explicit update on a lattice: the stencil form, with FP32 bond sums in mixed precision
   const size_t st_lds = resid ? 0 : flux_stencil_lds(c, mixed);
   if (st_lds) return launch_stencil<false>(c, a, st_lds, mixed);

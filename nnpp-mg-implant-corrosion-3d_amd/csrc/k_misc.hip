@@ -979,7 +979,10 @@ static int build_stencil(nnpp_ctx* c) {
       }
       T.fball_wsum += (double)(float)tf[t];
     }
-    for (int d = 0; d < NNPP_BALL_CLASSES; d++) T.fball_w32[d] = (float)wc[d];
+    for (int d = 0; d < NNPP_BALL_CLASSES; d++) {
+      T.fball_w32[d] = (float)wc[d];
+      T.fball_w64[d] = wc[d];
+    }
     T.fball = ok ? 1 : 0;
   }
   if (T.ball == NNPP_BALL_M) {  // the full box, numbered x fastest?

@@ -429,12 +429,6 @@ __device__ __forceinline__ T stencil_tile(const Slices& sl, const T* __restrict_
 // multiply-add chain over its bonds in ascending offset order (planes in z order, then dy,
 // then dx), the additions of the stencil and column forms, bit for bit. FP32 (zm_walk_ring):
 // the bonds by rings of equal in-plane distance, one FP32 sum per row.
-constexpr int ZM_TX = 32, ZM_TY = 8;  // a half-wave reads one x-row of 32 (no bank conflicts)
-constexpr int ZM_M = NNPP_BALL_M;
-constexpr int ZM_W = ZM_TX + 2 * ZM_M, ZM_H = ZM_TY + 2 * ZM_M;
-constexpr int ZM_PTS = ZM_W * ZM_H;
-constexpr int ZM_STAGE = (ZM_PTS + NNPP_BLOCK - 1) / NNPP_BLOCK;  // staged points per thread
-constexpr int ZM_SL = 2 * ZM_M + 1;                               // rows in flight
 // 16-byte staging (VEC, nx a multiple of 4): each staged row of the plane starts at x0 - 4 and
 // is ZM_W4 wide, whole quads of 4 nodes, one quad per thread
 constexpr int ZM_X4 = 4;  // the staged row's first column relative to x0 (a whole quad)
@@ -454,13 +448,6 @@ __device__ __forceinline__ void zm_st(T* __restrict__ p, uint32_t j, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + j * (uint32_t)sizeof(T)) = v;
 }
 
-struct ZmGeom {
-  int nx, ny, nz;
-  int tiles_x, tiles_y;
-  int lz, nzc;       // output planes per block, chunks along z
-  int64_t sz;        // nx * ny
-  int64_t items;     // tiles_x * tiles_y * nzc (the grid)
-};
 
 template <typename T>
 struct ZmPair {
@@ -676,23 +663,6 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
   for (int zi = ze; zi < zend + ZM_M; zi++) finish(zi);
 }
 
-template <int M>
-__device__ __forceinline__ constexpr bool zm_ring_first(int dx, int dy) {
-  // (dx, dy) is the first point of its ring in the walk order (dy outer, dx inner, ascending)
-  for (int y = -M; y <= M; y++)
-    for (int x = -M; x <= M; x++) {
-      if (x * x + y * y != dx * dx + dy * dy) continue;
-      return x == dx && y == dy;
-    }
-  return false;
-}
-template <int M>
-__device__ __forceinline__ constexpr bool zm_ring_exists(int k) {
-  for (int y = -M; y <= M; y++)
-    for (int x = -M; x <= M; x++)
-      if (x * x + y * y == k) return true;
-  return false;
-}
 
 template <typename Src, bool VEC>
 struct ZmRaw4 {
@@ -1711,24 +1681,7 @@ struct PotOp {
 
 // Work items of the z-marching kernels: column tiles times chunks of lz output planes, lz
 // chosen for about 1024 items (at most NNPP_RED_GRID: the reduction partials), at least 4.
-ZmGeom zm_geom(const nnpp_stencil& T) {
-  ZmGeom g;
-  g.nx = T.nx;
-  g.ny = T.ny;
-  g.nz = T.nz;
-  g.tiles_x = (T.nx + ZM_TX - 1) / ZM_TX;
-  g.tiles_y = (T.ny + ZM_TY - 1) / ZM_TY;
-  const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y;
-  int64_t lz = (T.nz * tiles + 1023) / 1024;
-  lz = std::max<int64_t>(lz, (T.nz * tiles + NNPP_RED_GRID - 1) / NNPP_RED_GRID);
-  lz = std::max<int64_t>(lz, 4);
-  lz = std::min<int64_t>(lz, T.nz);
-  g.lz = (int)lz;
-  g.nzc = (int)((T.nz + lz - 1) / lz);
-  g.sz = (int64_t)T.nx * T.ny;
-  g.items = tiles * g.nzc;
-  return g;
-}
+ZmGeom zm_geom(const nnpp_stencil& T) { return nnpp_zm_geom(T, 1024); }
 
 // 16-byte staging of the FP32 ring kernels: every staged quad lies wholly inside or outside the
 // box and starts 16-byte aligned.

@@ -641,7 +641,10 @@ int nnpp_step_once(nnpp_ctx* c, int precision, nnpp_step_stats& loc, nnpp_tape_s
   loc.nnz_flux = c->bonds.nnz;
   loc.nnz_ll = c->pot.nnz;
   loc.pot_stencil = nnpp_pot_form(c);
-  loc.flux_stencil = nnpp_flux_stencil_active(c) ? (precision == NNPP_MIXED ? 2 : 1) : 0;
+  loc.flux_stencil = !nnpp_flux_stencil_active(c) ? 0
+                     : precision == NNPP_MIXED      ? 2
+                     : nnpp_flux_zmarch_active(c)   ? 3
+                                                    : 1;
   loc.pot_col_bytes = c->pst.active ? 0 : (c->pot.wide ? 4 : 2);
   loc.pot_weight_bytes = c->pst.active ? 0 : (c->pot.ncls > 0 ? 1 : 4);
   return status;

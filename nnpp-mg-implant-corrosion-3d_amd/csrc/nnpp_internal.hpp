@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -153,6 +154,7 @@ struct nnpp_stencil {
   nnpp_ball_args ball_args = {};
   int fball = 0;  // ball == NNPP_BALL_M and the flux weights depend on dx^2 + dy^2 + dz^2 only
   float fball_w32[NNPP_BALL_CLASSES] = {};  // (bit for bit): the FP32 flux weight by class
+  double fball_w64[NNPP_BALL_CLASSES] = {};  // the FP64 flux weight by class
   double fball_wsum = 0.0;  // sum of the FP32 flux weights of the ball's offsets (in FP64)
   // Box lattice (ball == NNPP_BALL_M only): the nodes are the full nx x ny x nz box numbered
   // x fastest (row strides sy = nx, sz = nx ny, n = nx ny nz) and every row's bonds are exactly
@@ -589,7 +591,68 @@ int nnpp_stencil_ensure(nnpp_ctx* c);
 // 1 where the explicit flux takes the lattice stencil form (FP64 in both precisions) for the
 // current geometry.
 int nnpp_flux_stencil_active(nnpp_ctx* c);
+int nnpp_flux_zmarch_active(nnpp_ctx* c);
 // The potential SpMV's form: 0 column slices, 1 stencil masks with the table-driven loop,
+// ------------------------------------------------------------ z-marching geometry -------
+// Box lattices (nnpp_stencil::box): the z-marching kernels of the potential operator and of the
+// flux give a block of 256 threads a ZM_TX x ZM_TY column tile of (x, y) and a chunk of lz
+// output planes; it walks the input planes of the chunk with a halo of ZM_M planes, each
+// staged with a ZM_M-node halo in x and y.
+constexpr int ZM_TX = 32, ZM_TY = 8;  // a half-wave reads one x-row of 32 (no bank conflicts)
+constexpr int ZM_M = NNPP_BALL_M;
+constexpr int ZM_W = ZM_TX + 2 * ZM_M, ZM_H = ZM_TY + 2 * ZM_M;
+constexpr int ZM_PTS = ZM_W * ZM_H;
+constexpr int ZM_STAGE = (ZM_PTS + NNPP_BLOCK - 1) / NNPP_BLOCK;  // staged points per thread
+constexpr int ZM_SL = 2 * ZM_M + 1;                               // rows in flight
+
+struct ZmGeom {
+  int nx, ny, nz;
+  int tiles_x, tiles_y;
+  int lz, nzc;       // output planes per block, chunks along z
+  int64_t sz;        // nx * ny
+  int64_t items;     // tiles_x * tiles_y * nzc (the grid)
+};
+
+// Rings of the disc dx^2 + dy^2 <= M^2: the points of equal k = dx^2 + dy^2.
+template <int M>
+__device__ __forceinline__ constexpr bool zm_ring_first(int dx, int dy) {
+  // (dx, dy) is the first point of its ring in the walk order (dy outer, dx inner, ascending)
+  for (int y = -M; y <= M; y++)
+    for (int x = -M; x <= M; x++) {
+      if (x * x + y * y != dx * dx + dy * dy) continue;
+      return x == dx && y == dy;
+    }
+  return false;
+}
+template <int M>
+__device__ __forceinline__ constexpr bool zm_ring_exists(int k) {
+  for (int y = -M; y <= M; y++)
+    for (int x = -M; x <= M; x++)
+      if (x * x + y * y == k) return true;
+  return false;
+}
+
+// Work items: column tiles times chunks of lz output planes, lz chosen for about `target` items
+// (at most NNPP_RED_GRID: the reduction partials), at least 4 planes.
+inline ZmGeom nnpp_zm_geom(const nnpp_stencil& T, int64_t target) {
+  ZmGeom g;
+  g.nx = T.nx;
+  g.ny = T.ny;
+  g.nz = T.nz;
+  g.tiles_x = (T.nx + ZM_TX - 1) / ZM_TX;
+  g.tiles_y = (T.ny + ZM_TY - 1) / ZM_TY;
+  const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y;
+  int64_t lz = (T.nz * tiles + target - 1) / target;
+  lz = std::max<int64_t>(lz, (T.nz * tiles + NNPP_RED_GRID - 1) / NNPP_RED_GRID);
+  lz = std::max<int64_t>(lz, 4);
+  lz = std::min<int64_t>(lz, T.nz);
+  g.lz = (int)lz;
+  g.nzc = (int)((T.nz + lz - 1) / lz);
+  g.sz = (int64_t)T.nx * T.ny;
+  g.items = tiles * g.nzc;
+  return g;
+}
+
 // NNPP_BALL_M stencil masks with the lattice-ball loop, NNPP_POT_ZMARCH the z-marching kernels
 // of a full box lattice.
 #define NNPP_POT_ZMARCH 4

@@ -110,7 +110,8 @@ def test_c2_crop32_of_128_fp64(case128):
         assert np.array_equal(rp_g, rp) and np.array_equal(col_g, col)
         st = pb.step(10, nn.FP64)
         U, ph = pb.get_state(), pb.get_phase()
-    assert st["status"] == 0 and st["flux_stencil"] == 1 and st["pot_stencil"] != 0
+    # a full box lattice: the z-marching forms of the flux and of the potential operator
+    assert st["status"] == 0 and st["flux_stencil"] == 3 and st["pot_stencil"] == 4
     assert np.array_equal(ph, ref["phase"])
     for f in range(12):
         assert rel_err(U[f], ref["U"][f]) < 1e-12, (f, rel_err(U[f], ref["U"][f]))
@@ -129,7 +130,7 @@ def test_c2_full128_stencil_vs_column_forms(case128):
             say(name, "flux_stencil", st["flux_stencil"], "pot_stencil", st["pot_stencil"],
                 "cg", st["cg_iterations"])
     a, c = res["auto"], res["columns"]
-    assert a[3]["flux_stencil"] == 1 and a[3]["pot_stencil"] != 0
+    assert a[3]["flux_stencil"] == 3 and a[3]["pot_stencil"] == 4
     assert c[3]["flux_stencil"] == 0 and c[3]["pot_stencil"] == 0
     assert rel_err(a[0], c[0]) <= 1e-14  # one operator application, same bond sums
     assert np.array_equal(a[2], c[2])
@@ -153,8 +154,8 @@ def test_c3_mixed_vs_fp64_1000_steps(case128):
                 done += st["steps"]
                 say(name, done)
             out[name] = (pb.get_state(), pb.get_phase(), st)
-    # the lattice's stencil flux: FP64 bond sums, FP32 bond sums in mixed precision
-    assert out["fp64"][2]["flux_stencil"] == 1 and out["mixed"][2]["flux_stencil"] == 2
+    # the lattice's flux: the z-marching form in FP64, FP32 bond sums in mixed precision
+    assert out["fp64"][2]["flux_stencil"] == 3 and out["mixed"][2]["flux_stencil"] == 2
     errs = field_errors(out["mixed"][0], out["fp64"][0])
     say("per-field (max, L2):", errs)
     assert np.array_equal(out["mixed"][1], out["fp64"][1])

@@ -150,9 +150,10 @@ def _exact_run(shape, stencil):
     for prec in (nn.FP64, nn.MIXED):
         ref, U, ph, st = run_both(case, 4, prec)
         assert st["status"] == 0
-        # the lattices at least 7 wide hold the whole radius-3 ball of offsets
-        # (full boxes: the z-marching SpMV); the stencil flux sums in FP32 in mixed precision
-        fs = (1 if prec == nn.FP64 else 2) * int(stencil)
+        # the lattices at least 7 wide hold the whole radius-3 ball of offsets (full boxes:
+        # the z-marching SpMV and, in FP64, the z-marching flux); the stencil flux sums in FP32
+        # in mixed precision
+        fs = (3 if prec == nn.FP64 else 2) if stencil else 0
         assert st["flux_stencil"] == fs and st["pot_stencil"] == 4 * int(stencil), (shape, prec)
         assert np.array_equal(ph, ref["phase"])
         tol = 1e-12 if prec == nn.FP64 else 1e-6

@@ -231,7 +231,7 @@ def test_jacobian_and_implicit_steps_stencil_form():
         pb.set_neighbors(rp, col)
         pb.set_bc(case.bctypes, case.bcvalues)
         pb.init_state(case.icvalues)
-        assert pb.step(1)["flux_stencil"] == 1
+        assert pb.step(1)["flux_stencil"] == 3  # (the explicit step: the z-marching form)
     ka, da, Ua, pa = res[nn.FORMAT_NO_ZMARCH]
     # the potential SpMV's table-driven stencil loop; the column forms
     for other in (nn.FORMAT_NO_BALL, nn.FORMAT_NO_STENCIL):

@@ -326,7 +326,9 @@ def test_storage_formats_bit_identical(kind):
                                                                                     else 4))), fmt
             # the explicit flux takes the stencil form wherever the operator does (FP32 bond
             # sums in mixed precision)
-            assert st["flux_stencil"] == (0 if not stencil else (1 if prec == nn.FP64 else 2)), fmt
+            # (FP64 on the full box lattice: the z-marching flux)
+            assert st["flux_stencil"] == (0 if not stencil else (2 if prec == nn.MIXED
+                                                                   else (3 if zm else 1))), fmt
             if stencil:
                 assert st["pot_col_bytes"] == 0 and st["pot_weight_bytes"] == 0
             else:
