@@ -540,25 +540,26 @@ def main():
         bytes32 = nll * (cb + wb32) + n * (4 + 3 * 4)
         bytes64 = nll * (cb + wb64) + n * (4 + 3 * 8)
     S = case.S
-    # flux bytes per launch: column form, per bond a column and a weight; stencil form (lattices;
-    # FP32 bond sums in the mixed run), a 16-byte bond mask per row; both: the fields read once,
-    # phase, Dirichlet mask and the updated fields
-    fform = st_["flux_stencil"]
-    fstencil = bool(fform)
-    fbytes = lambda wb: ((n * 16 if fstencil else nnz * (4 + wb))  # noqa: E731
-                         + n * (8 * (S + 1) + 1 + 2 + 8 * S))
+    # flux bytes per launch: column form, per bond a column and a weight; tile stencil form
+    # (lattices), a 16-byte bond mask per row; z-marching form (full box lattices), nothing per
+    # bond or row; all: the fields read once, phase, Dirichlet mask and the updated fields
+    fform = st_["flux_stencil"]  # the mixed run's form
+    fform64 = fp64["stats"]["flux_stencil"]  # the FP64 run's form
+    fbytes = lambda form, wb: ((0 if form == 3 else n * 16 if form else nnz * (4 + wb))  # noqa: E731
+                               + n * (8 * (S + 1) + 1 + 2 + 8 * S))
     t_cgit = pbm.time_kernel(nn._abi.KERNEL_CG_ITER, nn.MIXED, reps) if zmarch else None
     kern = {
         "pot_spmv_fp32": {"ms": t_spmv32, "alg_GBps": alg_bytes / (t_spmv32 * 1e-3) / 1e9,
                           "format_GBps": bytes32 / (t_spmv32 * 1e-3) / 1e9},
         "pot_spmv_fp64": {"ms": t_spmv64, "alg_GBps": alg_bytes / (t_spmv64 * 1e-3) / 1e9,
                           "format_GBps": bytes64 / (t_spmv64 * 1e-3) / 1e9},
-        "flux_fp64": {"ms": t_flux64, "form": "stencil" if fstencil else "columns",
-                      "GBps": fbytes(8) / (t_flux64 * 1e-3) / 1e9},
+        "flux_fp64": {"ms": t_flux64,
+                      "form": {3: "z-marching", 1: "stencil"}.get(fform64, "columns"),
+                      "GBps": fbytes(fform64, 8) / (t_flux64 * 1e-3) / 1e9},
         "flux_mixed": {"ms": t_flux32,
                        "form": {2: "stencil (fp32 bond sums)", 1: "stencil (fp64)"}.get(
                            fform, "columns (fp32)"),
-                       "GBps": fbytes(4) / (t_flux32 * 1e-3) / 1e9},
+                       "GBps": fbytes(fform, 4) / (t_flux32 * 1e-3) / 1e9},
     }
     if zmarch:
         kern["cg_iter_fp32"] = {"ms": t_cgit, "GBps": bytes_cgit / (t_cgit * 1e-3) / 1e9,

@@ -334,7 +334,8 @@ int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol
  * per-node volumes): one 128-bit mask per row; otherwise 16-bit column steps unless a row
  * needs a larger step, and a 1-byte weight-class dictionary when the operator has at most 255
  * distinct nonzero weights. The flags force 32-bit columns, plain FP64/FP32 weights and/or the
- * column form. Every format gives bit-identical results; they differ in bytes read per bond. No reference counterpart (the reference stores an
+ * column form. Every format gives bit-identical operator rows (the z-marching explicit flux of a
+ * full box lattice sums in another order, within 1e-12); they differ in bytes read per bond. No reference counterpart (the reference stores an
  * Eigen::SparseMatrix, utils.h:1794-1848). Takes effect at the next operator build. */
 #define NNPP_FORMAT_AUTO 0
 #define NNPP_FORMAT_WIDE_COLUMNS 1
@@ -345,7 +346,9 @@ int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol
 #define NNPP_FORMAT_NO_ZMARCH 16 /* lattice-ball stencil on a full box lattice: the tile SpMV
                                     (one 256-row tile per LDS window) instead of the z-marching
                                     one (column tiles walking z); FP64 rows identical, dot
-                                    products summed in another order */
+                                    products summed in another order. Also keeps the explicit
+                                    flux in its tile stencil form (flux_stencil 1, the column
+                                    form's sums) instead of the z-marching one (3) */
 int nnpp_set_format(nnpp_ctx* ctx, int32_t flags);
 
 /* Solve the potential for the charge of slot N: phi^{n+1} on free nodes (liquid, with a
