@@ -514,9 +514,9 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
   using Raw = typename Src::Raw;
   using Pre = typename Emit::Pre;
   __shared__ ZmPair<T> lds[2][ZM_PTS];
-  const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
-  const int zc = (int)(item % g.nzc);
-  const int64_t tile = item / g.nzc;
+  int64_t tile;
+  int zc;
+  zm_item(g, blockIdx.x, tile, zc);
   const int x0 = (int)(tile % g.tiles_x) * ZM_TX, y0 = (int)(tile / g.tiles_x) * ZM_TY;
   const int z0 = zc * g.lz;
   const int zend = min(z0 + g.lz, g.nz);
@@ -709,9 +709,9 @@ __device__ __forceinline__ void zm_walk_ring(const ZmGeom& g, const nnpp_ball_ar
   constexpr int LW = VEC ? ZM_W4 : ZM_W;  // staged row width in LDS
   constexpr int LX = VEC ? ZM_X4 : ZM_M;  // the thread's column within the staged row
   __shared__ __align__(16) float ldsv[2][VEC ? LW * ZM_H : ZM_PTS];
-  const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
-  const int zc = (int)(item % g.nzc);
-  const int64_t tile = item / g.nzc;
+  int64_t tile;
+  int zc;
+  zm_item(g, blockIdx.x, tile, zc);
   const int x0 = (int)(tile % g.tiles_x) * ZM_TX, y0 = (int)(tile / g.tiles_x) * ZM_TY;
   const int z0 = zc * g.lz;
   const int zend = min(z0 + g.lz, g.nz);

@@ -613,6 +613,17 @@ struct ZmGeom {
   int64_t items;     // tiles_x * tiles_y * nzc (the grid)
 };
 
+// Work item of a potential-operator block. The blocks of one XCD take a contiguous range of
+// items (nnpp_xcd_block) and the items run chunk-major, so each XCD walks whole z-slabs (every
+// column tile of its chunks): the tiles that stage one another's halo rows read the same planes
+// at the same time, through that XCD's L2.
+__device__ __forceinline__ void zm_item(const ZmGeom& g, int64_t b, int64_t& tile, int& zc) {
+  const int64_t item = nnpp_xcd_block(b, g.items);
+  const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y;
+  zc = (int)(item / tiles);
+  tile = item % tiles;
+}
+
 // Rings of the disc dx^2 + dy^2 <= M^2: the points of equal k = dx^2 + dy^2.
 template <int M>
 __device__ __forceinline__ constexpr bool zm_ring_first(int dx, int dy) {
