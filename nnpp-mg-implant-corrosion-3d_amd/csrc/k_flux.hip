@@ -1157,12 +1157,13 @@ hipError_t launch_st32(const FluxArgs& a, const FluxSt& st, const float* tf32, c
 // bonds); A and M are applied per plane. A row is updated for the species when it completes
 // (U_k^{n+1}, Dirichlet row, clamp: explicit_update for one field).
 //
-// Per plane and wave: when the wave's disc region holds nodes of one phase only and every row
-// of the slot has that phase, the slot takes every bond with the weight of its class, no tests
-// (positions outside the box are staged as c = phi = 0: their bonds add -w c_o to the diffusion
-// sum and 0 to the migration sum, and w_out c_o, with w_out the weight of the row's bonds to
-// outside the box in that plane, takes it back). Otherwise the slot's bonds take their class's
-// coefficient bond by bond (outside the box: none).
+// The planes are staged as (c_j, q_j = phi_j c_j): the migration term of a bond is
+// q_j - phi_o c_j. Per plane and wave: when the wave's disc region holds nodes of one phase only
+// and every row of the slot has that phase, the slot takes every bond with the weight of its
+// class, no tests, from plain ring sums of c and q (positions outside the box are staged as
+// c = q = 0 and add nothing; the row's weight sum counts the bonds inside the box only).
+// Otherwise the slot's bonds take their class's coefficient bond by bond (outside the box:
+// none).
 struct ZmFluxW {
   double w[NNPP_BALL_CLASSES];  // FP64 flux weight by class dx^2 + dy^2 + dz^2
   double wz[NNPP_BALL_M + 1];   // weight sum of the disc at plane distance |dz| (the center
@@ -1205,7 +1206,7 @@ __device__ __forceinline__ void zm_update_one(const FluxArgs& a, int k, uint32_t
 template <int S>
 __global__ __launch_bounds__(NNPP_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
-  __shared__ double2 lcp[2][ZM_PTS];   // (c_j, phi_j); 0 outside the box
+  __shared__ double2 lcp[2][ZM_PTS];   // (c_j, phi_j c_j); 0 outside the box
   __shared__ uint8_t lcl[2][ZM_PTS];   // phase, 2 outside the box
   const int64_t n = a.n;
   const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
@@ -1320,7 +1321,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
       for (int e = 0; e < ZM_STAGE; e++) {
         const int pt = threadIdx.x + e * NNPP_BLOCK;
         if (pt < ZM_PTS) {
-          lcp[buf][pt] = make_double2(sc[e], sp[e]);
+          lcp[buf][pt] = make_double2(sc[e], sp[e] * sc[e]);
           lcl[buf][pt] = sq[e];
         }
       }
@@ -1356,18 +1357,20 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
         fast[q] = act[q] && single && __all(!col_ok || sb == cstar);
       }
       {
-        // one phase: the disc by rings of equal k = dx^2 + dy^2 around the column (c_c, phi_c):
-        // RD_k = sum (c_j - c_c), RQ_k = sum (phi_j - phi_c) c_j, and a slot at plane distance
-        // dz takes
-        //   sum w (c_j - c_o)          = G + W_dz (c_c - c_o),          G = sum_k w RD_k
-        //   sum w (phi_j - phi_o) c_j  = H + (phi_c - phi_o)(G + W_dz c_c),  H = sum_k w RQ_k
-        // (positions outside the box are staged as 0: RD takes -c_c there, RQ 0)
+        // one phase: the disc by rings of equal k = dx^2 + dy^2 around the column:
+        // SC_k = sum c_j, SQ_k = sum q_j (q = phi c), and a slot at plane distance dz, with
+        // Tc = sum_k w SC_k + w(dz^2) c_c and Tq alike (the column's own point for dz != 0),
+        // takes
+        //   sum w (c_j - c_o)          = Tc - c_o (W_dz - w_out)
+        //   sum w (phi_j - phi_o) c_j  = Tq - phi_o Tc
+        // (positions outside the box are staged as 0 and add nothing to Tc and Tq; W_dz - w_out
+        // is the weight of the row's bonds inside the box in that plane)
         bool anyfast = false;
 #pragma unroll
         for (int q = 0; q < ZM_SL; q++) anyfast = anyfast || fast[q];
         if (anyfast) {
           const double2 cen = lcp[buf][lb0 + ZM_M * ZM_W];
-          double RD[ZM_M * ZM_M + 1], RQ[ZM_M * ZM_M + 1];
+          double SC[ZM_M * ZM_M + 1], SQ[ZM_M * ZM_M + 1];
 #pragma unroll
           for (int dy = -ZM_M; dy <= ZM_M; dy++) {
             // one disc line at a time: the scheduler does not batch the whole disc's reads
@@ -1379,11 +1382,9 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
               const int kk = dx * dx + dy * dy;
               if (kk == 0 || kk > ZM_M * ZM_M) continue;
               const double2 v = line[dx];
-              const double d = v.x - cen.x;
-              const double qv = (v.y - cen.y) * v.x;
               const bool first = zm_ring_first<ZM_M>(dx, dy);
-              RD[kk] = first ? d : RD[kk] + d;
-              RQ[kk] = first ? qv : RQ[kk] + qv;
+              SC[kk] = first ? v.x : SC[kk] + v.x;
+              SQ[kk] = first ? v.y : SQ[kk] + v.y;
             }
           }
           __builtin_amdgcn_sched_barrier(0);
@@ -1397,19 +1398,16 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
             for (int kk = 1; kk <= ZM_M * ZM_M; kk++) {
               if (!zm_ring_exists<ZM_M>(kk) || kk + az * az > ZM_M * ZM_M) continue;
               const double w = fw.w[kk + az * az];
-              G = first ? w * RD[kk] : __fma_rn(w, RD[kk], G);
-              H = first ? w * RQ[kk] : __fma_rn(w, RQ[kk], H);
+              G = first ? w * SC[kk] : __fma_rn(w, SC[kk], G);
+              H = first ? w * SQ[kk] : __fma_rn(w, SQ[kk], H);
               first = false;
             }
-            double sd, sm;
-            if (az == 0) {  // the column itself: c_c = c_o, phi_c = phi_o
-              sd = G;
-              sm = H;
-            } else {
-              sd = __fma_rn(fw.wz[az], cen.x - co[q], G);
-              sm = __fma_rn(cen.y - pho[q], __fma_rn(fw.wz[az], cen.x, G), H);
+            if (az != 0) {  // the column's own point
+              G = __fma_rn(fw.w[az * az], cen.x, G);
+              H = __fma_rn(fw.w[az * az], cen.y, H);
             }
-            sd = __fma_rn(co[q], wout[az], sd);  // the bonds outside the box back out
+            const double sd = __fma_rn(-co[q], fw.wz[az] - wout[az], G);
+            const double sm = __fma_rn(-pho[q], G, H);
             acc[q] = __fma_rn(A_same[cstar], sd, acc[q]);
             acc[q] = __fma_rn(M_same[cstar], sm, acc[q]);
           }
@@ -1444,11 +1442,11 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
             const double w = fw.w[d2];
             const double ws = cj == po ? w : 0.0, wx = cj == (po ^ 1u) ? w : 0.0;
             anyx = anyx || cj == (po ^ 1u);
-            const double dc = v.x - co[q], dp = v.y - pho[q];
+            const double dc = v.x - co[q], dq = __fma_rn(-pho[q], v.x, v.y);  // (phi_j - phi_o) c_j
             ds = __fma_rn(ws, dc, ds);
-            ms = __fma_rn(ws * dp, v.x, ms);
+            ms = __fma_rn(ws, dq, ms);
             dxs = __fma_rn(wx, dc, dxs);
-            mxs = __fma_rn(wx * dp, v.x, mxs);
+            mxs = __fma_rn(wx, dq, mxs);
           }
         }
         acc[q] = __fma_rn(A_same[po], ds, acc[q]);
