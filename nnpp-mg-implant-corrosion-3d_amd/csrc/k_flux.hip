@@ -1164,6 +1164,9 @@ hipError_t launch_st32(const FluxArgs& a, const FluxSt& st, const float* tf32, c
 // c = q = 0 and add nothing; the row's weight sum counts the bonds inside the box only).
 // Otherwise the slot's bonds take their class's coefficient bond by bond (outside the box:
 // none).
+// output planes per block at most (the phases of every input plane stay in LDS)
+constexpr int ZM_FLZ = 16;
+
 struct ZmFluxW {
   double w[NNPP_BALL_CLASSES];  // FP64 flux weight by class dx^2 + dy^2 + dz^2
   double wz[NNPP_BALL_M + 1];   // weight sum of the disc at plane distance |dz| (the center
@@ -1206,8 +1209,8 @@ __device__ __forceinline__ void zm_update_one(const FluxArgs& a, int k, uint32_t
 template <int S>
 __global__ __launch_bounds__(NNPP_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
-  __shared__ double2 lcp[2][ZM_PTS];   // (c_j, phi_j c_j); 0 outside the box
-  __shared__ uint8_t lcl[2][ZM_PTS];   // phase, 2 outside the box
+  __shared__ double2 lcp[2][ZM_PTS];                  // (c_j, phi_j c_j); 0 outside the box
+  __shared__ uint8_t lcl[ZM_FLZ + 2 * ZM_M][ZM_PTS];  // phases of the input planes, 2 outside
   const int64_t n = a.n;
   const int64_t item = nnpp_xcd_block(blockIdx.x, g.items);
   const int zc = (int)(item % g.nzc);
@@ -1247,6 +1250,36 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
       }
     wout[az] = s;
   }
+  // the phases of every input plane, staged once for all species, and per plane whether the
+  // wave's disc region (staged lines 2 wv .. 2 wv + 7, 304 points) holds liquid / solid nodes
+  // (bit zi - zs of vl / vs; five class bytes per lane, then a vote)
+  for (int zi = zs; zi < ze; zi++) {
+    const uint32_t base = (uint32_t)zi * sz;
+#pragma unroll
+    for (int e = 0; e < ZM_STAGE; e++) {
+      const int pt = threadIdx.x + e * NNPP_BLOCK;
+      if (pt < ZM_PTS) lcl[zi - zs][pt] = soff[e] >= 0 ? a.phase[base + (uint32_t)soff[e]] : 2;
+    }
+  }
+  __syncthreads();
+  uint32_t vl = 0u, vs = 0u;
+  for (int zi = zs; zi < ze; zi++) {
+    bool hl = false, hs = false;
+    const uint8_t* __restrict__ rg = &lcl[zi - zs][2 * wv * ZM_W];
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < (8 * ZM_W + 63) / 64; r++) {
+      const int pt = lane + 64 * r;
+      if (pt < 8 * ZM_W) {
+        const uint32_t cq = rg[pt];
+        hl = hl || cq == 0u;
+        hs = hs || cq == 1u;
+      }
+    }
+    vl |= (__any(hl) ? 1u : 0u) << (zi - zs);
+    vs |= (__any(hs) ? 1u : 0u) << (zi - zs);
+  }
+  const int ctr = (ty + ZM_M) * ZM_W + tx + ZM_M;  // the thread's column in a staged plane
 
   int buf = 0;
 #pragma unroll 1
@@ -1262,7 +1295,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
         const uint32_t i = (uint32_t)zo * sz + col;
         c = Uk[i];
         p = phi[i];
-        solid = a.phase[i] != 0 ? 1u : 0u;
+        solid = lcl[zo - zs][ctr] != 0 ? 1u : 0u;
       } else {
         c = p = 0.0;
         solid = 0u;
@@ -1281,7 +1314,6 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
     row_load(zs + 1 + ZM_M, nc, np, nsb);
     // staged loads of the next plane
     double sc[ZM_STAGE], sp[ZM_STAGE];
-    uint8_t sq[ZM_STAGE];
     const auto fetch = [&](int zi) {
       const uint32_t base = (uint32_t)zi * sz;
 #pragma unroll
@@ -1290,7 +1322,6 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
         const uint32_t j = base + (uint32_t)(ok ? soff[e] : 0);
         sc[e] = ok ? Uk[j] : 0.0;
         sp[e] = ok ? phi[j] : 0.0;
-        sq[e] = ok ? a.phase[j] : (uint8_t)2;
       }
     };
     fetch(zs);
@@ -1320,30 +1351,11 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
 #pragma unroll
       for (int e = 0; e < ZM_STAGE; e++) {
         const int pt = threadIdx.x + e * NNPP_BLOCK;
-        if (pt < ZM_PTS) {
-          lcp[buf][pt] = make_double2(sc[e], sp[e] * sc[e]);
-          lcl[buf][pt] = sq[e];
-        }
+        if (pt < ZM_PTS) lcp[buf][pt] = make_double2(sc[e], sp[e] * sc[e]);
       }
       __syncthreads();
       if (zi + 1 < ze) fetch(zi + 1);
-      // the phases in the wave's disc region (staged lines 2 wv .. 2 wv + 7, 304 points): five
-      // class bytes per lane, then a vote
-      bool hl = false, hs = false;
-      {
-        const uint8_t* __restrict__ rg = &lcl[buf][2 * wv * ZM_W];
-        const int lane = threadIdx.x & 63;
-#pragma unroll
-        for (int r = 0; r < (8 * ZM_W + 63) / 64; r++) {
-          const int pt = lane + 64 * r;
-          if (pt < 8 * ZM_W) {
-            const uint32_t cq = rg[pt];
-            hl = hl || cq == 0u;
-            hs = hs || cq == 1u;
-          }
-        }
-      }
-      const bool hasL = __any(hl), hasS = __any(hs);
+      const bool hasL = (vl >> (zi - zs)) & 1u, hasS = (vs >> (zi - zs)) & 1u;
       const uint32_t cstar = hasS ? 1u : 0u;
       const bool single = !(hasL && hasS);
       // the thread's first disc line (dy = -M) in the staged plane
@@ -1431,7 +1443,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
           }
           __builtin_amdgcn_sched_barrier(0);  // one disc line at a time
           const double2* __restrict__ line = &lcp[buf][lb];
-          const uint8_t* __restrict__ linec = &lcl[buf][lb];
+          const uint8_t* __restrict__ linec = &lcl[zi - zs][lb];
           lb += ZM_W;
 #pragma unroll
           for (int dx = -ZM_M; dx <= ZM_M; dx++) {
@@ -1626,7 +1638,12 @@ int nnpp_flux_zmarch_active(nnpp_ctx* c) {
 
 static int launch_zmarch(nnpp_ctx* c, const FluxArgs& a) {
   const int S = c->p.n_species;
-  const ZmGeom g = nnpp_zm_geom(c->pst, 1024);
+  ZmGeom g = nnpp_zm_geom(c->pst, 1024);
+  if (g.lz > ZM_FLZ) {  // (no reduction partials here: the grid may grow)
+    g.lz = ZM_FLZ;
+    g.nzc = (g.nz + ZM_FLZ - 1) / ZM_FLZ;
+    g.items = (int64_t)g.tiles_x * g.tiles_y * g.nzc;
+  }
   ZmFluxW fw;
   for (int d = 0; d < NNPP_BALL_CLASSES; d++) fw.w[d] = c->pst.fball_w64[d];
   constexpr int M = NNPP_BALL_M;
