@@ -1205,9 +1205,9 @@ __device__ __forceinline__ void zm_update_one(const FluxArgs& a, int k, uint32_t
   }
 }
 
-// (held at 2 waves per SIMD: the register demand of the rows in flight, FP64)
+// (held at 3 waves per SIMD: the register demand of the rows in flight, FP64)
 template <int S>
-__global__ __launch_bounds__(NNPP_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+__global__ __launch_bounds__(NNPP_BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
   __shared__ double2 lcp[2][ZM_PTS];                  // (c_j, phi_j c_j); 0 outside the box
   __shared__ uint8_t lcl[ZM_FLZ + 2 * ZM_M][ZM_PTS];  // phases of the input planes, 2 outside
@@ -1234,8 +1234,9 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
   const uint32_t sz = (uint32_t)g.sz;
   const double* __restrict__ phi = a.Un + (int64_t)S * n;
   const auto in_chunk = [&](int zo) { return zo >= z0 && zo < zend; };
-  // weight of the row's bonds to in-plane positions outside the box, by plane distance |dz|
-  double wout[ZM_M + 1];
+  // weight of the row's bonds to in-plane positions inside the box, by plane distance |dz|
+  // (in LDS: the registers go to the rows in flight)
+  __shared__ double lwin[ZM_M + 1][NNPP_BLOCK];
 #pragma unroll
   for (int az = 0; az <= ZM_M; az++) {
     double s = 0.0;
@@ -1248,7 +1249,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
         const bool out = x + dx < 0 || x + dx >= g.nx || y + dy < 0 || y + dy >= g.ny;
         if (out) s += fw.w[d2];
       }
-    wout[az] = s;
+    lwin[az][threadIdx.x] = fw.wz[az] - s;
   }
   // the phases of every input plane, staged once for all species, and per plane whether the
   // wave's disc region (staged lines 2 wv .. 2 wv + 7, 304 points) holds liquid / solid nodes
@@ -1400,10 +1401,11 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
             }
           }
           __builtin_amdgcn_sched_barrier(0);
+          // the sums of one plane distance |dz|, shared by the slots dz = +-|dz|
 #pragma unroll
-          for (int q = 0; q < ZM_SL; q++) {
-            if (!fast[q]) continue;  // uniform over the wave
-            const int dz = ZM_M - q, az = dz < 0 ? -dz : dz;
+          for (int az = 0; az <= ZM_M; az++) {
+            const int qa = ZM_M - az, qb = ZM_M + az;  // the slots at dz = az, -az
+            if (!fast[qa] && !fast[qb]) continue;      // uniform over the wave
             double G = 0.0, H = 0.0;
             bool first = true;
 #pragma unroll
@@ -1418,10 +1420,16 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
               G = __fma_rn(fw.w[az * az], cen.x, G);
               H = __fma_rn(fw.w[az * az], cen.y, H);
             }
-            const double sd = __fma_rn(-co[q], fw.wz[az] - wout[az], G);
-            const double sm = __fma_rn(-pho[q], G, H);
-            acc[q] = __fma_rn(A_same[cstar], sd, acc[q]);
-            acc[q] = __fma_rn(M_same[cstar], sm, acc[q]);
+            const double win = lwin[az][threadIdx.x];
+#pragma unroll
+            for (int h = 0; h < (az ? 2 : 1); h++) {
+              const int q = h ? qb : qa;
+              if (!fast[q]) continue;  // uniform over the wave
+              const double sd = __fma_rn(-co[q], win, G);
+              const double sm = __fma_rn(-pho[q], G, H);
+              acc[q] = __fma_rn(A_same[cstar], sd, acc[q]);
+              acc[q] = __fma_rn(M_same[cstar], sm, acc[q]);
+            }
           }
         }
       }
