@@ -1158,12 +1158,12 @@ hipError_t launch_st32(const FluxArgs& a, const FluxSt& st, const float* tf32, c
 // (U_k^{n+1}, Dirichlet row, clamp: explicit_update for one field).
 //
 // The planes are staged as (c_j, q_j = phi_j c_j): the migration term of a bond is
-// q_j - phi_o c_j. Per plane and wave: when the wave's disc region holds nodes of one phase only
-// and every row of the slot has that phase, the slot takes every bond with the weight of its
-// class, no tests, from plain ring sums of c and q (positions outside the box are staged as
-// c = q = 0 and add nothing; the row's weight sum counts the bonds inside the box only).
-// Otherwise the slot's bonds take their class's coefficient bond by bond (outside the box:
-// none).
+// q_j - phi_o c_j. Per plane, every active slot takes all its bonds with the coefficients of the
+// row's own phase from plain ring sums of c and q (positions outside the box are staged as
+// c = q = 0 and add nothing; the row's weight sum counts the bonds inside the box only). Where
+// the wave's disc region holds both phases, or rows of the slot differ in phase, the slot then
+// takes the difference of the solid-liquid coefficients over its bonds to nodes of the other
+// phase, bond by bond (this also finds the interface rows).
 // output planes per block at most (the phases of every input plane stay in LDS)
 constexpr int ZM_FLZ = 16;
 
@@ -1370,18 +1370,18 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
         fast[q] = act[q] && single && __all(!col_ok || sb == cstar);
       }
       {
-        // one phase: the disc by rings of equal k = dx^2 + dy^2 around the column:
-        // SC_k = sum c_j, SQ_k = sum q_j (q = phi c), and a slot at plane distance dz, with
-        // Tc = sum_k w SC_k + w(dz^2) c_c and Tq alike (the column's own point for dz != 0),
-        // takes
+        // every active slot, all bonds with the coefficients of the row's own phase: the disc
+        // by rings of equal k = dx^2 + dy^2 around the column, SC_k = sum c_j, SQ_k = sum q_j
+        // (q = phi c), and a slot at plane distance dz, with Tc = sum_k w SC_k + w(dz^2) c_c and
+        // Tq alike (the column's own point for dz != 0), takes
         //   sum w (c_j - c_o)          = Tc - c_o (W_dz - w_out)
         //   sum w (phi_j - phi_o) c_j  = Tq - phi_o Tc
         // (positions outside the box are staged as 0 and add nothing to Tc and Tq; W_dz - w_out
         // is the weight of the row's bonds inside the box in that plane)
-        bool anyfast = false;
+        bool anyact = false;
 #pragma unroll
-        for (int q = 0; q < ZM_SL; q++) anyfast = anyfast || fast[q];
-        if (anyfast) {
+        for (int q = 0; q < ZM_SL; q++) anyact = anyact || act[q];
+        if (anyact) {
           const double2 cen = lcp[buf][lb0 + ZM_M * ZM_W];
           double SC[ZM_M * ZM_M + 1], SQ[ZM_M * ZM_M + 1];
 #pragma unroll
@@ -1405,7 +1405,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
 #pragma unroll
           for (int az = 0; az <= ZM_M; az++) {
             const int qa = ZM_M - az, qb = ZM_M + az;  // the slots at dz = az, -az
-            if (!fast[qa] && !fast[qb]) continue;      // uniform over the wave
+            if (!act[qa] && !act[qb]) continue;        // uniform over the wave
             double G = 0.0, H = 0.0;
             bool first = true;
 #pragma unroll
@@ -1424,22 +1424,24 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
 #pragma unroll
             for (int h = 0; h < (az ? 2 : 1); h++) {
               const int q = h ? qb : qa;
-              if (!fast[q]) continue;  // uniform over the wave
+              if (!act[q]) continue;  // uniform over the wave
+              const bool po = (pos >> q) & 1u;
               const double sd = __fma_rn(-co[q], win, G);
               const double sm = __fma_rn(-pho[q], G, H);
-              acc[q] = __fma_rn(A_same[cstar], sd, acc[q]);
-              acc[q] = __fma_rn(M_same[cstar], sm, acc[q]);
+              acc[q] = __fma_rn(po ? A_same[1] : A_same[0], sd, acc[q]);
+              acc[q] = __fma_rn(po ? M_same[1] : M_same[0], sm, acc[q]);
             }
           }
         }
       }
-      // the other active slots: bond by bond, the coefficient of each bond's phase pair
+      // the slots whose wave region holds both phases: the bonds to nodes of the other phase
+      // take the solid-liquid coefficients instead, a correction summed bond by bond
 #pragma unroll
       for (int q = 0; q < ZM_SL; q++) {
         if (!act[q] || fast[q]) continue;  // uniform over the wave
         const int dz = ZM_M - q;
         const uint32_t po = (pos >> q) & 1u;
-        double ds = 0.0, ms = 0.0, dxs = 0.0, mxs = 0.0;  // same-phase and solid-liquid sums
+        double dxs = 0.0, mxs = 0.0;  // the bonds to the other phase
         bool anyx = false;
         int lb = lb0;
         __builtin_amdgcn_sched_barrier(0);  // one slot at a time
@@ -1459,20 +1461,16 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
             if (d2 > ZM_M * ZM_M || d2 == 0) continue;
             const double2 v = line[dx];
             const uint32_t cj = linec[dx];
-            const double w = fw.w[d2];
-            const double ws = cj == po ? w : 0.0, wx = cj == (po ^ 1u) ? w : 0.0;
-            anyx = anyx || cj == (po ^ 1u);
+            const bool x = cj == (po ^ 1u);
+            const double wx = x ? fw.w[d2] : 0.0;
+            anyx = anyx || x;
             const double dc = v.x - co[q], dq = __fma_rn(-pho[q], v.x, v.y);  // (phi_j - phi_o) c_j
-            ds = __fma_rn(ws, dc, ds);
-            ms = __fma_rn(ws, dq, ms);
             dxs = __fma_rn(wx, dc, dxs);
             mxs = __fma_rn(wx, dq, mxs);
           }
         }
-        acc[q] = __fma_rn(A_same[po], ds, acc[q]);
-        acc[q] = __fma_rn(M_same[po], ms, acc[q]);
-        acc[q] = __fma_rn(A_x, dxs, acc[q]);
-        acc[q] = __fma_rn(M_x, mxs, acc[q]);
+        acc[q] = __fma_rn(A_x - A_same[po], dxs, acc[q]);
+        acc[q] = __fma_rn(M_x - M_same[po], mxs, acc[q]);
         cross |= (anyx ? 1u : 0u) << q;
       }
       finish(zi);
