@@ -1280,6 +1280,8 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
     vl |= (__any(hl) ? 1u : 0u) << (zi - zs);
     vs |= (__any(hs) ? 1u : 0u) << (zi - zs);
   }
+  vl = __builtin_amdgcn_readfirstlane(vl);  // (uniform over the wave)
+  vs = __builtin_amdgcn_readfirstlane(vs);
   const int ctr = (ty + ZM_M) * ZM_W + tx + ZM_M;  // the thread's column in a staged plane
 
   int buf = 0;
@@ -1362,13 +1364,19 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
       // the thread's first disc line (dy = -M) in the staged plane
       const int lb0 = ty * ZM_W + tx + ZM_M;
       // per slot: active (row in the chunk), fast (one phase: every bond, no tests)
-      bool act[ZM_SL], fast[ZM_SL];
+      // (bit q of actm / fastm; scalar masks, uniform over the wave)
+      uint32_t actm = 0u, fastm = 0u;
 #pragma unroll
       for (int q = 0; q < ZM_SL; q++) {
-        act[q] = in_chunk(zi - ZM_M + q);
+        const bool act = in_chunk(zi - ZM_M + q);
         const uint32_t sb = (pos >> q) & 1u;
-        fast[q] = act[q] && single && __all(!col_ok || sb == cstar);
+        actm |= (act ? 1u : 0u) << q;
+        fastm |= (act && single && __all(!col_ok || sb == cstar) ? 1u : 0u) << q;
       }
+      actm = __builtin_amdgcn_readfirstlane(actm);
+      fastm = __builtin_amdgcn_readfirstlane(fastm);
+      const auto act = [&](int q) { return ((actm >> q) & 1u) != 0u; };
+      const auto fast = [&](int q) { return ((fastm >> q) & 1u) != 0u; };
       {
         // every active slot, all bonds with the coefficients of the row's own phase: the disc
         // by rings of equal k = dx^2 + dy^2 around the column, SC_k = sum c_j, SQ_k = sum q_j
@@ -1378,10 +1386,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
         //   sum w (phi_j - phi_o) c_j  = Tq - phi_o Tc
         // (positions outside the box are staged as 0 and add nothing to Tc and Tq; W_dz - w_out
         // is the weight of the row's bonds inside the box in that plane)
-        bool anyact = false;
-#pragma unroll
-        for (int q = 0; q < ZM_SL; q++) anyact = anyact || act[q];
-        if (anyact) {
+        if (actm != 0u) {
           const double2 cen = lcp[buf][lb0 + ZM_M * ZM_W];
           double SC[ZM_M * ZM_M + 1], SQ[ZM_M * ZM_M + 1];
 #pragma unroll
@@ -1405,7 +1410,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
 #pragma unroll
           for (int az = 0; az <= ZM_M; az++) {
             const int qa = ZM_M - az, qb = ZM_M + az;  // the slots at dz = az, -az
-            if (!act[qa] && !act[qb]) continue;        // uniform over the wave
+            if (!act(qa) && !act(qb)) continue;        // uniform over the wave
             double G = 0.0, H = 0.0;
             bool first = true;
 #pragma unroll
@@ -1424,7 +1429,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
 #pragma unroll
             for (int h = 0; h < (az ? 2 : 1); h++) {
               const int q = h ? qb : qa;
-              if (!act[q]) continue;  // uniform over the wave
+              if (!act(q)) continue;  // uniform over the wave
               const bool po = (pos >> q) & 1u;
               const double sd = __fma_rn(-co[q], win, G);
               const double sm = __fma_rn(-pho[q], G, H);
@@ -1438,7 +1443,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
       // take the solid-liquid coefficients instead, a correction summed bond by bond
 #pragma unroll
       for (int q = 0; q < ZM_SL; q++) {
-        if (!act[q] || fast[q]) continue;  // uniform over the wave
+        if (!act(q) || fast(q)) continue;  // uniform over the wave
         const int dz = ZM_M - q;
         const uint32_t po = (pos >> q) & 1u;
         double dxs = 0.0, mxs = 0.0;  // the bonds to the other phase
