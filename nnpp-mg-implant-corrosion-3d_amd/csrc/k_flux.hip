@@ -1487,8 +1487,24 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
   }
 }
 
+// Work items: lz chosen so that every block is resident at once (one round: a second, partly
+// filled round would run at a fraction of the occupancy), 4 <= lz <= ZM_FLZ.
 template <int S>
-hipError_t launch_zm(const FluxArgs& a, const ZmGeom& g, const ZmFluxW& fw, hipStream_t s) {
+hipError_t launch_zm(const FluxArgs& a, ZmGeom g, const ZmFluxW& fw, hipStream_t s,
+                     int device) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zm_flux<S>, NNPP_BLOCK, 0) ==
+          hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) ==
+          hipSuccess &&
+      per_cu > 0 && cus > 0) {
+    const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y, cap = (int64_t)per_cu * cus;
+    int64_t lz = (g.nz * tiles + cap - 1) / cap;
+    lz = std::min<int64_t>(std::max<int64_t>(lz, 4), std::min(ZM_FLZ, g.nz));
+    g.lz = (int)lz;
+    g.nzc = (int)((g.nz + lz - 1) / lz);
+    g.items = tiles * g.nzc;
+  }
   k_zm_flux<S><<<g.items, NNPP_BLOCK, 0, s>>>(a, g, fw);
   return hipGetLastError();
 }
@@ -1670,7 +1686,7 @@ static int launch_zmarch(nnpp_ctx* c, const FluxArgs& a) {
   hipError_t e = hipSuccess;
   switch (S) {
 #define NNPP_CASE(SS) \
-  case SS: e = launch_zm<SS>(a, g, fw, c->stream); break;
+  case SS: e = launch_zm<SS>(a, g, fw, c->stream, c->device); break;
     NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
     NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
     NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)
