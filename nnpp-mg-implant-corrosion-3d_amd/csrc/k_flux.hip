@@ -1205,8 +1205,19 @@ __device__ __forceinline__ void zm_update_one(const FluxArgs& a, int k, uint32_t
   }
 }
 
+// The Jacobian apply of the implicit step (frozen Jacobian, lagged phi^n; as the stencil and
+// column kernels' JAC mode, with d0 given): out_k = v_k on Dirichlet rows, else d0_k v_k - acc_k,
+// the bond sums of v in place of U^n.
+__device__ __forceinline__ void zm_jac_one(const FluxArgs& a, int k, uint32_t i, double v,
+                                           double acc) {
+  const int64_t j = (int64_t)k * a.n + i;
+  const bool dir = (a.bcmask[i] >> k) & 1u;
+  a.out[j] = dir ? v : __fma_rn(a.d0[j], v, -acc);
+}
+
+// JAC: the Jacobian apply to v = a.U1 (species-major) instead of the explicit update.
 // (held at 3 waves per SIMD: the register demand of the rows in flight, FP64)
-template <int S>
+template <int S, bool JAC>
 __global__ __launch_bounds__(NNPP_BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
   __shared__ double2 lcp[2][ZM_PTS];                  // (c_j, phi_j c_j); 0 outside the box
@@ -1287,7 +1298,7 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
   int buf = 0;
 #pragma unroll 1
   for (int k = 0; k < S; k++) {
-    const double* __restrict__ Uk = a.Un + (int64_t)k * n;
+    const double* __restrict__ Uk = (JAC ? a.U1 : a.Un) + (int64_t)k * n;
     const double A_same[2] = {a.A_LL[k], a.A_SS[k]}, M_same[2] = {a.M_LL[k], a.M_SS[k]};
     const double A_x = a.A_SL[k], M_x = a.M_SL[k];
     // rows in flight: slot q holds output plane zi - M + q while input plane zi is processed
@@ -1331,9 +1342,13 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
     const auto finish = [&](int zi) {
       const int zo = zi - ZM_M;
       if (col_ok && in_chunk(zo)) {
-        const int pi = (int)(pos & 1u);
-        const int intf = (pi == 1 && (cross & 1u)) ? 1 : 0;  // utils.h:82-89
-        zm_update_one<S>(a, k, (uint32_t)zo * sz + col, pi, intf, co[0], acc[0]);
+        if constexpr (JAC) {
+          zm_jac_one(a, k, (uint32_t)zo * sz + col, co[0], acc[0]);
+        } else {
+          const int pi = (int)(pos & 1u);
+          const int intf = (pi == 1 && (cross & 1u)) ? 1 : 0;  // utils.h:82-89
+          zm_update_one<S>(a, k, (uint32_t)zo * sz + col, pi, intf, co[0], acc[0]);
+        }
       }
 #pragma unroll
       for (int q = 0; q + 1 < ZM_SL; q++) {
@@ -1489,11 +1504,11 @@ k_zm_flux(const FluxArgs a, const ZmGeom g, const ZmFluxW fw) {
 
 // Work items: lz chosen so that every block is resident at once (one round: a second, partly
 // filled round would run at a fraction of the occupancy), 4 <= lz <= ZM_FLZ.
-template <int S>
+template <int S, bool JAC>
 hipError_t launch_zm(const FluxArgs& a, ZmGeom g, const ZmFluxW& fw, hipStream_t s,
                      int device) {
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zm_flux<S>, NNPP_BLOCK, 0) ==
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zm_flux<S, JAC>, NNPP_BLOCK, 0) ==
           hipSuccess &&
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) ==
           hipSuccess &&
@@ -1505,7 +1520,7 @@ hipError_t launch_zm(const FluxArgs& a, ZmGeom g, const ZmFluxW& fw, hipStream_t
     g.nzc = (int)((g.nz + lz - 1) / lz);
     g.items = tiles * g.nzc;
   }
-  k_zm_flux<S><<<g.items, NNPP_BLOCK, 0, s>>>(a, g, fw);
+  k_zm_flux<S, JAC><<<g.items, NNPP_BLOCK, 0, s>>>(a, g, fw);
   return hipGetLastError();
 }
 
@@ -1663,7 +1678,7 @@ int nnpp_flux_zmarch_active(nnpp_ctx* c) {
   return (T.box && T.fball && T.ball == NNPP_BALL_M) ? 1 : 0;
 }
 
-static int launch_zmarch(nnpp_ctx* c, const FluxArgs& a) {
+static int launch_zmarch(nnpp_ctx* c, const FluxArgs& a, bool jac = false) {
   const int S = c->p.n_species;
   ZmGeom g = nnpp_zm_geom(c->pst, 1024);
   if (g.lz > ZM_FLZ) {  // (no reduction partials here: the grid may grow)
@@ -1686,7 +1701,10 @@ static int launch_zmarch(nnpp_ctx* c, const FluxArgs& a) {
   hipError_t e = hipSuccess;
   switch (S) {
 #define NNPP_CASE(SS) \
-  case SS: e = launch_zm<SS>(a, g, fw, c->stream, c->device); break;
+  case SS:                                                                   \
+    e = jac ? launch_zm<SS, true>(a, g, fw, c->stream, c->device)            \
+            : launch_zm<SS, false>(a, g, fw, c->stream, c->device);          \
+    break;
     NNPP_CASE(1) NNPP_CASE(2) NNPP_CASE(3) NNPP_CASE(4) NNPP_CASE(5) NNPP_CASE(6)
     NNPP_CASE(7) NNPP_CASE(8) NNPP_CASE(9) NNPP_CASE(10) NNPP_CASE(11) NNPP_CASE(12)
     NNPP_CASE(13) NNPP_CASE(14) NNPP_CASE(15) NNPP_CASE(16)
@@ -1733,7 +1751,9 @@ int nnpp_launch_jac(nnpp_ctx* c, double BV, const double* Un, const double* vin,
   a.d0_out = d0_out;
   a.diag_out = diag_out;
   if (!c->n) return NNPP_OK;
-  // lattice: the stencil form reads v species-major
+  // full box lattice, d0 given: the z-marching form; otherwise on lattices the stencil form
+  // (both read v species-major)
+  if (vin && d0 && !d0_out && nnpp_flux_zmarch_active(c)) return launch_zmarch(c, a, true);
   const size_t st_lds = vin ? flux_stencil_lds(c) : 0;
   if (st_lds) return launch_stencil<true>(c, a, st_lds);
   const int64_t nb = nnpp_nblocks(c->n);
