@@ -348,7 +348,8 @@ int nnpp_set_solver(nnpp_ctx* ctx, double rtol, int32_t maxit, double inner_rtol
                                     one (column tiles walking z); FP64 rows identical, dot
                                     products summed in another order. Also keeps the explicit
                                     flux in its tile stencil form (flux_stencil 1, the column
-                                    form's sums) instead of the z-marching one (3) */
+                                    form's sums) instead of the z-marching one (3), and the
+                                    implicit step's Jacobian apply alike */
 int nnpp_set_format(nnpp_ctx* ctx, int32_t flags);
 
 /* Solve the potential for the charge of slot N: phi^{n+1} on free nodes (liquid, with a
