@@ -138,6 +138,27 @@ def test_c2_full128_stencil_vs_column_forms(case128):
         assert rel_err(a[1][f], c[1][f]) < 1e-12, (f, rel_err(a[1][f], c[1][f]))
 
 
+def test_c2_full128_implicit_jacobian_forms(case128):
+    """128^3: one implicit step with the Jacobian apply in the z-marching form (the default on a
+    full box lattice) and in the tile stencil form (NNPP_FORMAT_NO_ZMARCH). Both solve the same
+    Newton systems with the bond sums added in another order, so with tight Newton and linear
+    tolerances the states agree far below what a wrong bond or row would leave."""
+    say("case C2 128^3: implicit step, z-marching against tile-form Jacobian apply")
+    tight = dict(newton_tol=1e-11, newton_maxit=100, lin_rtol=1e-12, lin_maxit=5000)
+    res = {}
+    for name, fmt in (("zmarch", nn.FORMAT_AUTO), ("tile", nn.FORMAT_NO_ZMARCH)):
+        with gpu_problem(case128, fmt=fmt) as pb:
+            st = pb.step_implicit(1, **tight)
+            res[name] = (pb.get_state(), pb.get_phase(), st)
+            say(name, "newton", st["newton_iterations"], "lin", st["lin_iterations"],
+                "cg", st["cg_iterations"])
+    (Uz, pz, sz), (Ut, pt, stt) = res["zmarch"], res["tile"]
+    assert sz["status"] == 0 and stt["status"] == 0
+    assert np.array_equal(pz, pt)
+    for f in range(12):
+        assert rel_err(Uz[f], Ut[f]) < 1e-9, (f, rel_err(Uz[f], Ut[f]))
+
+
 def test_c3_mixed_vs_fp64_1000_steps(case128):
     say("case C3 128^3 mixed vs FP64 after 1000 steps")
     case = case128
