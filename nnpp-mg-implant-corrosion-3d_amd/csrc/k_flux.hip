@@ -1147,23 +1147,24 @@ hipError_t launch_st32(const FluxArgs& a, const FluxSt& st, const float* tf32, c
 
 // ------------------------------------------------- z-marching form (full box lattices) -------
 // The explicit update on a full box lattice with the radius-3 ball of offsets and flux weights by
-// class dx^2 + dy^2 + dz^2 (FP64 sums). Blocks take the potential operator's z-marching work
-// items (a 32 x 8 column tile and a chunk of lz output planes, ZmGeom) and the species one at a
-// time: per species the block walks its input planes, staging each plane's tile with a 3-node
-// halo as (c_j, phi^n_j) pairs and classes (the phase, 2 outside the box) into LDS, and each row
-// of the thread's column in flight (up to 7) takes the plane's bonds,
-//   acc += A(p_o, p_j) sum w (c_j - c_o) + M(p_o, p_j) sum w (phi_j - phi_o) c_j,
+// class dx^2 + dy^2 + dz^2 (FP64 sums). Blocks take z-marching work items (a 32 x 8 column tile
+// and a chunk of lz output planes, ZmGeom; lz chosen by the launch) and the species one at a
+// time. The phases of every input plane of the chunk (2 outside the box) are staged into LDS
+// once per block; per species the block walks its input planes, staging each plane's tile with a
+// 3-node halo as pairs (c_j, q_j = phi^n_j c_j) into LDS, and each row of the thread's column in
+// flight (up to 7) takes the plane's bonds,
+//   acc += A(p_o, p_j) sum w (c_j - c_o) + M(p_o, p_j) sum w (q_j - phi_o c_j),
 // with the phase pair's coefficients (utils.h:100-159: D by phase pair, BV on solid-liquid
 // bonds); A and M are applied per plane. A row is updated for the species when it completes
 // (U_k^{n+1}, Dirichlet row, clamp: explicit_update for one field).
 //
-// The planes are staged as (c_j, q_j = phi_j c_j): the migration term of a bond is
-// q_j - phi_o c_j. Per plane, every active slot takes all its bonds with the coefficients of the
-// row's own phase from plain ring sums of c and q (positions outside the box are staged as
-// c = q = 0 and add nothing; the row's weight sum counts the bonds inside the box only). Where
-// the wave's disc region holds both phases, or rows of the slot differ in phase, the slot then
-// takes the difference of the solid-liquid coefficients over its bonds to nodes of the other
-// phase, bond by bond (this also finds the interface rows).
+// Per plane, every active slot takes all its bonds with the coefficients of the row's own phase
+// from plain ring sums of c and q (positions outside the box are staged as c = q = 0 and add
+// nothing; the row's weight sum counts the bonds inside the box only). Where the wave's disc
+// region holds both phases, or rows of the slot differ in phase, the slot then takes the
+// difference of the solid-liquid coefficients over its bonds to nodes of the other phase, bond by
+// bond (this also finds the interface rows).
+
 // output planes per block at most (the phases of every input plane stay in LDS)
 constexpr int ZM_FLZ = 16;
 
