@@ -7,12 +7,16 @@ fused 11-species flux + explicit update + potential solve to relative residual 1
 operator application (the flux counts it once for all species; each potential-operator
 application counts its liquid-liquid bonds).
 
-Reported value: bond-updates/s of the mixed-precision step (inputs resident in HBM). The FP64
-step is timed from the same state; the mixed/FP64 ratio and the per-field error of the mixed
-run against the FP64 run are reported beside it. `roofline` is for the dominant kernel (the
-potential-operator SpMV of the solve in the measured precision), timed with HIP events on the
-library's stream; `cpu_baseline` times the CPU oracle (OpenMP FP64 restatement of the
-reference loops) on the same workload for a bounded number of operator applications.
+Reported value: bond-updates/s of the mixed-precision step (inputs resident in HBM), over the
+timed steps after the warm-up steps (default 2; the potential's extrapolated warm start uses the
+two previous potentials and reaches its steady iteration count after a few steps, so a longer
+warm-up gives shorter steps with fewer CG iterations each). The FP64 step is timed from the same state; the mixed/FP64 ratio and
+the per-field error of the mixed run against the FP64 run after 1000 steps are reported beside
+it. `roofline` is for the dominant kernel (on the box lattice the fused FP32 CG iteration,
+k_zm_m_cgs), timed with HIP events on the library's stream, with the HBM traffic per launch from
+the counter summary in profiles/ when it was taken on the same sources; `cpu_baseline` times the
+reference's own utils::Residuals (compiled from its sources into oracle/_ref) once on the same
+lattice, or the oracle's port of it where that library is absent.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 128]
 """
