@@ -664,6 +664,16 @@ __device__ __forceinline__ void zm_walk(const ZmGeom& g, const nnpp_ball_args& b
 }
 
 
+// Emit::RawTag true: the handler takes the row's own operands (Src::Raw) from the walk's
+// staging instead of loading them (emit(i, y, p_i, pre, raw)).
+template <typename E, typename = void>
+struct ZmEmitRaw : std::false_type {};
+template <typename E>
+struct ZmEmitRaw<E, std::void_t<typename E::RawTag>> : E::RawTag {};
+// planes whose tile operands the walk keeps for the rows (a row completes 3 planes after its
+// own; a slot is written again 5 planes later, two barriers after its last read)
+constexpr int ZM_RR = 5;
+
 template <typename Src, bool VEC>
 struct ZmRaw4 {
   using type = typename Src::Raw;
@@ -698,7 +708,9 @@ struct ZmRaw4<Src, true> {
 //
 // VEC (nx a multiple of 4): the plane is staged in 16-byte quads (Src::load4 / value4), the
 // quads of the next two planes in flight; otherwise single values (Src::load / value), the
-// next plane's in flight.
+// next plane's in flight. With Emit::RawTag (VEC), the staged operand quads of the tile's own
+// columns also go to an LDS ring of ZM_RR planes, from which each row takes its own operands
+// when it completes: they are not loaded again (by then L2 has dropped most of them).
 template <bool VEC, typename Src, typename Emit>
 __device__ __forceinline__ void zm_walk_ring(const ZmGeom& g, const nnpp_ball_args& ball,
                                              const uint8_t* __restrict__ pfree,
@@ -709,6 +721,9 @@ __device__ __forceinline__ void zm_walk_ring(const ZmGeom& g, const nnpp_ball_ar
   constexpr int LW = VEC ? ZM_W4 : ZM_W;  // staged row width in LDS
   constexpr int LX = VEC ? ZM_X4 : ZM_M;  // the thread's column within the staged row
   __shared__ __align__(16) float ldsv[2][VEC ? LW * ZM_H : ZM_PTS];
+  constexpr bool RAWROW = VEC && ZmEmitRaw<Emit>::value;
+  // [plane slot][operand][tile column ty * ZM_TX + tx]: the rows' own operands
+  __shared__ __align__(16) float lraw[RAWROW ? ZM_RR : 1][4][RAWROW ? NNPP_BLOCK : 4];
   int64_t tile;
   int zc;
   zm_item(g, blockIdx.x, tile, zc);
@@ -806,7 +821,14 @@ __device__ __forceinline__ void zm_walk_ring(const ZmGeom& g, const nnpp_ball_ar
         line = __fmaf_rn(wz[a], t, line);
       }
       const float yv = f0 ? __fmaf_rn(d0, uo, rs[0] + line) : d0 * uo;
-      emit((int64_t)((uint32_t)zo * sz + col), yv, uo, pre0);
+      if constexpr (RAWROW) {
+        const int sl = (zo - zs) % ZM_RR;
+        const Raw raw{lraw[sl][0][threadIdx.x], lraw[sl][1][threadIdx.x],
+                      lraw[sl][2][threadIdx.x], lraw[sl][3][threadIdx.x]};
+        emit((int64_t)((uint32_t)zo * sz + col), yv, uo, pre0, raw);
+      } else {
+        emit((int64_t)((uint32_t)zo * sz + col), yv, uo, pre0);
+      }
     }
 #pragma unroll
     for (int q = 0; q + 1 < ZM_SL; q++) rs[q] = rs[q + 1];
@@ -822,8 +844,20 @@ __device__ __forceinline__ void zm_walk_ring(const ZmGeom& g, const nnpp_ball_ar
   // input plane zi; its staged loads are in cur, which then takes plane zi + 2 (VEC) or zi + 1
   const auto plane = [&](int zi, SvRaw (&cur)[NST]) {
     if constexpr (VEC) {
-      if (threadIdx.x < ZM_Q4)
+      if (threadIdx.x < ZM_Q4) {
         reinterpret_cast<float4*>(&ldsv[buf][0])[threadIdx.x] = src.value4(cur[0]);
+        if constexpr (RAWROW) {
+          // quads 1 .. ZM_TX / 4 of staged lines M .. M + ZM_TY - 1: the tile's own columns
+          const int qx = threadIdx.x % (ZM_W4 / 4), qy = threadIdx.x / (ZM_W4 / 4);
+          if (qx >= 1 && qx <= ZM_TX / 4 && qy >= ZM_M && qy < ZM_M + ZM_TY) {
+            const int o = (qy - ZM_M) * ZM_TX + (qx - 1) * 4, sl = (zi - zs) % ZM_RR;
+            *reinterpret_cast<float4*>(&lraw[sl][0][o]) = cur[0].r;
+            *reinterpret_cast<float4*>(&lraw[sl][1][o]) = cur[0].w;
+            *reinterpret_cast<float4*>(&lraw[sl][2][o]) = cur[0].s;
+            *reinterpret_cast<float4*>(&lraw[sl][3][o]) = cur[0].d;
+          }
+        }
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < NST; k++) {
@@ -1133,7 +1167,10 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_arg
   } src{vec.r_in, vec.w_in, vec.s_in, vec.pinv, alpha, beta};
   // rows: the stored vectors' old values, loaded a plane ahead
   double v[3] = {0.0, 0.0, 0.0};
+  // (RawTag, 16-byte staging: r, w, s and D^-1 of the row from the walk's staging; RESTART
+  // loads them, its staged w and s being zero)
   struct Em {
+    using RawTag = std::bool_constant<!RESTART>;
     struct Pre {
       typename Src::Raw a;
       float p, e;
@@ -1143,20 +1180,23 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_arg
     double (&v)[3];
     __device__ __forceinline__ Pre pre(uint32_t i) const {
       Pre q;
-      q.a = src.load(i);
-      q.a.s = zm_ld(vec.s_in, i);  // restart: s is copied
+      if (!(RawTag::value && VEC)) {
+        q.a = src.load(i);
+        q.a.s = zm_ld(vec.s_in, i);  // restart: s is copied
+      }
       q.p = RESTART ? 0.0f : zm_ld((const float*)vec.p, i);
       q.e = RESTART ? 0.0f : zm_ld((const float*)vec.e, i);
       return q;
     }
-    __device__ __forceinline__ void operator()(int64_t i64, float y, float ui, const Pre& q) {
+    __device__ __forceinline__ void operator()(int64_t i64, float y, float ui, const Pre& q,
+                                               const typename Src::Raw& a) {
       const uint32_t i = (uint32_t)i64;
-      const float ri = src.r_new(q.a);
+      const float ri = src.r_new(a);
       if (RESTART) {
-        zm_st(vec.s_out, i, q.a.s);
+        zm_st(vec.s_out, i, a.s);
       } else {
-        const float si = __fmaf_rn(src.beta, q.a.s, q.a.w);
-        const float pi_ = __fmaf_rn(src.beta, q.p, q.a.d * q.a.r);
+        const float si = __fmaf_rn(src.beta, a.s, a.w);
+        const float pi_ = __fmaf_rn(src.beta, q.p, a.d * a.r);
         zm_st(vec.p, i, pi_);
         zm_st(vec.e, i, __fmaf_rn(src.alpha, pi_, q.e));
         zm_st(vec.s_out, i, si);
@@ -1166,6 +1206,9 @@ __global__ __launch_bounds__(NNPP_BLOCK) void k_zm_m_cgs(ZmGeom g, nnpp_ball_arg
       v[0] += (double)ri * (double)ui;
       v[1] += (double)ui * (double)y;
       v[2] += (double)ri * (double)ri;
+    }
+    __device__ __forceinline__ void operator()(int64_t i64, float y, float ui, const Pre& q) {
+      (*this)(i64, y, ui, q, q.a);
     }
   } em{src, vec, v};
   zm_walk_ring<VEC>(g, ball, pfree, dd, src, em);
